@@ -1,0 +1,296 @@
+/*
+ * dice_hip.h -- C ABI of the MI355X HIP backend for the portable Digital ICE
+ * processing core (LS-5000, selector 8, ICE Normal, metric 4000).
+ *
+ * The library (libdice_hip.so) is loaded by Python through ctypes.  Every
+ * entry point takes plain pointers and sizes; no torch or HIP types cross
+ * this boundary.  Host buffers are C-contiguous and owned by the caller; the
+ * library keeps no pointer after a call returns.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo
+ * rohanpandula/digital-fauxice):
+ *   dice_process_job          process_cuda(job, ...)
+ *                             src/portable_digital_ice/cuda_backend/process.py:25-159
+ *                             (same orchestration as engine.py:110-267)
+ *   dice_replay               run_streaming_replay_cuda(main_rgbi, output, ...)
+ *                             src/portable_digital_ice/cuda_backend/engine.py:183-590
+ *                             (semantics: streaming.py:365-552)
+ *   dice_producer_schedule    derive_producer_record_schedule_cuda(module, rgbi, lut)
+ *                             src/portable_digital_ice/cuda_backend/producer.py:38-246
+ *                             (semantics: producer_parameters.py:380-495)
+ *   dice_prepass              reduce_prepass_frame(frame, ...)
+ *                             src/portable_digital_ice/prepass.py:96-236
+ *   dice_stage_table_from_schedule
+ *                             ContentDerivedStageParameterProvider
+ *                             src/portable_digital_ice/producer_parameters.py:498-541
+ *                             + StageCalibration.from_setter_inputs
+ *                             src/portable_digital_ice/stage_parameters.py:93-197
+ *   dice_probe                cuda_device_summary()
+ *                             src/portable_digital_ice/cuda_backend/engine.py:77-100
+ *   dice_response_lut         generate_nikon_response_lut()  x3a.py:111-132
+ *   dice_inverse_factors      InverseResponseFactors.recovered_16bit()  output.py:39-58
+ *   dice_t_*                  the level-1 test wrappers appended to the CUDA
+ *                             kernel source in tests/test_cuda_level1_primitives.py:45-99
+ *                             and the level-1 kernels k_convert_and_auxiliary,
+ *                             k_score_and_weighted, k_decision_eligibility
+ *                             (cuda_backend/kernels.py:219-332)
+ *
+ * Return codes (every int-returning entry point):
+ *   0 DICE_OK
+ *   1 DICE_UNAVAILABLE       -> HipBackendUnavailable (no device, no HBM, ...)
+ *   2 DICE_INVALID           -> ValueError (message from dice_last_error)
+ *   3 DICE_NONFINITE_AUX     -> ValueError("auxiliary score input must be a finite HxW plane")
+ *   4 DICE_NONFINITE_WORK    -> ValueError("work RGB must be finite")
+ *   5 DICE_CANCELLED         -> ProcessingCancelled
+ *   6 DICE_OVER_BUDGET       -> HipBackendUnavailable (HBM budget)
+ *   7 DICE_INTERNAL          -> HipBackendUnavailable (HIP runtime error)
+ */
+#ifndef DICE_HIP_H
+#define DICE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DICE_ABI_VERSION 1
+
+enum dice_status {
+  DICE_OK = 0,
+  DICE_UNAVAILABLE = 1,
+  DICE_INVALID = 2,
+  DICE_NONFINITE_AUX = 3,
+  DICE_NONFINITE_WORK = 4,
+  DICE_CANCELLED = 5,
+  DICE_OVER_BUDGET = 6,
+  DICE_INTERNAL = 7
+};
+
+/* Stable (non-stage) replay parameters.  Every float is the binary32 value
+ * the reference quantizes the dataclass field to before use
+ * (x3a.py:193-286, reconstruction.py:614-670, dither.py:34-56,
+ * profile.py:108-173). */
+typedef struct dice_params {
+  int32_t visible_channel;      /* AuxiliaryParameters.selected_visible_channel */
+  float score_addend;           /* ScoreParameters.base_addend */
+  float score_scale;            /* ScoreParameters.scale */
+  float score_offset;           /* ScoreParameters.offset */
+  float score_floor;            /* ScoreParameters.floor */
+  int32_t horizontal_minimum;   /* cutoff < resolution_metric (x3a.py:266-268) */
+  float decision_threshold;     /* DecisionParameters.sample_threshold */
+  int32_t decision_count_limit; /* DecisionParameters.count_limit */
+  int32_t decision_radius;      /* DecisionParameters.perpendicular_radius */
+  int32_t cross_neighbor;       /* metric > cross_neighbor_cutoff (reconstruction.py:561-572) */
+  int32_t coarse_enabled;
+  float coarse_slopes[3];
+  int32_t band_enabled[3];
+  float band_scales[3];
+  float factors_a[9];           /* [band][channel] */
+  float factors_b[9];
+  float configured_strengths[3];
+  int32_t gate_primary;         /* driver_gate_primary */
+  float dither_scales[3];
+  float dither_low;             /* DitherBounds.low */
+  float dither_high;            /* DitherBounds.high */
+} dice_params;
+
+/* Per-stage calibration, index s-1 for scheduler stage s = 1..count.
+ * Row y consumes aux stage y+1, score stage y+2 and writer stage y+7
+ * (stage_parameters.py:207-232); the hidden startup consumes stages 1..6.
+ * A replay of H rows therefore needs count >= H + 6. */
+typedef struct dice_stage_table {
+  int64_t count;
+  const float* aux_alpha;
+  const float* aux_offset;
+  const float* aux_replacement;
+  const uint8_t* aux_has_replacement;
+  const float* score_base;
+  const float* coarse_reference;
+  const uint8_t* gate_secondary;
+  const int32_t* row_gate;
+} dice_stage_table;
+
+/* StreamingReplayResult (streaming.py:63-74) + StartupReplayResult
+ * (startup.py:37-43). */
+typedef struct dice_replay_result {
+  uint64_t attempted_pixels;
+  uint64_t written_pixels;
+  uint64_t public_rng_advances;
+  uint64_t changed_pixels;
+  uint32_t final_rng_state;
+  uint32_t startup_final_rng_state;
+  int64_t startup_attempted[6];
+  int64_t startup_advances[6];
+  char output_sha256[72]; /* lowercase hex, NUL-terminated */
+} dice_replay_result;
+
+/* ProcessingDiagnostics planes (engine.py:40-66); any pointer may be NULL. */
+typedef struct dice_diagnostics {
+  float* score_plane;    /* H*W float32 */
+  uint8_t* at_floor_mask; /* H*W bool */
+  uint8_t* changed_mask;  /* H*W bool */
+} dice_diagnostics;
+
+/* ProducerRecordSchedule (producer_parameters.py:84-129).  Every array has
+ * `rows` elements except payloads (rows * 64 bytes).  Caller-allocated. */
+typedef struct dice_schedule {
+  int64_t rows;
+  int64_t active_width;
+  double* weighted_visible_sum;
+  double* weighted_infrared_sum;
+  double* weight_sum;
+  float* mean_visible;
+  float* mean_infrared;
+  uint32_t* accepted;
+  float* scaled_input;
+  float* input_scale;
+  float* scale_denominator;
+  float* scale_numerator;
+  float* scale_add_denominator;
+  float* scale_add_numerator;
+  uint8_t* payloads;
+} dice_schedule;
+
+/* Prepass constants (prepass.py:42-53). */
+typedef struct dice_prepass_params {
+  float valid_ir_threshold;
+  float alpha_scale;
+  float ratio_minimum;
+  float ratio_maximum;
+  float zero_denominator_alpha;
+  uint32_t completion_marker;
+} dice_prepass_params;
+
+typedef struct dice_prepass_result {
+  uint8_t payload[64];
+  int64_t valid_pixels;
+  int64_t fully_valid_tiles;
+} dice_prepass_result;
+
+/* Progress / cancellation hooks, called between stages on the calling
+ * thread.  phase: 0 validating, 1 prepass, 2 producer, 3 reconstruction,
+ * 4 complete (engine.py:22-37).  cancelled() returning nonzero aborts with
+ * DICE_CANCELLED before any caller-owned output is written. */
+typedef void (*dice_progress_fn)(void* user, int32_t phase, int64_t completed,
+                                 int64_t total, int64_t attempted,
+                                 int64_t written);
+typedef int32_t (*dice_cancelled_fn)(void* user);
+typedef struct dice_callbacks {
+  dice_progress_fn progress;
+  dice_cancelled_fn cancelled;
+  void* user;
+} dice_callbacks;
+
+/* Stage timings in seconds (cuda_backend/engine.py:119-165 keys). */
+typedef struct dice_stage_timings {
+  double prepass;
+  double producer_device;
+  double upload;
+  double analysis_device;
+  double download_sites;
+  double writer_host;
+  double finish_device;
+  double download_output;
+  double sha256_host;
+  double total;
+} dice_stage_timings;
+
+typedef struct dice_device_info {
+  char name[128];
+  char arch[64];
+  int32_t compute_units;
+  int32_t device_count;
+  uint64_t total_hbm_bytes;
+  uint64_t free_hbm_bytes;
+  char build_flags[256];
+  char reason[256];
+} dice_device_info;
+
+typedef struct dice_ctx dice_ctx;
+
+/* Job flags for dice_process_job / dice_replay. */
+#define DICE_MAIN_ON_DEVICE 0x1u /* main RGBI pointer is device memory */
+#define DICE_OUT_ON_DEVICE 0x2u  /* output RGB16 pointer is device memory */
+#define DICE_SKIP_SHA256 0x4u    /* leave output_sha256 empty */
+
+int32_t dice_abi_version(void);
+const char* dice_last_error(void);
+
+int32_t dice_probe(dice_device_info* info);
+int32_t dice_ctx_create(int32_t device, uint64_t hbm_budget, dice_ctx** out);
+void dice_ctx_destroy(dice_ctx* ctx);
+
+int32_t dice_response_lut(float* lut65536);
+int32_t dice_inverse_factors(uint32_t* high256, uint32_t* low256);
+
+int32_t dice_prepass(const uint16_t* pixels, int64_t height, int64_t width,
+                     const float* lut, const dice_prepass_params* params,
+                     dice_prepass_result* out);
+
+int32_t dice_producer_schedule(dice_ctx* ctx, const uint16_t* rgbi,
+                               int64_t height, int64_t width,
+                               const float* lut, dice_schedule* out);
+
+/* Arrays of `count` stages derived from a producer schedule of `rows` rows;
+ * stage s reads record row min(s-1, rows-1). */
+int32_t dice_stage_table_from_schedule(
+    const dice_schedule* schedule, float auxiliary_factor_b,
+    int64_t count, float* aux_alpha, float* aux_offset, float* aux_replacement,
+    uint8_t* aux_has_replacement, float* score_base, float* coarse_reference,
+    uint8_t* gate_secondary, int32_t* row_gate);
+
+int32_t dice_replay(dice_ctx* ctx, const uint16_t* rgbi, int64_t height,
+                    int64_t width, const float* lut, const dice_params* params,
+                    const dice_stage_table* stages, uint32_t rng_state,
+                    uint32_t flags, uint16_t* out_rgb16,
+                    dice_replay_result* result, dice_diagnostics* diagnostics,
+                    const dice_callbacks* callbacks,
+                    dice_stage_timings* timings);
+
+int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi,
+                         int64_t height, int64_t width,
+                         const uint16_t* prepass_rgbi, int64_t prepass_height,
+                         int64_t prepass_width, const dice_params* params,
+                         const dice_prepass_params* prepass_params,
+                         uint32_t flags, uint16_t* out_rgb16,
+                         dice_replay_result* result,
+                         dice_diagnostics* diagnostics,
+                         const dice_callbacks* callbacks,
+                         dice_stage_timings* timings);
+
+/* Level-1 test entry points: the same device functions the replay kernels
+ * use, launched over host-provided vectors. */
+int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches,
+                                 int64_t count, double* out3);
+int32_t dice_t_rgb_unscaled_averages(dice_ctx* ctx, const float* patches,
+                                     int64_t count, double* out9);
+int32_t dice_t_dither_chain(dice_ctx* ctx, const double* values,
+                            const float* scales, int64_t count,
+                            uint32_t state_in, float low, float high,
+                            double* deltas, uint32_t* state_out,
+                            uint64_t* advances_out);
+int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count,
+                    uint16_t* out);
+/* Planes for every row with explicit per-row parameter arrays:
+ * working H*W*4, aux/score/waux H*W, wrgb H*W*3. */
+int32_t dice_t_planes(dice_ctx* ctx, const uint16_t* rgbi, int64_t height,
+                      int64_t width, const float* lut,
+                      const dice_params* params, const float* row_alpha,
+                      const uint8_t* row_alpha_is_one,
+                      const float* row_replacement, const float* row_offset,
+                      const float* row_score_base, float* working, float* aux,
+                      float* score, float* waux, float* wrgb,
+                      uint32_t* error_flags);
+/* Decision fallback (1) / eligible (0) for every pixel of an aux plane,
+ * with the floor gate disabled and row gate 0. */
+int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height,
+                        int64_t width, const dice_params* params,
+                        uint8_t* fallback);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DICE_HIP_H */
