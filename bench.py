@@ -1,0 +1,204 @@
+"""Benchmark: bit-exact Mpix/s of the Digital ICE processing core on MI355X.
+
+Default workload (N=1): the plain single-frame HIP path on a synthetic
+5782x3946 RGBI uint16 frame (400 dust disks, 40 scratches <= 400 px).
+One step = one whole job with the main frame already resident in HBM:
+producer schedule, planes / decision / features / candidates / combiner,
+hidden startup, the serial conditional-dither writer (host), output patch,
+and the SHA-256 of the public RGB16 output (which needs the output bytes on
+the host).  Timed: exactly K steps after W warm-up steps, bracketed by
+device synchronization.
+
+The JSON line also carries the HBM roofline of the streaming analysis
+kernel (14 algorithmic bytes per pixel: 8 B RGBI in + 6 B RGB out, timed
+with HIP events on the library's stream) and the CPU baseline: the
+compiled CPU oracle (the C restatement of the reference's CPU path) timed
+on this host's cores on the same frame, whose output also checks the GPU
+result bit for bit.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "digital-fauxice_amd"))
+
+METRIC = "bit-exact Mpix/s at 5782x3946 RGBI16 (batched vs plain); p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PIXEL = 14  # 8 B RGBI uint16 in + 6 B RGB uint16 out
+
+
+def parse_args() -> argparse.Namespace:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--no-cpu-baseline", action="store_true",
+                        help="skip the CPU oracle leg (and its bit-exact check)")
+    parser.add_argument("--cpu-seconds", type=float, default=15.0,
+                        help="approximate CPU time budget of the baseline sample")
+    return parser.parse_args()
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(jobs shard across GPUs without collectives; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from dice_amd import _abi, synth
+    from dice_amd._abi import DiceReplayResult, DiceStageTimings, ptr
+    from dice_amd.hip_backend import engine
+    from dice_amd.hip_backend.process import stable_profile_params
+    from dice_amd.profile import DEFAULT_PROFILE
+
+    import torch
+
+    spec = synth.config2_full_frame()
+    main_px = synth.frame(spec)
+    prepass = np.ascontiguousarray(synth.prepass_for(main_px))
+    height, width = spec.height, spec.width
+    mpix = height * width / 1e6
+    torch.cuda.set_device(0)
+    d_main = torch.from_numpy(main_px.view(np.int16)).to("cuda")
+    d_out = torch.empty((height, width, 3), dtype=torch.int16, device="cuda")
+    torch.cuda.synchronize()
+
+    lib = engine.library()
+    ctx = engine.context()
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    flags = _abi.DICE_MAIN_ON_DEVICE | _abi.DICE_OUT_ON_DEVICE
+
+    def step(timings: DiceStageTimings) -> DiceReplayResult:
+        res = DiceReplayResult()
+        code = lib.dice_process_job(ctx, C.c_void_p(d_main.data_ptr()), height, width,
+                                    ptr(prepass, C.c_uint16), prepass.shape[0],
+                                    prepass.shape[1], C.byref(params),
+                                    C.byref(prepass_params), flags,
+                                    C.c_void_p(d_out.data_ptr()), C.byref(res), None, None,
+                                    C.byref(timings))
+        engine.raise_for(code)
+        return res
+
+    first = None
+    for i in range(args.warmup):
+        res = step(DiceStageTimings())
+        first = first or res
+        print(f"bench: warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    latencies, stage_rows = [], []
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        timings = DiceStageTimings()
+        res = step(timings)
+        latencies.append(time.perf_counter() - t0)
+        stage_rows.append(timings.as_dict())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    stage_ms = {key: statistics.mean(row[key] for row in stage_rows) * (1.0 if key.endswith("_ms") else 1e3)
+                for key in stage_rows[0]}
+    scan_s = stage_ms["scan_kernel_ms"] / 1e3
+    achieved = BYTES_PER_PIXEL * height * width / scan_s / 1e9 if scan_s > 0 else 0.0
+    latencies_ms = sorted(v * 1e3 for v in latencies)
+    p50 = float(np.percentile(latencies_ms, 50))
+    p99 = float(np.percentile(latencies_ms, 99))
+
+    cpu_baseline = None
+    bit_exact = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib
+
+        threads = min(os.cpu_count() or 1, 16)
+        runs = []
+        oracle = None
+        budget_start = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            oracle = oracle_lib.process_job(main_px, prepass, params, prepass_params,
+                                            threads=threads)
+            runs.append(time.perf_counter() - t0)
+            print(f"bench: cpu baseline run {len(runs)} {runs[-1]:.2f} s", file=sys.stderr,
+                  flush=True)
+            if len(runs) >= 4 or time.perf_counter() - budget_start > args.cpu_seconds:
+                break
+        timed_runs = runs[1:] if len(runs) > 1 else runs
+        median = statistics.median(timed_runs)
+        bit_exact = (
+            res.output_sha256.decode() == oracle.output_sha256
+            and res.attempted_pixels == oracle.attempted_pixels
+            and res.written_pixels == oracle.written_pixels
+            and res.public_rng_advances == oracle.public_rng_advances
+            and res.final_rng_state == oracle.final_rng_state
+            and res.changed_pixels == oracle.changed_pixels
+        )
+        cpu_baseline = {
+            "value": round(mpix / median, 4),
+            "unit": "Mpix/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": (f"{len(timed_runs)} timed full-frame jobs (5782x3946, after "
+                       f"{len(runs) - len(timed_runs)} warm-up) of the compiled C oracle, "
+                       f"OpenMP row-parallel analysis + serial writer; median {median:.3f} s"),
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": round(args.steps * mpix / elapsed, 4),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain single-frame HIP path, 5782x3946 RGBI16, 400 disks + 40 scratches",
+            "global_batch": 1,
+            "height": height,
+            "width": width,
+            "attempted_sites": int(res.attempted_pixels),
+            "parallelism": "single GPU",
+        },
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
+        "bit_exact": bit_exact,
+        "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+        "writer_share": round(stage_ms["writer_host"] / stage_ms["total"], 4)
+        if stage_ms["total"] else None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_scan",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu_baseline,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,1239 @@
+// dice_host.cpp -- host runtime and C ABI of libdice_hip.so.
+//
+// Orchestrates one job the way process_cuda / run_streaming_replay_cuda do
+// (rohanpandula/digital-fauxice, src/portable_digital_ice/cuda_backend/
+// process.py:25-159, engine.py:183-590), MI355X-first:
+//   host    prepass (0x40 record, tiny), LUT/factor tables (hash-pinned)
+//   device  producer schedule incl. the sequential chains and the setter
+//   device  fused analysis tiles + hidden-startup sites + row-major compaction
+//   host    the strictly serial conditional-dither writer over Site records
+//   device/host  patch of changed pixels, SHA-256 of the public output
+#include <hip/hip_runtime.h>
+#include <openssl/evp.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dice_hip.h"
+#include "dice_device.hpp"
+#include "dice_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace dice;
+
+namespace {
+
+thread_local char g_error[512];
+
+int32_t set_error(int32_t code, const char* fmt, ...) {
+  va_list args;
+  va_start(args, fmt);
+  vsnprintf(g_error, sizeof g_error, fmt, args);
+  va_end(args);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return set_error(DICE_INTERNAL, "%s failed: %s", #expr,                  \
+                       hipGetErrorString(_e));                                \
+  } while (0)
+
+double now_seconds() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// --------------------------------------------------------------------------
+// SHA-256 (OpenSSL EVP) and the two hash-pinned tables
+// --------------------------------------------------------------------------
+
+bool sha256_hex(const void* data, size_t bytes, char out[72]) {
+  unsigned char digest[32];
+  unsigned int length = 0;
+  EVP_MD_CTX* md = EVP_MD_CTX_new();
+  if (!md) return false;
+  bool ok = EVP_DigestInit_ex(md, EVP_sha256(), nullptr) == 1 &&
+            EVP_DigestUpdate(md, data, bytes) == 1 &&
+            EVP_DigestFinal_ex(md, digest, &length) == 1;
+  EVP_MD_CTX_free(md);
+  if (!ok) return false;
+  static const char* hex = "0123456789abcdef";
+  for (int i = 0; i < 32; ++i) {
+    out[2 * i] = hex[digest[i] >> 4];
+    out[2 * i + 1] = hex[digest[i] & 15];
+  }
+  out[64] = 0;
+  return true;
+}
+
+constexpr const char* kLutSha256 =
+    "5fd225e25719a544df9475315e3017f9b8027d593ac43e392245de59f49f3fcb";  // x3a.py:26-28
+constexpr const char* kHighSha256 =
+    "fdc378493f13916e36d3dbd9b69b714e888d7993680509df4550486db6fcf328";  // output.py:19
+constexpr const char* kLowSha256 =
+    "b7f3b1b5c0ea816ddb20196e1cf7614d364ea4e164295e0dcb78119c88ef404d";  // output.py:20
+
+struct Tables {
+  std::vector<float> lut;
+  std::vector<uint32_t> high, low;
+  int32_t status = -1;
+};
+
+// x3a.py:111-132 and output.py:39-58: the binary64 closed forms, narrowed
+// once, then pinned by SHA-256.  A platform libm that differs in any last
+// bit fails closed.
+const Tables& tables() {
+  static Tables t;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    t.lut.resize(65536);
+    for (int i = 0; i < 65536; ++i)
+      t.lut[i] = (float)(65535.0 * std::log2((double)i + 1.0) / 16.0);
+    t.high.resize(256);
+    t.low.resize(256);
+    const double scale = 16.0 * std::log10(2.0) / 65535.0;
+    for (int d = 0; d < 256; ++d) {
+      const double dd = (double)d;
+      t.high[d] = (uint32_t)std::floor(16.0 * std::pow(10.0, 256.0 * dd * scale) + 0.5);
+      t.low[d] = (uint32_t)std::floor(65536.0 * std::pow(10.0, dd * scale) + 0.5);
+    }
+    char h1[72], h2[72], h3[72];
+    if (!sha256_hex(t.lut.data(), 65536 * 4, h1) ||
+        !sha256_hex(t.high.data(), 256 * 4, h2) ||
+        !sha256_hex(t.low.data(), 256 * 4, h3)) {
+      t.status = DICE_INTERNAL;
+    } else if (strcmp(h1, kLutSha256) != 0 || strcmp(h2, kHighSha256) != 0 ||
+               strcmp(h3, kLowSha256) != 0) {
+      t.status = DICE_UNAVAILABLE;
+    } else {
+      t.status = DICE_OK;
+    }
+  });
+  return t;
+}
+
+int32_t require_tables() {
+  const Tables& t = tables();
+  if (t.status == DICE_UNAVAILABLE)
+    return set_error(DICE_UNAVAILABLE,
+                     "platform logarithm/power does not reproduce the hash-pinned "
+                     "response LUT and inverse factors bit-exact");
+  if (t.status != DICE_OK) return set_error(DICE_INTERNAL, "SHA-256 unavailable");
+  return DICE_OK;
+}
+
+// --------------------------------------------------------------------------
+// Device buffers
+// --------------------------------------------------------------------------
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+  }
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t rounded = (need + 0xFFFFF) & ~(size_t)0xFFFFF;
+    hipError_t e = hipMalloc(&ptr, rounded);
+    if (e == hipSuccess) bytes = rounded;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct PinnedBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t rounded = (need + 0xFFFF) & ~(size_t)0xFFFF;
+    hipError_t e = hipHostMalloc(&ptr, rounded, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = rounded;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// Context
+// --------------------------------------------------------------------------
+
+struct dice_ctx {
+  int device = 0;
+  hipEvent_t events[8] = {};
+  uint64_t hbm_budget = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mutex;
+  int tile_rows = 16, tile_cols = 64;
+  // tables on the device
+  DeviceBuffer lut, high, low;
+  // job workspace
+  DeviceBuffer rgbi, out, site_lists, tile_count, seg_count, seg_offset, sites, counters,
+      flags;
+  DeviceBuffer diag_score, diag_floor;
+  DeviceBuffer su_aux, su_score, su_work, su_sites, su_attempted;
+  DeviceBuffer failpos, row_sums, accepted, epoch_terms, epoch_adds, cumulative,
+      epoch_wide, epoch_store;
+  DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse,
+      st_gate, st_row_gate;
+  DeviceBuffer patch_pixels, patch_values;
+  DeviceBuffer sched[12];
+  // pinned host staging
+  PinnedBuffer h_counters, h_sites, h_su_sites, h_su_attempted, h_patch_pixels,
+      h_patch_values, h_out;
+};
+
+namespace {
+
+// Kernel-group timing with HIP events on the context stream (only when the
+// caller asks for stage timings).
+struct EventTimer {
+  dice_ctx* ctx;
+  bool on;
+  int32_t record(int slot) {
+    if (on) HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));
+    return DICE_OK;
+  }
+  double elapsed_ms(int a, int b) {
+    if (!on) return 0.0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->events[a], ctx->events[b]) != hipSuccess) return 0.0;
+    return (double)ms;
+  }
+};
+
+int32_t activate(dice_ctx* ctx) {
+  if (!ctx) return set_error(DICE_INVALID, "null context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return DICE_OK;
+}
+
+int32_t upload_tables(dice_ctx* ctx) {
+  const Tables& t = tables();
+  if (ctx->lut.bytes >= 65536 * 4) return DICE_OK;
+  HIP_TRY(ctx->lut.ensure(65536 * 4));
+  HIP_TRY(ctx->high.ensure(256 * 4));
+  HIP_TRY(ctx->low.ensure(256 * 4));
+  HIP_TRY(hipMemcpy(ctx->lut.ptr, t.lut.data(), 65536 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->high.ptr, t.high.data(), 256 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->low.ptr, t.low.data(), 256 * 4, hipMemcpyHostToDevice));
+  return DICE_OK;
+}
+
+DevParams device_params(const dice_params& p) {
+  DevParams P;
+  P.visible_channel = p.visible_channel;
+  P.score_addend = p.score_addend;
+  P.score_scale = p.score_scale;
+  P.score_offset = p.score_offset;
+  P.score_floor = p.score_floor;
+  P.horizontal_minimum = p.horizontal_minimum;
+  P.decision_threshold = p.decision_threshold;
+  P.count_limit = p.decision_count_limit;
+  P.radius = p.decision_radius;
+  P.cross_neighbor = p.cross_neighbor;
+  P.gate_primary = p.gate_primary;
+  P.combine.coarse_enabled = p.coarse_enabled;
+  for (int i = 0; i < 3; ++i) {
+    P.combine.coarse_slopes[i] = p.coarse_slopes[i];
+    P.combine.band_enabled[i] = p.band_enabled[i];
+    P.combine.band_scales[i] = p.band_scales[i];
+    P.combine.configured_strengths[i] = p.configured_strengths[i];
+  }
+  for (int i = 0; i < 9; ++i) {
+    P.combine.factors_a[i] = p.factors_a[i];
+    P.combine.factors_b[i] = p.factors_b[i];
+  }
+  return P;
+}
+
+bool all_finite(const float* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+// Parameter checks the reference performs before or during the replay
+// (streaming.py:386-396, startup.py:203-216, fast_cpu/engine.py:389-407).
+int32_t validate_replay(int64_t H, int64_t W, const dice_params& p) {
+  if (H < 4) return set_error(DICE_INVALID, "streaming startup requires at least four main rows");
+  if (W < 1) return set_error(DICE_INVALID, "RGBI frame cannot have an empty dimension");
+  if (H * W >= (int64_t)1 << 31)
+    return set_error(DICE_INVALID, "frame exceeds 2^31 pixels");
+  if (p.visible_channel < 0 || p.visible_channel > 2)
+    return set_error(DICE_INVALID, "selected_visible_channel must be 0, 1, or 2");
+  if (p.decision_radius < 1 || p.decision_radius > 4)
+    return set_error(DICE_INVALID, "perpendicular_radius must be in [1, 4]");
+  if (p.cross_neighbor && H < 5)
+    return set_error(DICE_INVALID, "cross-neighbor startup requires the first five real rows");
+  if (p.cross_neighbor && W < 4)
+    return set_error(DICE_INVALID, "cross-neighbor startup requires at least four columns");
+  if (!(std::isfinite(p.score_floor) && p.score_floor > 0.0f))
+    return set_error(DICE_INVALID, "startup score floor must be finite and positive");
+  if (!(std::isfinite(p.dither_low) && std::isfinite(p.dither_high) &&
+        all_finite(p.dither_scales, 3)))
+    return set_error(DICE_INVALID, "dither inputs must be finite");
+  if (!(all_finite(p.coarse_slopes, 3) && all_finite(p.band_scales, 3) &&
+        all_finite(p.factors_a, 9) && all_finite(p.factors_b, 9) &&
+        all_finite(p.configured_strengths, 3)))
+    return set_error(DICE_INVALID, "reconstruction parameters must be finite");
+  return DICE_OK;
+}
+
+struct Callbacks {
+  const dice_callbacks* cb;
+  void progress(int32_t phase, int64_t completed, int64_t total, int64_t attempted,
+                int64_t written) const {
+    if (cb && cb->progress) cb->progress(cb->user, phase, completed, total, attempted, written);
+  }
+  bool cancelled() const { return cb && cb->cancelled && cb->cancelled(cb->user) != 0; }
+};
+
+int32_t cancelled_error() {
+  return set_error(DICE_CANCELLED, "portable correction was cancelled");
+}
+
+// Sizes the workspace for an H x W job and fills the JobDev view.
+int32_t prepare_job(dice_ctx* ctx, int64_t H, int64_t W, bool want_diag, JobDev& J) {
+  const int TY = ctx->tile_rows, TX = ctx->tile_cols;
+  memset(&J, 0, sizeof J);
+  J.height = (int32_t)H;
+  J.width = (int32_t)W;
+  J.tile_rows = TY;
+  J.tile_cols = TX;
+  J.tiles_x = (int32_t)((W + TX - 1) / TX);
+  J.tiles_y = (int32_t)((H + TY - 1) / TY);
+  const size_t pixels = (size_t)H * W;
+  const size_t tiles = (size_t)J.tiles_x * J.tiles_y;
+  const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
+  const size_t stages = (size_t)H + 6;
+  const size_t cells = (size_t)W / 8;
+  const size_t fail_rows = (size_t)(H + 7) / 8;
+  const size_t epochs = (size_t)H / 8;
+  HIP_TRY(ctx->site_lists.ensure(tiles * TY * TX * sizeof(uint16_t)));
+  HIP_TRY(ctx->tile_count.ensure(tiles * 4));
+  HIP_TRY(ctx->seg_count.ensure(segments * 4));
+  HIP_TRY(ctx->seg_offset.ensure(segments * 4));
+  HIP_TRY(ctx->counters.ensure(64));
+  HIP_TRY(ctx->flags.ensure(64));
+  HIP_TRY(ctx->su_aux.ensure(5 * W * 4));
+  HIP_TRY(ctx->su_score.ensure(5 * W * 4));
+  HIP_TRY(ctx->su_work.ensure(5 * W * 12));
+  HIP_TRY(ctx->su_sites.ensure(6 * W * sizeof(Site)));
+  HIP_TRY(ctx->su_attempted.ensure(6 * W));
+  HIP_TRY(ctx->failpos.ensure(fail_rows * cells + 8));
+  HIP_TRY(ctx->row_sums.ensure(H * 3 * 8));
+  HIP_TRY(ctx->accepted.ensure(H * 4));
+  HIP_TRY(ctx->epoch_terms.ensure(epochs * cells * 64 + 64));
+  HIP_TRY(ctx->epoch_adds.ensure(epochs * 16 + 16));
+  HIP_TRY(ctx->cumulative.ensure(H * 24));
+  HIP_TRY(ctx->epoch_wide.ensure(epochs * 16 + 16));
+  HIP_TRY(ctx->epoch_store.ensure(epochs * 8 + 8));
+  HIP_TRY(ctx->st_alpha.ensure(stages * 4));
+  HIP_TRY(ctx->st_offset.ensure(stages * 4));
+  HIP_TRY(ctx->st_replacement.ensure(stages * 4));
+  HIP_TRY(ctx->st_has.ensure(stages));
+  HIP_TRY(ctx->st_score_base.ensure(stages * 4));
+  HIP_TRY(ctx->st_coarse.ensure(stages * 4));
+  HIP_TRY(ctx->st_gate.ensure(stages));
+  HIP_TRY(ctx->st_row_gate.ensure(stages * 4));
+  if (want_diag) {
+    HIP_TRY(ctx->diag_score.ensure(pixels * 4));
+    HIP_TRY(ctx->diag_floor.ensure(pixels));
+  }
+  HIP_TRY(ctx->h_counters.ensure(64));
+  J.lut = ctx->lut.as<float>();
+  J.factor_high = ctx->high.as<uint32_t>();
+  J.factor_low = ctx->low.as<uint32_t>();
+  J.stages.alpha = ctx->st_alpha.as<float>();
+  J.stages.offset = ctx->st_offset.as<float>();
+  J.stages.replacement = ctx->st_replacement.as<float>();
+  J.stages.has_replacement = ctx->st_has.as<uint8_t>();
+  J.stages.score_base = ctx->st_score_base.as<float>();
+  J.stages.coarse = ctx->st_coarse.as<float>();
+  J.stages.gate_secondary = ctx->st_gate.as<uint8_t>();
+  J.stages.row_gate = ctx->st_row_gate.as<int32_t>();
+  J.site_lists = ctx->site_lists.as<uint16_t>();
+  J.tile_count = ctx->tile_count.as<uint32_t>();
+  J.seg_count = ctx->seg_count.as<uint32_t>();
+  J.seg_offset = ctx->seg_offset.as<uint32_t>();
+  J.counters = ctx->counters.as<uint32_t>();
+  J.error_flags = ctx->flags.as<uint32_t>();
+  J.diag_score = want_diag ? ctx->diag_score.as<float>() : nullptr;
+  J.diag_floor = want_diag ? ctx->diag_floor.as<uint8_t>() : nullptr;
+  J.su_aux = ctx->su_aux.as<float>();
+  J.su_score = ctx->su_score.as<float>();
+  J.su_work = ctx->su_work.as<float>();
+  J.su_sites = ctx->su_sites.as<Site>();
+  J.su_attempted = ctx->su_attempted.as<uint8_t>();
+  J.failpos = ctx->failpos.as<uint8_t>();
+  J.row_sums = ctx->row_sums.as<double>();
+  J.accepted = ctx->accepted.as<uint32_t>();
+  J.epoch_terms = ctx->epoch_terms.as<double>();
+  J.epoch_adds = ctx->epoch_adds.as<double>();
+  J.cumulative = ctx->cumulative.as<double>();
+  J.epoch_wide = ctx->epoch_wide.as<double>();
+  J.epoch_store = ctx->epoch_store.as<float>();
+  return DICE_OK;
+}
+
+int32_t flag_error(uint32_t flags) {
+  if (flags & kErrNonfiniteAux)
+    return set_error(DICE_NONFINITE_AUX, "auxiliary score input must be a finite HxW plane");
+  if (flags & kErrSetterScaleOne)
+    return set_error(DICE_INVALID, "X3A setter scale cannot equal one");
+  if (flags & kErrStageNonfinite)
+    return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
+  return DICE_OK;
+}
+
+// The strictly serial writer over the six hidden stages and then every
+// attempted site in row-major order (streaming.py:476-531,
+// startup.py:231-341, fast_cpu/kernels.py:1139-1204).
+struct WriterOutput {
+  uint64_t attempted = 0, written = 0, advances = 0, changed = 0;
+  uint32_t state = 0;
+  uint32_t patches = 0;
+  bool nonfinite = false;
+};
+
+WriterOutput run_writer(const dice_params& p, uint32_t state, int64_t W,
+                        const Site* su_sites, const uint8_t* su_attempted,
+                        const Site* sites, uint32_t count, uint32_t* patch_pixels,
+                        uint16_t* patch_values, dice_replay_result* result) {
+  const Tables& T = tables();
+  WriterConsts K;
+  K.low = (double)p.dither_low;
+  K.high = (double)p.dither_high;
+  K.low_lt_high = K.low < K.high;
+  K.scales[0] = p.dither_scales[0];
+  K.scales[1] = p.dither_scales[1];
+  K.scales[2] = p.dither_scales[2];
+  float values[3];
+  for (int s = 0; s < 6; ++s) {
+    int64_t attempted = 0, advances = 0;
+    for (int64_t x = 0; x < W; ++x) {
+      if (!su_attempted[s * W + x]) continue;
+      const Site& site = su_sites[s * W + x];
+      attempted += 1;
+      advances += write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0,
+                              K, state, values);
+    }
+    result->startup_attempted[s] = attempted;
+    result->startup_advances[s] = advances;
+  }
+  result->startup_final_rng_state = state;
+
+  WriterOutput o;
+  for (uint32_t i = 0; i < count; ++i) {
+    const Site& site = sites[i];
+    o.advances += (uint64_t)write_pixel(site.cand, site.orig,
+                                        (site.pixel & kSiteFloorBit) != 0, K, state,
+                                        values);
+    if (values[0] != site.orig[0] || values[1] != site.orig[1] ||
+        values[2] != site.orig[2])
+      o.written += 1;
+    if (!(std::isfinite(values[0]) && std::isfinite(values[1]) &&
+          std::isfinite(values[2]))) {
+      o.nonfinite = true;
+      continue;
+    }
+    uint16_t rendered[3], noop[3];
+    bool changed = false;
+    for (int c = 0; c < 3; ++c) {
+      rendered[c] = emit_value(values[c], T.high.data(), T.low.data());
+      noop[c] = emit_value(site.orig[c], T.high.data(), T.low.data());
+      changed |= rendered[c] != noop[c];
+    }
+    if (changed) {
+      o.changed += 1;
+      patch_pixels[o.patches] = site.pixel & kSitePixelMask;
+      patch_values[o.patches * 3 + 0] = rendered[0];
+      patch_values[o.patches * 3 + 1] = rendered[1];
+      patch_values[o.patches * 3 + 2] = rendered[2];
+      o.patches += 1;
+    }
+  }
+  o.attempted = count;
+  o.state = state;
+  return o;
+}
+
+// The analysis + writer + output phases of one job whose input is already
+// in `J.rgbi` and whose stage table is on the device.
+int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng_state,
+                    uint32_t flags, uint16_t* out_host, dice_replay_result* result,
+                    dice_diagnostics* diag, const Callbacks& cb,
+                    dice_stage_timings* timings) {
+  hipStream_t s = ctx->stream;
+  const int64_t H = J.height, W = J.width;
+  const size_t pixels = (size_t)H * W;
+  DevParams P = device_params(p);
+  cb.progress(3, 0, H, 0, 0);
+  double t0 = now_seconds();
+  EventTimer ev{ctx, timings != nullptr};
+  if (int32_t e = ev.record(2)) return e;
+  HIP_TRY(launch_scan(J, P, s));
+  if (int32_t e = ev.record(3)) return e;
+  uint32_t* hc = ctx->h_counters.as<uint32_t>();
+  HIP_TRY(hipMemcpyAsync(hc, J.counters, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint32_t count = hc[0];
+  if (int32_t e = flag_error(hc[1])) return e;
+  if (cb.cancelled()) return cancelled_error();
+  // Sites go straight to their row-major slot; size the array by the count.
+  HIP_TRY(ctx->sites.ensure((size_t)count * sizeof(Site) + sizeof(Site)));
+  J.sites = ctx->sites.as<Site>();
+  if (int32_t e = ev.record(4)) return e;
+  HIP_TRY(launch_sites(J, P, s));
+  if (int32_t e = ev.record(5)) return e;
+  HIP_TRY(launch_startup(J, P, s));
+  if (int32_t e = ev.record(6)) return e;
+  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double t1 = now_seconds();
+  if (timings) {
+    timings->analysis_device += t1 - t0;
+    timings->scan_kernel_ms += ev.elapsed_ms(2, 3);
+    timings->sites_kernel_ms += ev.elapsed_ms(4, 5);
+    timings->startup_kernels_ms += ev.elapsed_ms(5, 6);
+  }
+  if (int32_t e = flag_error(hc[1])) return e;
+
+  HIP_TRY(ctx->h_sites.ensure((size_t)count * sizeof(Site) + 64));
+  HIP_TRY(ctx->h_su_sites.ensure(6 * W * sizeof(Site)));
+  HIP_TRY(ctx->h_su_attempted.ensure(6 * W));
+  if (count)
+    HIP_TRY(hipMemcpyAsync(ctx->h_sites.ptr, J.sites, (size_t)count * sizeof(Site),
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(ctx->h_su_sites.ptr, J.su_sites, 6 * W * sizeof(Site),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(ctx->h_su_attempted.ptr, J.su_attempted, 6 * W,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double t2 = now_seconds();
+  if (timings) timings->download_sites += t2 - t1;
+
+  HIP_TRY(ctx->h_patch_pixels.ensure((size_t)count * 4 + 64));
+  HIP_TRY(ctx->h_patch_values.ensure((size_t)count * 6 + 64));
+  uint32_t* patch_pixels = ctx->h_patch_pixels.as<uint32_t>();
+  uint16_t* patch_values = ctx->h_patch_values.as<uint16_t>();
+  WriterOutput w = run_writer(p, rng_state, W, ctx->h_su_sites.as<Site>(),
+                              ctx->h_su_attempted.as<uint8_t>(), ctx->h_sites.as<Site>(),
+                              count, patch_pixels, patch_values, result);
+  double t3 = now_seconds();
+  if (timings) timings->writer_host += t3 - t2;
+  if (w.nonfinite) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
+  if (cb.cancelled()) return cancelled_error();
+
+  const bool out_on_device = (flags & DICE_OUT_ON_DEVICE) != 0;
+  const bool want_sha = (flags & DICE_SKIP_SHA256) == 0;
+  result->output_sha256[0] = 0;
+  if (out_on_device) {
+    if (w.patches) {
+      HIP_TRY(ctx->patch_pixels.ensure((size_t)w.patches * 4));
+      HIP_TRY(ctx->patch_values.ensure((size_t)w.patches * 6));
+      HIP_TRY(hipMemcpyAsync(ctx->patch_pixels.ptr, patch_pixels, (size_t)w.patches * 4,
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(ctx->patch_values.ptr, patch_values, (size_t)w.patches * 6,
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(launch_apply_patches(J.out, ctx->patch_pixels.as<uint32_t>(),
+                                   ctx->patch_values.as<uint16_t>(), w.patches, s));
+    }
+    if (want_sha) {
+      HIP_TRY(ctx->h_out.ensure(pixels * 6));
+      HIP_TRY(hipMemcpyAsync(ctx->h_out.ptr, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    double t4 = now_seconds();
+    if (timings) timings->finish_device += t4 - t3;
+    if (want_sha && !sha256_hex(ctx->h_out.ptr, pixels * 6, result->output_sha256))
+      return set_error(DICE_INTERNAL, "SHA-256 failed");
+    if (timings) timings->sha256_host += now_seconds() - t4;
+  } else {
+    HIP_TRY(hipMemcpyAsync(out_host, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double t4 = now_seconds();
+    if (timings) timings->download_output += t4 - t3;
+    for (uint32_t i = 0; i < w.patches; ++i) {
+      uint16_t* o = out_host + (size_t)patch_pixels[i] * 3;
+      o[0] = patch_values[i * 3 + 0];
+      o[1] = patch_values[i * 3 + 1];
+      o[2] = patch_values[i * 3 + 2];
+    }
+    if (want_sha && !sha256_hex(out_host, pixels * 6, result->output_sha256))
+      return set_error(DICE_INTERNAL, "SHA-256 failed");
+    if (timings) timings->sha256_host += now_seconds() - t4;
+  }
+  if (diag) {
+    if (diag->score_plane)
+      HIP_TRY(hipMemcpy(diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
+    if (diag->at_floor_mask)
+      HIP_TRY(hipMemcpy(diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
+    if (diag->changed_mask) {
+      memset(diag->changed_mask, 0, pixels);
+      for (uint32_t i = 0; i < w.patches; ++i) diag->changed_mask[patch_pixels[i]] = 1;
+    }
+  }
+  result->attempted_pixels = w.attempted;
+  result->written_pixels = w.written;
+  result->public_rng_advances = w.advances;
+  result->changed_pixels = w.changed;
+  result->final_rng_state = w.state;
+  cb.progress(3, H, H, (int64_t)w.attempted, (int64_t)w.written);
+  return DICE_OK;
+}
+
+// Places the main RGBI input (and the output target) for one job.
+int32_t bind_io(dice_ctx* ctx, JobDev& J, const uint16_t* rgbi, uint16_t* out,
+                uint32_t flags, dice_stage_timings* timings) {
+  const size_t pixels = (size_t)J.height * J.width;
+  double t0 = now_seconds();
+  if (flags & DICE_MAIN_ON_DEVICE) {
+    J.rgbi = rgbi;
+  } else {
+    HIP_TRY(ctx->rgbi.ensure(pixels * 8));
+    HIP_TRY(hipMemcpyAsync(ctx->rgbi.ptr, rgbi, pixels * 8, hipMemcpyHostToDevice,
+                           ctx->stream));
+    J.rgbi = ctx->rgbi.as<uint16_t>();
+  }
+  if (flags & DICE_OUT_ON_DEVICE) {
+    J.out = out;
+  } else {
+    HIP_TRY(ctx->out.ensure(pixels * 6));
+    J.out = ctx->out.as<uint16_t>();
+  }
+  if (timings) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    timings->upload += now_seconds() - t0;
+  }
+  return DICE_OK;
+}
+
+
+// ---- prepass (prepass.py:67-236) --------------------------------------
+
+void put32(uint8_t* p, int off, const void* v) { memcpy(p + off, v, 4); }
+void put64(uint8_t* p, int off, double v) { memcpy(p + off, &v, 8); }
+float get32f(const uint8_t* p, int off) {
+  float v;
+  memcpy(&v, p + off, 4);
+  return v;
+}
+
+void prepass_quadrants(const float v[8][8], float out[4]) {
+  int q = 0;
+  for (int fy = 0; fy < 8; fy += 4)
+    for (int fx = 0; fx < 8; fx += 4) {
+      double total = 0.0;
+      for (int y = fy; y < fy + 4; ++y)
+        for (int x = fx; x < fx + 4; ++x) total += (double)v[y][x];
+      out[q++] = (float)(total * 0.0625);
+    }
+}
+
+float prepass_mean(const float m[4]) {
+  double total = (double)m[0];
+  total += (double)m[1];
+  total += (double)m[2];
+  total += (double)m[3];
+  return (float)(total * 0.25);
+}
+
+int32_t host_prepass(const uint16_t* px, int64_t H, int64_t W, const float* lut,
+                     const dice_prepass_params& prm, dice_prepass_result* out) {
+  if (H < 1 || W < 1) return set_error(DICE_INVALID, "RGBI frame cannot have an empty dimension");
+  const int64_t PH = (H + 7) & ~(int64_t)7, PW = (W + 7) & ~(int64_t)7;
+  const double ratio_min = (double)prm.ratio_minimum, ratio_max = (double)prm.ratio_maximum;
+  double cal_r = 0.0, cal_ir = 0.0, cal_sq = 0.0;
+  float reg_den = 0.0f, reg_num = 0.0f;
+  int64_t valid = 0, tiles = 0;
+  for (int64_t fy = 0; fy < PH; fy += 8)
+    for (int64_t fx = 0; fx < PW; fx += 8) {
+      float resp_r[8][8] = {}, resp_ir[8][8] = {};
+      float sum_raw_ir = 0.0f;
+      bool tile_valid = true;
+      for (int y = 0; y < 8; ++y)
+        for (int x = 0; x < 8; ++x) {
+          uint16_t raw[4] = {0, 0, 0, 0};
+          if (fy + y < H && fx + x < W) memcpy(raw, px + ((fy + y) * W + fx + x) * 4, 8);
+          const int ir = raw[3];
+          if ((float)ir <= prm.valid_ir_threshold) {
+            tile_valid = false;
+            continue;
+          }
+          valid += 1;
+          const float cr = lut[raw[0]], cir = lut[ir];
+          resp_r[y][x] = cr;
+          resp_ir[y][x] = cir;
+          const double square = (double)((int64_t)ir * ir);
+          cal_r += square * (double)cr;
+          cal_ir += square * (double)cir;
+          cal_sq += square;
+          sum_raw_ir = (float)((double)sum_raw_ir + (double)ir);
+        }
+      if (!tile_valid) continue;
+      tiles += 1;
+      float rm[4], im[4];
+      prepass_quadrants(resp_r, rm);
+      prepass_quadrants(resp_ir, im);
+      const float rg = prepass_mean(rm), ig = prepass_mean(im);
+      double den = (double)reg_den, num = (double)reg_num;
+      const double scalar = (double)sum_raw_ir;
+      for (int q = 0; q < 4; ++q) {
+        const float rd = (float)((double)rm[q] - (double)rg);
+        const float id = (float)((double)im[q] - (double)ig);
+        if (rd == 0.0f) continue;
+        const double ratio = (double)id / (double)rd;
+        if (ratio < ratio_min || ratio > ratio_max) continue;
+        const double weight = (double)rd * (double)rd * scalar * scalar;
+        den += weight;
+        num += ratio * weight;
+      }
+      reg_den = (float)den;
+      reg_num = (float)num;
+    }
+  double unrounded = reg_den == 0.0f ? (double)prm.zero_denominator_alpha
+                                     : (double)reg_num / (double)reg_den;
+  const float alpha = (float)unrounded;
+  const float scaled = (float)(unrounded * (double)prm.alpha_scale);
+  uint8_t* p = out->payload;
+  memset(p, 0, 64);
+  const uint32_t one = 1;
+  put32(p, 0x00, &one);
+  put32(p, 0x04, &alpha);
+  put32(p, 0x08, &scaled);
+  put64(p, 0x10, cal_r);
+  put64(p, 0x18, cal_ir);
+  put64(p, 0x20, cal_sq);
+  if (cal_sq != 0.0) {
+    const float mr = (float)(cal_r / cal_sq), mi = (float)(cal_ir / cal_sq);
+    put32(p, 0x28, &mr);
+    put32(p, 0x2C, &mi);
+  }
+  put32(p, 0x30, &reg_den);
+  put32(p, 0x34, &reg_num);
+  put32(p, 0x38, &prm.completion_marker);
+  out->valid_pixels = valid;
+  out->fully_valid_tiles = tiles;
+  return DICE_OK;
+}
+
+}  // namespace
+
+// Scratch device buffers for the level-1 entry points.
+
+namespace {
+template <class F>
+int32_t with_device_buffers(dice_ctx* ctx, F&& body) {
+  if (int32_t e = activate(ctx)) return e;
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  return body();
+}
+
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T) + 16) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+
+extern "C" {
+
+int32_t dice_abi_version(void) { return DICE_ABI_VERSION; }
+
+const char* dice_last_error(void) { return g_error; }
+
+int32_t dice_probe(dice_device_info* info) {
+  memset(info, 0, sizeof *info);
+  snprintf(info->build_flags, sizeof info->build_flags, "%s", DICE_BUILD_FLAGS);
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1) {
+    snprintf(info->reason, sizeof info->reason, "HIP runtime reports no usable device: %s",
+             e == hipSuccess ? "device count 0" : hipGetErrorString(e));
+    return set_error(DICE_UNAVAILABLE, "%s", info->reason);
+  }
+  info->device_count = count;
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess)
+    return set_error(DICE_UNAVAILABLE, "hipGetDeviceProperties failed: %s", hipGetErrorString(e));
+  snprintf(info->name, sizeof info->name, "%s", prop.name);
+  snprintf(info->arch, sizeof info->arch, "%s", prop.gcnArchName);
+  info->compute_units = prop.multiProcessorCount;
+  size_t free_bytes = 0, total_bytes = 0;
+  (void)hipSetDevice(0);
+  e = hipMemGetInfo(&free_bytes, &total_bytes);
+  if (e != hipSuccess)
+    return set_error(DICE_UNAVAILABLE, "hipMemGetInfo failed: %s", hipGetErrorString(e));
+  info->total_hbm_bytes = total_bytes;
+  info->free_hbm_bytes = free_bytes;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    snprintf(info->reason, sizeof info->reason,
+             "device architecture %s is not gfx950 (this library is built for gfx950 only)",
+             prop.gcnArchName);
+    return set_error(DICE_UNAVAILABLE, "%s", info->reason);
+  }
+  return DICE_OK;
+}
+
+int32_t dice_ctx_create(int32_t device, uint64_t hbm_budget, dice_ctx** out) {
+  *out = nullptr;
+  if (int32_t e = require_tables()) return e;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return set_error(DICE_UNAVAILABLE, "HIP device %d is not available", device);
+  dice_ctx* ctx = new dice_ctx();
+  ctx->device = device;
+  ctx->hbm_budget = hbm_budget;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return set_error(DICE_UNAVAILABLE, "cannot create a HIP stream: %s", hipGetErrorString(e));
+  }
+  for (hipEvent_t& event : ctx->events) {
+    if (hipEventCreate(&event) != hipSuccess) {
+      dice_ctx_destroy(ctx);
+      return set_error(DICE_UNAVAILABLE, "cannot create HIP events");
+    }
+  }
+  if (int32_t rc = upload_tables(ctx)) {
+    dice_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return DICE_OK;
+}
+
+void dice_ctx_destroy(dice_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t event : ctx->events)
+    if (event) (void)hipEventDestroy(event);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int32_t dice_response_lut(float* lut) {
+  if (int32_t e = require_tables()) return e;
+  memcpy(lut, tables().lut.data(), 65536 * 4);
+  return DICE_OK;
+}
+
+int32_t dice_inverse_factors(uint32_t* high, uint32_t* low) {
+  if (int32_t e = require_tables()) return e;
+  memcpy(high, tables().high.data(), 256 * 4);
+  memcpy(low, tables().low.data(), 256 * 4);
+  return DICE_OK;
+}
+
+int32_t dice_prepass(const uint16_t* pixels, int64_t height, int64_t width,
+                     const float* lut, const dice_prepass_params* params,
+                     dice_prepass_result* out) {
+  return host_prepass(pixels, height, width, lut, *params, out);
+}
+
+int32_t dice_stage_table_from_schedule(const dice_schedule* schedule,
+                                       float auxiliary_factor_b, int64_t count,
+                                       float* aux_alpha, float* aux_offset,
+                                       float* aux_replacement,
+                                       uint8_t* aux_has_replacement, float* score_base,
+                                       float* coarse_reference, uint8_t* gate_secondary,
+                                       int32_t* row_gate) {
+  if (count < schedule->rows)
+    return set_error(DICE_INVALID, "producer hold cannot end before its source rows");
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t row = i < schedule->rows - 1 ? i : schedule->rows - 1;
+    const uint8_t* payload = schedule->payloads + row * 64;
+    float p1 = get32f(payload, 0x08), p2 = get32f(payload, 0x04);
+    const float p3 = get32f(payload, 0x28), p4 = get32f(payload, 0x2C);
+    if (!(std::isfinite(p1) && std::isfinite(p2) && std::isfinite(p3) && std::isfinite(p4)))
+      return set_error(DICE_INVALID, "X3A setter inputs must be finite float32");
+    if (p2 < 0.0f) {
+      p1 = -p1;
+      p2 = -p2;
+    }
+    const double d1 = 1.0 - (double)p1, d2 = 1.0 - (double)p2;
+    if (d1 == 0.0 || d2 == 0.0) return set_error(DICE_INVALID, "X3A setter scale cannot equal one");
+    const float base = (float)(((double)p4 - (double)p3 * (double)p2) / d2);
+    const float alpha = p2 * auxiliary_factor_b;
+    if (!(std::isfinite(alpha) && std::isfinite(base)))
+      return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
+    aux_alpha[i] = alpha;
+    aux_offset[i] = 1.0f;
+    aux_replacement[i] = base;
+    aux_has_replacement[i] = 1;
+    score_base[i] = base;
+    coarse_reference[i] = base;
+    gate_secondary[i] = !(p3 != 0.0f && p2 <= 0.075f);
+    row_gate[i] = 0;
+  }
+  return DICE_OK;
+}
+
+int32_t dice_producer_schedule(dice_ctx* ctx, const uint16_t* rgbi, int64_t height,
+                               int64_t width, const float* lut, dice_schedule* out) {
+  if (int32_t e = activate(ctx)) return e;
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  if (height <= 0) return set_error(DICE_INVALID, "producer record schedule cannot be empty");
+  if (width / 8 <= 0) return set_error(DICE_INVALID, "producer active width is outside the RGBI row");
+  if (lut && memcmp(lut, tables().lut.data(), 65536 * 4) != 0)
+    return set_error(DICE_INVALID, "response LUT is not the hash-pinned Nikon LUT");
+  JobDev J;
+  if (int32_t e = prepare_job(ctx, height, width, false, J)) return e;
+  if (int32_t e = bind_io(ctx, J, rgbi, nullptr, 0, nullptr)) return e;
+  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, ctx->stream));
+  HIP_TRY(launch_producer(J, 1.0f, ctx->stream));
+  ScheduleDev S;
+  const size_t H = (size_t)height;
+  const size_t sizes[12] = {8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+  for (int i = 0; i < 12; ++i) HIP_TRY(ctx->sched[i].ensure(H * sizes[i]));
+  S.weighted_visible_sum = ctx->sched[0].as<double>();
+  S.weighted_infrared_sum = ctx->sched[1].as<double>();
+  S.weight_sum = ctx->sched[2].as<double>();
+  S.mean_visible = ctx->sched[3].as<float>();
+  S.mean_infrared = ctx->sched[4].as<float>();
+  S.accepted = ctx->sched[5].as<uint32_t>();
+  S.scaled_input = ctx->sched[6].as<float>();
+  S.input_scale = ctx->sched[7].as<float>();
+  S.scale_denominator = ctx->sched[8].as<float>();
+  S.scale_numerator = ctx->sched[9].as<float>();
+  S.scale_add_denominator = ctx->sched[10].as<float>();
+  S.scale_add_numerator = ctx->sched[11].as<float>();
+  HIP_TRY(launch_schedule_rows(J, S, ctx->stream));
+  void* dst[12] = {out->weighted_visible_sum, out->weighted_infrared_sum, out->weight_sum,
+                   out->mean_visible,         out->mean_infrared,         out->accepted,
+                   out->scaled_input,         out->input_scale,           out->scale_denominator,
+                   out->scale_numerator,      out->scale_add_denominator, out->scale_add_numerator};
+  for (int i = 0; i < 12; ++i)
+    HIP_TRY(hipMemcpyAsync(dst[i], ctx->sched[i].ptr, H * sizes[i], hipMemcpyDeviceToHost,
+                           ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  out->rows = height;
+  out->active_width = (width / 8) * 8;
+  // producer_parameters.py:462-479: the 0x40-byte payload of every row.
+  for (size_t y = 0; y < H; ++y) {
+    uint8_t* p = out->payloads + y * 64;
+    memset(p, 0, 64);
+    put32(p, 0x04, &out->input_scale[y]);
+    put32(p, 0x08, &out->scaled_input[y]);
+    put64(p, 0x10, out->weighted_visible_sum[y]);
+    put64(p, 0x18, out->weighted_infrared_sum[y]);
+    put64(p, 0x20, out->weight_sum[y]);
+    put32(p, 0x28, &out->mean_visible[y]);
+    put32(p, 0x2C, &out->mean_infrared[y]);
+    put32(p, 0x30, &out->scale_denominator[y]);
+    put32(p, 0x34, &out->scale_numerator[y]);
+  }
+  return DICE_OK;
+}
+
+int32_t dice_replay(dice_ctx* ctx, const uint16_t* rgbi, int64_t height, int64_t width,
+                    const float* lut, const dice_params* params,
+                    const dice_stage_table* stages, uint32_t rng_state, uint32_t flags,
+                    uint16_t* out_rgb16, dice_replay_result* result,
+                    dice_diagnostics* diagnostics, const dice_callbacks* callbacks,
+                    dice_stage_timings* timings) {
+  if (int32_t e = activate(ctx)) return e;
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  memset(result, 0, sizeof *result);
+  if (timings) memset(timings, 0, sizeof *timings);
+  const double start = now_seconds();
+  Callbacks cb{callbacks};
+  if (int32_t e = validate_replay(height, width, *params)) return e;
+  if (rng_state > kLcgMask) return set_error(DICE_INVALID, "LCG24 state must fit in 24 bits");
+  if (stages->count < height + 6)
+    return set_error(DICE_INVALID, "stage table must cover stages 1..H+6");
+  if (lut && memcmp(lut, tables().lut.data(), 65536 * 4) != 0)
+    return set_error(DICE_INVALID, "response LUT is not the hash-pinned Nikon LUT");
+  const int64_t count = height + 6;
+  for (int64_t i = 0; i < count; ++i) {
+    if (!(std::isfinite(stages->aux_alpha[i]) && std::isfinite(stages->aux_offset[i]) &&
+          std::isfinite(stages->score_base[i]) && std::isfinite(stages->coarse_reference[i])))
+      return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
+    if (i < height && stages->aux_alpha[i] == 1.0f && !stages->aux_has_replacement[i])
+      return set_error(DICE_INVALID,
+                       "alpha==1 requires the runtime-configured auxiliary replacement");
+  }
+  JobDev J;
+  const bool want_diag = diagnostics && (diagnostics->score_plane || diagnostics->at_floor_mask);
+  if (int32_t e = prepare_job(ctx, height, width, want_diag, J)) return e;
+  if (int32_t e = bind_io(ctx, J, rgbi, out_rgb16, flags, timings)) return e;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.alpha, stages->aux_alpha, count * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.offset, stages->aux_offset, count * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.replacement, stages->aux_replacement, count * 4,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.has_replacement, stages->aux_has_replacement, count,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.score_base, stages->score_base, count * 4,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.coarse, stages->coarse_reference, count * 4,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.gate_secondary, stages->gate_secondary, count,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(J.stages.row_gate, stages->row_gate, count * 4,
+                         hipMemcpyHostToDevice, s));
+  int32_t rc = replay_core(ctx, J, *params, rng_state, flags, out_rgb16, result, diagnostics,
+                           cb, timings);
+  if (timings) timings->total = now_seconds() - start;
+  return rc;
+}
+
+int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t height,
+                         int64_t width, const uint16_t* prepass_rgbi,
+                         int64_t prepass_height, int64_t prepass_width,
+                         const dice_params* params,
+                         const dice_prepass_params* prepass_params, uint32_t flags,
+                         uint16_t* out_rgb16, dice_replay_result* result,
+                         dice_diagnostics* diagnostics, const dice_callbacks* callbacks,
+                         dice_stage_timings* timings) {
+  if (int32_t e = activate(ctx)) return e;
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  memset(result, 0, sizeof *result);
+  if (timings) memset(timings, 0, sizeof *timings);
+  const double start = now_seconds();
+  Callbacks cb{callbacks};
+  // engine.py:131-145 / profile.py:75-106 (shape part of validate_job).
+  cb.progress(0, 0, 1, 0, 0);
+  if (height < 5 || width < 8)
+    return set_error(DICE_INVALID, "main RGBI frame must be at least 5 rows by 8 columns");
+  if (int32_t e = validate_replay(height, width, *params)) return e;
+  if (cb.cancelled()) return cancelled_error();
+  cb.progress(0, 1, 1, 0, 0);
+
+  cb.progress(1, 0, 1, 0, 0);
+  double t0 = now_seconds();
+  dice_prepass_result prepass;
+  if (int32_t e = host_prepass(prepass_rgbi, prepass_height, prepass_width,
+                               tables().lut.data(), *prepass_params, &prepass))
+    return e;
+  // profile.py:48-55 and the parameter dataclasses' finiteness checks on
+  // the record-derived fields (replaced per row by the provider).
+  {
+    const float alpha = get32f(prepass.payload, 0x04);
+    const float mean_r = get32f(prepass.payload, 0x28);
+    const float mean_ir = get32f(prepass.payload, 0x2C);
+    const double denominator = 1.0 - (double)alpha;
+    if (denominator == 0.0) return set_error(DICE_INVALID, "prepass alpha cannot equal one");
+    const float base = (float)(((double)mean_ir - (double)alpha * (double)mean_r) / denominator);
+    if (!std::isfinite(alpha)) return set_error(DICE_INVALID, "alpha must be finite");
+    if (!std::isfinite(base)) return set_error(DICE_INVALID, "alpha_one_replacement must be finite");
+  }
+  if (timings) timings->prepass = now_seconds() - t0;
+  if (cb.cancelled()) return cancelled_error();
+  cb.progress(1, 1, 1, 0, 0);
+
+  cb.progress(2, 0, 1, 0, 0);
+  JobDev J;
+  const bool want_diag = diagnostics && (diagnostics->score_plane || diagnostics->at_floor_mask);
+  if (int32_t e = prepare_job(ctx, height, width, want_diag, J)) return e;
+  if (int32_t e = bind_io(ctx, J, main_rgbi, out_rgb16, flags, timings)) return e;
+  double t1 = now_seconds();
+  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, ctx->stream));
+  EventTimer ev{ctx, timings != nullptr};
+  if (int32_t e = ev.record(0)) return e;
+  HIP_TRY(launch_producer(J, 1.0f, ctx->stream));
+  if (int32_t e = ev.record(1)) return e;
+  if (timings) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    timings->producer_device = now_seconds() - t1;
+    timings->producer_kernels_ms = ev.elapsed_ms(0, 1);
+  }
+  if (cb.cancelled()) return cancelled_error();
+  cb.progress(2, 1, 1, 0, 0);
+  int32_t rc = replay_core(ctx, J, *params, 0x3045u, flags, out_rgb16, result, diagnostics,
+                           cb, timings);
+  if (rc == DICE_OK) cb.progress(4, 1, 1, 0, 0);
+  if (timings) timings->total = now_seconds() - start;
+  return rc;
+}
+
+
+int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches, int64_t count,
+                                 double* out3) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    Scratch S;
+    float* dp = S.alloc<float>(count * 81);
+    double* dq = S.alloc<double>(count * 3);
+    if (!dp || !dq) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(dp, patches, count * 81 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_t_unscaled_averages(dp, count, dq, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(out3, dq, count * 3 * 8, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
+int32_t dice_t_rgb_unscaled_averages(dice_ctx* ctx, const float* patches, int64_t count,
+                                     double* out9) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    Scratch S;
+    float* dp = S.alloc<float>(count * 243);
+    double* dq = S.alloc<double>(count * 9);
+    if (!dp || !dq) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(dp, patches, count * 243 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_t_rgb_unscaled_averages(dp, count, dq, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(out9, dq, count * 9 * 8, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
+int32_t dice_t_dither_chain(dice_ctx* ctx, const double* values, const float* scales,
+                            int64_t count, uint32_t state_in, float low, float high,
+                            double* deltas, uint32_t* state_out, uint64_t* advances_out) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    Scratch S;
+    double* dv = S.alloc<double>(count);
+    float* ds = S.alloc<float>(count);
+    double* dd = S.alloc<double>(count);
+    uint32_t* dst = S.alloc<uint32_t>(1);
+    unsigned long long* dadv = S.alloc<unsigned long long>(1);
+    if (!dv || !ds || !dd || !dst || !dadv) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(dv, values, count * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ds, scales, count * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_t_dither_chain(dv, ds, count, state_in, low, high, dd, dst, dadv,
+                                  ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(deltas, dd, count * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(state_out, dst, 4, hipMemcpyDeviceToHost));
+    unsigned long long adv = 0;
+    HIP_TRY(hipMemcpy(&adv, dadv, 8, hipMemcpyDeviceToHost));
+    *advances_out = adv;
+    return DICE_OK;
+  });
+}
+
+int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count, uint16_t* out) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    Scratch S;
+    float* dv = S.alloc<float>(count);
+    uint16_t* dout = S.alloc<uint16_t>(count);
+    uint32_t* dflags = S.alloc<uint32_t>(1);
+    if (!dv || !dout || !dflags) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(dv, values, count * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dflags, 0, 4));
+    HIP_TRY(launch_t_emit(dv, count, dout, ctx->high.as<uint32_t>(), ctx->low.as<uint32_t>(),
+                          dflags, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t flags = 0;
+    HIP_TRY(hipMemcpy(&flags, dflags, 4, hipMemcpyDeviceToHost));
+    if (flags & 2u) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
+    HIP_TRY(hipMemcpy(out, dout, count * 2, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
+int32_t dice_t_planes(dice_ctx* ctx, const uint16_t* rgbi, int64_t height, int64_t width,
+                      const float* lut, const dice_params* params, const float* row_alpha,
+                      const uint8_t* row_alpha_is_one, const float* row_replacement,
+                      const float* row_offset, const float* row_score_base, float* working,
+                      float* aux, float* score, float* waux, float* wrgb,
+                      uint32_t* error_flags) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    const size_t n = (size_t)height * width;
+    Scratch S;
+    uint16_t* drgbi = S.alloc<uint16_t>(n * 4);
+    float* dlut = S.alloc<float>(65536);
+    float* dalpha = S.alloc<float>(height);
+    uint8_t* dis_one = S.alloc<uint8_t>(height);
+    float* drepl = S.alloc<float>(height);
+    float* doff = S.alloc<float>(height);
+    float* dbase = S.alloc<float>(height);
+    float* dwork = S.alloc<float>(n * 4);
+    float* daux = S.alloc<float>(n);
+    float* dscore = S.alloc<float>(n);
+    float* dwaux = S.alloc<float>(n);
+    float* dwrgb = S.alloc<float>(n * 3);
+    uint32_t* dflags = S.alloc<uint32_t>(1);
+    if (!drgbi || !dlut || !dalpha || !dis_one || !drepl || !doff || !dbase || !dwork ||
+        !daux || !dscore || !dwaux || !dwrgb || !dflags)
+      return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(drgbi, rgbi, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dlut, lut, 65536 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dalpha, row_alpha, height * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dis_one, row_alpha_is_one, height, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(drepl, row_replacement, height * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(doff, row_offset, height * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dbase, row_score_base, height * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dflags, 0, 4));
+    DevParams P = device_params(*params);
+    HIP_TRY(launch_t_planes(drgbi, (int)height, (int)width, dlut, P, dalpha, dis_one, drepl,
+                            doff, dbase, dwork, daux, dscore, dwaux, dwrgb, dflags,
+                            ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(working, dwork, n * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(aux, daux, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(score, dscore, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(waux, dwaux, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(wrgb, dwrgb, n * 12, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(error_flags, dflags, 4, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
+int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height, int64_t width,
+                        const dice_params* params, uint8_t* fallback) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    const size_t n = (size_t)height * width;
+    Scratch S;
+    float* daux = S.alloc<float>(n);
+    uint8_t* dout = S.alloc<uint8_t>(n);
+    if (!daux || !dout) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(daux, aux, n * 4, hipMemcpyHostToDevice));
+    DevParams P = device_params(*params);
+    HIP_TRY(launch_t_decision(daux, (int)height, (int)width, P, dout, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(fallback, dout, n, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,1165 @@
+// dice_kernels.hip -- CDNA4 (gfx950) kernels of the Digital ICE hot path.
+//
+// Layout in HBM (per job): interleaved RGBI uint16 input (8 B/px), RGB
+// uint16 output (6 B/px); every intermediate plane of the reference
+// (working, aux, score, weighted aux/RGB) lives only in LDS tiles.  The
+// analysis kernel reads the input once per tile (plus its 9x9-window halo),
+// writes the no-op emit of every pixel and one 40-byte Site record per
+// attempted pixel; the serial writer runs on the host over those records.
+//
+// Reference semantics (rohanpandula/digital-fauxice,
+// src/portable_digital_ice/): streaming.py (row loop), x3a.py (planes,
+// decision), reconstruction.py (averages, candidates, combiner),
+// startup.py (hidden rows), producer_parameters.py + stage_parameters.py
+// (producer schedule and setter).
+#include <hip/hip_runtime.h>
+
+#include "dice_device.hpp"
+#include "dice_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace dice {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr float kProducerThreshold = 8847.2255859375f;  // producer_parameters.py:36
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Loads the four uint16 lanes of one pixel with one 64-bit load.
+__device__ __forceinline__ void load_pixel(const uint16_t* rgbi, size_t index,
+                                           uint32_t lanes[4]) {
+  uint2 q = *reinterpret_cast<const uint2*>(rgbi + index * 4);
+  lanes[0] = q.x & 0xFFFFu;
+  lanes[1] = q.x >> 16;
+  lanes[2] = q.y & 0xFFFFu;
+  lanes[3] = q.y >> 16;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of work items so neighbouring tiles share an L2.
+__device__ __forceinline__ int xcd_contiguous_index(int block, int blocks) {
+  int per = blocks / 8, rem = blocks % 8;
+  int xcd = block % 8, slot = block / 8;
+  int start = xcd * per + (xcd < rem ? xcd : rem);
+  return start + slot;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value,
+                                                         uint32_t* wave_sums,
+                                                         uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inclusive = value;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(inclusive, d, 64);
+    if (lane >= d) inclusive += t;
+  }
+  if (lane == 63) wave_sums[wave] = inclusive;
+  __syncthreads();
+  uint32_t offset = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    uint32_t s = wave_sums[w];
+    offset += w < wave ? s : 0u;
+    sum += s;
+  }
+  *total = sum;
+  __syncthreads();
+  return offset + inclusive - value;
+}
+
+// streaming.py:157-198: four nine-sample bars, edge-clamped by `aux_at`.
+template <class AuxAt>
+__device__ __forceinline__ bool decision_fallback(const AuxAt& aux_at,
+                                                  float threshold, int radius,
+                                                  int limit) {
+  int vertical_left = 0, vertical_right = 0, above = 0, below = 0;
+#pragma unroll
+  for (int d = -4; d <= 4; ++d) {
+    vertical_left += aux_at(d, -radius) < threshold;
+    vertical_right += aux_at(d, radius) < threshold;
+    above += aux_at(-radius, d) < threshold;
+    below += aux_at(radius, d) < threshold;
+  }
+  return vertical_left > limit || above > limit || vertical_right > limit ||
+         below > limit;
+}
+
+// reconstruction.py:425-462: C69/C21/C16 with the explicit float32 W.
+template <class WrgbAt>
+__device__ __forceinline__ void rgb_candidates(const WrgbAt& at,
+                                               const float weights[4],
+                                               float out[3][3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    auto lane = [&](int dy, int dx) { return at(dy, dx, k); };
+    double q[3];
+    unscaled_averages(lane, q);
+    q[2] = rgb_q16(lane);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      if (weights[s] > 0.0f) {
+        double reciprocal = 1.0 / (double)weights[s];
+        out[s][k] = (float)(q[s] * reciprocal);
+      } else {
+        out[s][k] = (float)q[s];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ==========================================================================
+// Producer (producer_parameters.py:266-495)
+// ==========================================================================
+
+// First failing pixel (IR <= threshold) of every 8x8 (block, cell) in
+// row-major order, 64 when none fails: the cell-validity latch.
+__global__ __launch_bounds__(kThreads) void k_failpos(JobDev J) {
+  const int cells = J.width / 8;
+  const int blocks = (J.height + 7) / 8;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)blocks * cells) return;
+  const int b = (int)(idx / cells), c = (int)(idx % cells);
+  const double threshold = (double)kProducerThreshold;
+  int fail = 64;
+  for (int r = 0; r < 8 && fail == 64; ++r) {
+    int y = b * 8 + r;
+    if (y >= J.height) break;
+    for (int k = 0; k < 8; ++k) {
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)y * J.width + c * 8 + k, lanes);
+      if (!(threshold < (double)lanes[3])) {
+        fail = r * 8 + k;
+        break;
+      }
+    }
+  }
+  J.failpos[idx] = (uint8_t)fail;
+}
+
+// Per-row column-ordered binary64 sums (one lane per row, rows staged
+// through LDS for coalesced loads).
+constexpr int kRowSumRows = 64;
+constexpr int kRowSumCols = 32;
+
+__global__ __launch_bounds__(64) void k_row_sums(JobDev J) {
+  __shared__ uint32_t tile[kRowSumRows][kRowSumCols + 1];
+  const int lane = threadIdx.x;
+  const int row0 = blockIdx.x * kRowSumRows;
+  const int y = row0 + lane;
+  const int cells = J.width / 8;
+  const int active = cells * 8;
+  double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
+  uint32_t accepted = 0;
+  const uint8_t* fail_row = J.failpos + (size_t)(y / 8) * cells;
+  const int cell_row = (y % 8) * 8;
+  for (int c0 = 0; c0 < active; c0 += kRowSumCols) {
+    for (int k = 0; k < kRowSumRows * kRowSumCols / 64; ++k) {
+      int e = k * 64 + lane;
+      int r = e / kRowSumCols, c = e % kRowSumCols;
+      int yy = row0 + r, xx = c0 + c;
+      uint32_t v = 0;
+      if (yy < J.height && xx < active) {
+        uint32_t lanes[4];
+        load_pixel(J.rgbi, (size_t)yy * J.width + xx, lanes);
+        v = lanes[0] | (lanes[3] << 16);
+      }
+      tile[r][c] = v;
+    }
+    __syncthreads();
+    if (y < J.height) {
+      const int ncols = min(kRowSumCols, active - c0);
+      for (int c = 0; c < ncols; ++c) {
+        const int x = c0 + c;
+        if (cell_row + (x & 7) >= (int)fail_row[x >> 3]) continue;
+        const uint32_t v = tile[lane][c];
+        const uint32_t red = v & 0xFFFFu, infrared = v >> 16;
+        const double weight = (double)((int64_t)infrared * (int64_t)infrared);
+        sum_visible += weight * (double)J.lut[red];
+        sum_infrared += weight * (double)J.lut[infrared];
+        sum_weight += weight;
+        accepted += 1;
+      }
+    }
+    __syncthreads();
+  }
+  if (y < J.height) {
+    J.row_sums[(size_t)y * 3 + 0] = sum_visible;
+    J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
+    J.row_sums[(size_t)y * 3 + 2] = sum_weight;
+    J.accepted[y] = accepted;
+  }
+}
+
+namespace {
+// producer_parameters.py:136-167: the asymmetric 4x4 tree per quadrant.
+__device__ __forceinline__ void producer_quadrants(const double v[8][8],
+                                                   float out[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r0 = (q >> 1) * 4, c0 = (q & 1) * 4;
+    double first = v[r0][c0];
+    first += v[r0][c0 + 1];
+    first += v[r0][c0 + 2];
+    first += v[r0][c0 + 3];
+    first += v[r0 + 1][c0];
+    first += v[r0 + 1][c0 + 1];
+    double second_tail = v[r0 + 1][c0 + 2] + v[r0 + 1][c0 + 3];
+    first += second_tail;
+    double third = v[r0 + 2][c0] + v[r0 + 2][c0 + 1];
+    double third_tail = v[r0 + 2][c0 + 2] + v[r0 + 2][c0 + 3];
+    third += third_tail;
+    double fourth = v[r0 + 3][c0] + v[r0 + 3][c0 + 1];
+    double fourth_tail = v[r0 + 3][c0 + 2] + v[r0 + 3][c0 + 3];
+    fourth += fourth_tail;
+    third += fourth;
+    double total = first + third;
+    out[q] = (float)(total * 0.0625);
+  }
+}
+
+// producer_parameters.py:170-180
+__device__ __forceinline__ void producer_center(const float m[4], float out[4]) {
+  double total = (double)m[0];
+  total += (double)m[1];
+  total += (double)m[2];
+  total += (double)m[3];
+  float mean = (float)(total * 0.25);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = (float)((double)m[q] - (double)mean);
+}
+}  // namespace
+
+// producer_parameters.py:209-262: the four (term, ratio*term) pairs of every
+// whole 8x8 block of every complete epoch; skipped blocks are marked by
+// failpos and contribute nothing.
+__global__ __launch_bounds__(kThreads) void k_epoch_terms(JobDev J) {
+  const int cells = J.width / 8;
+  const int epochs = J.height / 8;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)epochs * cells) return;
+  const int e = (int)(idx / cells), c = (int)(idx % cells);
+  if (J.failpos[(size_t)e * cells + c] != 64) return;
+  double visible[8][8], infrared[8][8];
+  float raw_sum = 0.0f;
+  for (int r = 0; r < 8; ++r)
+    for (int k = 0; k < 8; ++k) {
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)(e * 8 + r) * J.width + c * 8 + k, lanes);
+      visible[r][k] = (double)J.lut[lanes[0]];
+      infrared[r][k] = (double)J.lut[lanes[3]];
+      raw_sum = (float)((double)raw_sum + (double)lanes[3]);
+    }
+  float vm[4], im[4], vd[4], id[4];
+  producer_quadrants(visible, vm);
+  producer_quadrants(infrared, im);
+  producer_center(vm, vd);
+  producer_center(im, id);
+  const double low = (double)-0.125f, full_low = (double)0.0f;
+  const double full_high = (double)0.3f, high = (double)0.425f;
+  const double outer_weight = (double)0.70710677f;
+  const double raw_wide = (double)raw_sum;
+  double* out = J.epoch_terms + idx * 8;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double visible_wide = (double)vd[q];
+    double ratio, weight;
+    if (visible_wide == 0.0) {
+      ratio = 0.0;
+      weight = 0.0;
+    } else {
+      ratio = (double)id[q] / visible_wide;
+      if (ratio < low || ratio > high)
+        weight = 0.0;
+      else if (ratio < full_low || ratio > full_high)
+        weight = visible_wide * outer_weight;
+      else
+        weight = visible_wide;
+    }
+    double term = weight * weight;
+    term *= raw_wide;
+    term *= raw_wide;
+    out[q * 2 + 0] = term;
+    out[q * 2 + 1] = ratio * term;
+  }
+}
+
+// Per-epoch sums in block then quadrant order (one lane per epoch).
+__global__ __launch_bounds__(kThreads) void k_epoch_sums(JobDev J) {
+  const int cells = J.width / 8;
+  const int epochs = J.height / 8;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= epochs) return;
+  double denominator = 0.0, numerator = 0.0;
+  for (int c = 0; c < cells; ++c) {
+    if (J.failpos[(size_t)e * cells + c] != 64) continue;
+    const double* t = J.epoch_terms + ((size_t)e * cells + c) * 8;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      denominator += t[q * 2 + 0];
+      numerator += t[q * 2 + 1];
+    }
+  }
+  J.epoch_adds[e * 2 + 0] = (double)(float)denominator;
+  J.epoch_adds[e * 2 + 1] = (double)(float)numerator;
+}
+
+// The strictly sequential cross-row chains: cumulative row sums (lanes
+// 0..2 of wave 0) and the epoch accumulators (lane 0 of wave 1).
+__global__ __launch_bounds__(128) void k_producer_chain(JobDev J) {
+  const int t = threadIdx.x;
+  if (t < 3) {
+    double total = 0.0;
+    for (int y = 0; y < J.height; ++y) {
+      total += J.row_sums[(size_t)y * 3 + t];
+      J.cumulative[(size_t)y * 3 + t] = total;
+    }
+  } else if (t == 64) {
+    const int epochs = J.height / 8;
+    float denominator = 0.0f, numerator = 0.0f;
+    for (int e = 0; e < epochs; ++e) {
+      double denominator_wide = (double)denominator + J.epoch_adds[e * 2 + 0];
+      double numerator_wide = (double)numerator + J.epoch_adds[e * 2 + 1];
+      denominator = (float)denominator_wide;
+      numerator = (float)numerator_wide;
+      J.epoch_wide[e * 2 + 0] = denominator_wide;
+      J.epoch_wide[e * 2 + 1] = numerator_wide;
+      J.epoch_store[e * 2 + 0] = denominator;
+      J.epoch_store[e * 2 + 1] = numerator;
+    }
+  }
+}
+
+namespace {
+struct RowRecord {
+  float mean_visible, mean_infrared, p1, p2, denominator, numerator;
+};
+
+// The record fields of row y after the chains (producer_parameters.py:
+// 358-360, 437-460): means update whenever the cumulative weight is
+// nonzero; p1/p2 update at every epoch end with a nonzero denominator.
+__device__ __forceinline__ RowRecord row_record(const JobDev& J, int y) {
+  RowRecord r;
+  const double total_weight = J.cumulative[(size_t)y * 3 + 2];
+  if (total_weight != 0.0) {
+    r.mean_visible = (float)(J.cumulative[(size_t)y * 3 + 0] / total_weight);
+    r.mean_infrared = (float)(J.cumulative[(size_t)y * 3 + 1] / total_weight);
+  } else {
+    r.mean_visible = 46618.78515625f;
+    r.mean_infrared = 56239.65625f;
+  }
+  r.p1 = 0.15f;
+  r.p2 = 0.15f;
+  r.denominator = 0.0f;
+  r.numerator = 0.0f;
+  const int last_epoch = (y + 1) / 8 - 1;
+  if (last_epoch >= 0) {
+    r.denominator = J.epoch_store[last_epoch * 2 + 0];
+    r.numerator = J.epoch_store[last_epoch * 2 + 1];
+    const double denominator_wide = J.epoch_wide[last_epoch * 2 + 0];
+    if (denominator_wide != 0.0) {
+      const double ratio = J.epoch_wide[last_epoch * 2 + 1] / denominator_wide;
+      r.p2 = (float)ratio;
+      r.p1 = (float)((double)1.5f * ratio);
+    }
+  }
+  return r;
+}
+}  // namespace
+
+__global__ __launch_bounds__(kThreads) void k_schedule_rows(JobDev J,
+                                                            ScheduleDev S) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= J.height) return;
+  RowRecord r = row_record(J, y);
+  S.weighted_visible_sum[y] = J.cumulative[(size_t)y * 3 + 0];
+  S.weighted_infrared_sum[y] = J.cumulative[(size_t)y * 3 + 1];
+  S.weight_sum[y] = J.cumulative[(size_t)y * 3 + 2];
+  S.mean_visible[y] = r.mean_visible;
+  S.mean_infrared[y] = r.mean_infrared;
+  S.accepted[y] = J.accepted[y];
+  S.scaled_input[y] = r.p1;
+  S.input_scale[y] = r.p2;
+  S.scale_denominator[y] = r.denominator;
+  S.scale_numerator[y] = r.numerator;
+  const bool epoch_end = (y % 8) == 7;
+  S.scale_add_denominator[y] = epoch_end ? (float)J.epoch_adds[(y / 8) * 2 + 0] : 0.0f;
+  S.scale_add_numerator[y] = epoch_end ? (float)J.epoch_adds[(y / 8) * 2 + 1] : 0.0f;
+}
+
+// ContentDerivedStageParameterProvider (producer_parameters.py:498-541) +
+// derive_x3a_setter_fields / StageCalibration (stage_parameters.py:93-197):
+// stage s reads record row min(s-1, H-1).
+__global__ __launch_bounds__(kThreads) void k_stage_table(JobDev J,
+                                                          float factor_b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int count = J.height + 6;
+  if (i >= count) return;
+  const int row = i < J.height - 1 ? i : J.height - 1;
+  RowRecord r = row_record(J, row);
+  float p1 = r.p1, p2 = r.p2, p3 = r.mean_visible, p4 = r.mean_infrared;
+  uint32_t error = 0;
+  if (!(__builtin_isfinite(p1) && __builtin_isfinite(p2) &&
+        __builtin_isfinite(p3) && __builtin_isfinite(p4)))
+    error |= kErrStageNonfinite;
+  if (p2 < 0.0f) {
+    p1 = -p1;
+    p2 = -p2;
+  }
+  const double denominator_p1 = 1.0 - (double)p1;
+  const double denominator_p2 = 1.0 - (double)p2;
+  if (denominator_p1 == 0.0 || denominator_p2 == 0.0) error |= kErrSetterScaleOne;
+  const float base = (float)(((double)p4 - (double)p3 * (double)p2) / denominator_p2);
+  const bool gate = !(p3 != 0.0f && p2 <= 0.075f);
+  const float alpha = p2 * factor_b;
+  if (!(__builtin_isfinite(alpha) && __builtin_isfinite(base))) error |= kErrStageNonfinite;
+  if (error) atomicOr(J.error_flags, error);
+  J.stages.alpha[i] = alpha;
+  J.stages.offset[i] = 1.0f;
+  J.stages.replacement[i] = base;
+  J.stages.has_replacement[i] = 1;
+  J.stages.score_base[i] = base;
+  J.stages.coarse[i] = base;
+  J.stages.gate_secondary[i] = gate ? 1 : 0;
+  J.stages.row_gate[i] = 0;
+}
+
+// ==========================================================================
+// Analysis, in two tile kernels over the same TY x TX tiling:
+//   k_scan   (all tiles, light): aux plane on the edge-clamped halo, the
+//            centre scores, decision + eligibility, no-op emit of every
+//            pixel, and a row-major list of the tile's sites.
+//   k_sites  (tiles with sites, f64-heavy): the full history halo in LDS
+//            (score, weighted aux, weighted RGB) and, per site, the W/A
+//            records, candidates and combiner; each Site record goes
+//            straight to its row-major position in the compacted array.
+// ==========================================================================
+
+// Row parameters of the halo rows: aux stage y+1 (index y), score stage y+2
+// (index y+1), writer stage y+7 (index y+6) -- stage_parameters.py:207-232.
+struct RowParams {
+  float alpha, offset, replacement, score_base;
+  uint8_t is_one;
+};
+
+__device__ __forceinline__ RowParams load_row_params(const JobDev& J, int sy) {
+  RowParams p;
+  p.alpha = J.stages.alpha[sy];
+  p.is_one = p.alpha == 1.0f;
+  p.offset = J.stages.offset[sy];
+  p.replacement = J.stages.replacement[sy];
+  p.score_base = J.stages.score_base[sy + 1];
+  return p;
+}
+
+template <int TY, int TX>
+struct ScanShared {
+  static constexpr int AR = TY + 8, AC = TX + 8;  // aux, edge-clamped, +-4
+  float aux[AR][AC];
+  RowParams rows[AR];
+  uint8_t writer_floor[TY];
+  int32_t writer_gate[TY];
+  uint32_t row_count[TY];
+  uint32_t wave_sums[kThreads / 64];
+};
+
+template <int TY, int TX>
+__global__ __launch_bounds__(kThreads) void k_scan(JobDev J, DevParams P) {
+  using Sh = ScanShared<TY, TX>;
+  constexpr int AR = Sh::AR, AC = Sh::AC;
+  static_assert((TY * TX) % kThreads == 0, "tile must be a multiple of the block");
+  __shared__ Sh S;
+  const int tiles = J.tiles_x * J.tiles_y;
+  const int t = xcd_contiguous_index(blockIdx.x, gridDim.x);
+  if (t >= tiles) return;
+  const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
+  const int H = J.height, W = J.width;
+  const int y0 = tile_y * TY, x0 = tile_x * TX;
+  const int tid = threadIdx.x;
+  const float floor = P.score_floor;
+
+  for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
+  for (int i = tid; i < TY; i += kThreads) {
+    const int y = min(y0 + i, H - 1);
+    S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
+    S.writer_gate[i] = J.stages.row_gate[y + 6];
+    S.row_count[i] = 0;
+  }
+  __syncthreads();
+
+  // x3a.py:193-221 on the clamped halo; no-op emit (output.py:76-94) of the
+  // centres.
+  uint32_t aux_error = 0;
+  for (int i = tid; i < AR * AC; i += kThreads) {
+    const int r = i / AC, c = i % AC;
+    const int ly = y0 - 4 + r, lx = x0 - 4 + c;
+    uint32_t lanes[4];
+    load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
+    const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
+    const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
+    const float visible =
+        P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
+    const RowParams& rp = S.rows[r];
+    const float a = rp.is_one ? rp.replacement : auxiliary(visible, work3, rp.alpha, rp.offset);
+    if (!__builtin_isfinite(a)) aux_error = kErrNonfiniteAux;
+    S.aux[r][c] = a;
+    const int ty = r - 4, tx = c - 4;
+    if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
+      uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
+      o[0] = emit_value(work0, J.factor_high, J.factor_low);
+      o[1] = emit_value(work1, J.factor_high, J.factor_low);
+      o[2] = emit_value(work2, J.factor_high, J.factor_low);
+    }
+  }
+  if (aux_error) atomicOr(J.error_flags, aux_error);
+  __syncthreads();
+
+  // Centre score (x3a.py:245-286), decision bars (streaming.py:157-198) and
+  // eligibility (streaming.py:461-466); each thread owns a contiguous run of
+  // centres so the queue comes out row-major.
+  constexpr int kPer = TY * TX / kThreads;
+  uint32_t eligible_bits = 0, eligible_count = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int li = tid * kPer + k;
+    const int ty = li / TX, tx = li % TX;
+    const int y = y0 + ty, x = x0 + tx;
+    if (y >= H || x >= W) continue;
+    const int ar = ty + 4, ac = tx + 4;
+    float sample = S.aux[ar][ac];
+    if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
+      sample = fmin_plain(fmin_plain(S.aux[ar][ac - 1], S.aux[ar][ac]), S.aux[ar][ac + 1]);
+    const float sc = score_value(sample, S.rows[ar].score_base, P.score_addend,
+                                 P.score_scale, P.score_offset, floor);
+    if (J.diag_score) {
+      J.diag_score[(size_t)y * W + x] = sc;
+      J.diag_floor[(size_t)y * W + x] = sc == floor ? 1 : 0;
+    }
+    if (S.writer_gate[ty] != 0) continue;
+    if (S.writer_floor[ty] && sc >= 1.0f) continue;
+    auto aux_at = [&](int dy, int dx) { return S.aux[ar + dy][ac + dx]; };
+    if (decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit)) continue;
+    eligible_bits |= 1u << k;
+    eligible_count += 1;
+  }
+  uint32_t total;
+  uint32_t base = block_exclusive_scan(eligible_count, S.wave_sums, &total);
+  uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (eligible_bits & (1u << k)) {
+      const int li = tid * kPer + k;
+      list[base++] = (uint16_t)li;
+      atomicAdd(&S.row_count[li / TX], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < TY; i += kThreads)
+    J.seg_count[(size_t)(y0 + i) * J.tiles_x + tile_x] = S.row_count[i];
+  if (tid == 0) J.tile_count[t] = total;
+}
+
+template <int TY, int TX>
+struct SitesShared {
+  static constexpr int AR = TY + 10, AC = TX + 12;  // aux, edge-clamped, +-5/+-6
+  static constexpr int HR = TY + 10, HC = TX + 10;  // score / waux history, +-5
+  static constexpr int RR = TY + 8, RC = TX + 8;    // weighted RGB history, +-4
+  float aux[AR][AC];
+  float score[HR][HC];
+  float waux[HR][HC];
+  float wrgb[3][RR][RC];
+  RowParams rows[AR];
+  float writer_coarse[TY];
+  uint8_t writer_floor[TY];
+  uint32_t row_start[TY];
+  uint32_t row_offset[TY];
+};
+
+template <int TY, int TX>
+__global__ __launch_bounds__(kThreads) void k_sites(JobDev J, DevParams P) {
+  using Sh = SitesShared<TY, TX>;
+  constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
+  constexpr int RR = Sh::RR, RC = Sh::RC;
+  __shared__ Sh S;
+  const int tiles = J.tiles_x * J.tiles_y;
+  const int t = xcd_contiguous_index(blockIdx.x, gridDim.x);
+  if (t >= tiles) return;
+  const uint32_t total = J.tile_count[t];
+  if (total == 0) return;
+  const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
+  const int H = J.height, W = J.width;
+  const int y0 = tile_y * TY, x0 = tile_x * TX;
+  const int tid = threadIdx.x;
+  const float floor = P.score_floor;
+
+  for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
+  if (tid == 0) {
+    uint32_t start = 0;
+    for (int i = 0; i < TY; ++i) {
+      const size_t seg = (size_t)(y0 + i) * J.tiles_x + tile_x;
+      const int y = min(y0 + i, H - 1);
+      S.row_start[i] = start;
+      S.row_offset[i] = J.seg_offset[seg];
+      start += J.seg_count[seg];
+      S.writer_coarse[i] = J.stages.coarse[y + 6];
+      S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+
+  // Working lanes and aux on the clamped halo; working RGB staged for the
+  // weighted-RGB history.
+  for (int i = tid; i < AR * AC; i += kThreads) {
+    const int r = i / AC, c = i % AC;
+    const int ly = y0 - 5 + r, lx = x0 - 6 + c;
+    uint32_t lanes[4];
+    load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
+    const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
+    const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
+    const float visible =
+        P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
+    const RowParams& rp = S.rows[r];
+    S.aux[r][c] = rp.is_one ? rp.replacement : auxiliary(visible, work3, rp.alpha, rp.offset);
+    const int rr = r - 1, rc = c - 2;
+    if (rr >= 0 && rr < RR && rc >= 0 && rc < RC) {
+      S.wrgb[0][rr][rc] = work0;
+      S.wrgb[1][rr][rc] = work1;
+      S.wrgb[2][rr][rc] = work2;
+    }
+  }
+  __syncthreads();
+
+  // Score / weighted-aux history with the asymmetric guards
+  // (streaming.py:201-233): zero outside the row and above row -1, the
+  // floor pseudo-row at -1, the last row repeated below.
+  for (int i = tid; i < HR * HC; i += kThreads) {
+    const int r = i / HC, c = i % HC;
+    const int ly = y0 - 5 + r, lx = x0 - 5 + c;
+    float s, wa;
+    if (lx < 0 || lx >= W || ly < -1) {
+      s = 0.0f;
+      wa = 0.0f;
+    } else if (ly == -1) {
+      s = floor;
+      wa = S.aux[r][c + 1] * floor;
+    } else {
+      const float a = S.aux[r][c + 1];
+      float sample = a;
+      if (P.horizontal_minimum && W > 2 && lx >= 1 && lx <= W - 2)
+        sample = fmin_plain(fmin_plain(S.aux[r][c], a), S.aux[r][c + 2]);
+      s = score_value(sample, S.rows[r].score_base, P.score_addend, P.score_scale,
+                      P.score_offset, floor);
+      wa = s * a;
+    }
+    S.score[r][c] = s;
+    S.waux[r][c] = wa;
+  }
+  __syncthreads();
+  for (int i = tid; i < RR * RC; i += kThreads) {
+    const int rr = i / RC, rc = i % RC;
+    const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
+    if (lx < 0 || lx >= W || ly < -1) {
+      S.wrgb[0][rr][rc] = 0.0f;
+      S.wrgb[1][rr][rc] = 0.0f;
+      S.wrgb[2][rr][rc] = 0.0f;
+    } else {
+      const float s = S.score[rr + 1][rc + 1];
+      S.wrgb[0][rr][rc] = s * S.wrgb[0][rr][rc];
+      S.wrgb[1][rr][rc] = s * S.wrgb[1][rr][rc];
+      S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
+    }
+  }
+  __syncthreads();
+
+  // One site per thread.  Eligibility already implies the driver gate
+  // passes (reconstruction.py:890-904 re-tests the same row gate and the
+  // same point score), so every listed centre is attempted.
+  const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+  for (uint32_t q = tid; q < total; q += kThreads) {
+    const int li = list[q];
+    const int ty = li / TX, tx = li % TX;
+    const int y = y0 + ty, x = x0 + tx;
+    const int cr = ty + 5, cc = tx + 5;  // history index of the centre
+    const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
+    const float fallback = S.writer_coarse[ty];
+    auto record = [&](int r, int c, float point_aux, float point_score, float w[4],
+                      float f[4]) {
+      auto score_at = [&](int dy, int dx) { return S.score[r - 4 + dy][c - 4 + dx]; };
+      auto waux_at = [&](int dy, int dx) { return S.waux[r - 4 + dy][c - 4 + dx]; };
+      feature_record(score_at, waux_at, point_aux, point_score, fallback, w, f);
+    };
+    float weights[4], features[5][4];
+    record(cr, cc, S.aux[ar][ac], S.score[cr][cc], weights, features[0]);
+    int record_count = 1;
+    if (P.cross_neighbor) {
+      // streaming.py:259-308: neighbours share the centre row's fallback;
+      // out-of-row neighbours are all-zero records; rows -1 and H take the
+      // pseudo-row / repeated-row point values already in the halo.
+      record_count = 5;
+      float unused[4];
+      if (x > 0) {
+        record(cr, cc - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], unused, features[1]);
+      } else {
+        features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+      }
+      if (x < W - 1) {
+        record(cr, cc + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], unused, features[2]);
+      } else {
+        features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
+      }
+      record(cr - 1, cc, S.aux[ar - 1][ac], S.score[cr - 1][cc], unused, features[3]);
+      record(cr + 1, cc, S.aux[ar + 1][ac], S.score[cr + 1][cc], unused, features[4]);
+    }
+    auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
+    float candidates[3][3];
+    rgb_candidates(wrgb_at, weights, candidates);
+    uint32_t lanes[4];
+    load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+    const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
+    Site site;
+    combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
+                      features, record_count, fallback, P.combine, site.cand);
+    site.orig[0] = original[0];
+    site.orig[1] = original[1];
+    site.orig[2] = original[2];
+    site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
+    J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
+  }
+}
+
+// Exclusive scan of the (row, tile-column) segment counts in row-major
+// order: the site order the serial writer requires (host_writer.py:73-93).
+__global__ __launch_bounds__(1024) void k_scan_segments(JobDev J) {
+  __shared__ uint32_t partial[1024];
+  const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+  const int t = threadIdx.x;
+  const long chunk = (n + 1023) / 1024;
+  const long begin = t * chunk;
+  const long end = min(n, begin + chunk);
+  uint32_t sum = 0;
+  for (long i = begin; i < end; ++i) sum += J.seg_count[i];
+  partial[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    uint32_t v = t >= d ? partial[t - d] : 0u;
+    __syncthreads();
+    partial[t] += v;
+    __syncthreads();
+  }
+  uint32_t offset = partial[t] - sum;
+  for (long i = begin; i < end; ++i) {
+    J.seg_offset[i] = offset;
+    offset += J.seg_count[i];
+  }
+  if (t == 1023) J.counters[0] = partial[1023];
+}
+
+// ==========================================================================
+// Hidden startup (startup.py:74-353)
+// ==========================================================================
+
+// Planes of the first rows (5 in cross-neighbour mode) for the six hidden
+// writer stages.
+__global__ __launch_bounds__(kThreads) void k_startup_rows(JobDev J, DevParams P,
+                                                           int rows) {
+  const int W = J.width;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * W) return;
+  const int y = i / W, x = i % W;
+  const float alpha = J.stages.alpha[y];
+  const bool is_one = alpha == 1.0f;
+  float aux3[3];
+#pragma unroll
+  for (int d = -1; d <= 1; ++d) {
+    const int xx = clampi(x + d, 0, W - 1);
+    uint32_t lanes[4];
+    load_pixel(J.rgbi, (size_t)y * W + xx, lanes);
+    const float visible = J.lut[lanes[P.visible_channel]];
+    aux3[d + 1] = is_one ? J.stages.replacement[y]
+                         : auxiliary(visible, J.lut[lanes[3]], alpha, J.stages.offset[y]);
+  }
+  float sample = aux3[1];
+  if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
+    sample = fmin_plain(fmin_plain(aux3[0], aux3[1]), aux3[2]);
+  const float s = score_value(sample, J.stages.score_base[y + 1], P.score_addend,
+                              P.score_scale, P.score_offset, P.score_floor);
+  uint32_t lanes[4];
+  load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+  J.su_aux[i] = aux3[1];
+  J.su_score[i] = s;
+  J.su_work[(size_t)i * 3 + 0] = J.lut[lanes[0]];
+  J.su_work[(size_t)i * 3 + 1] = J.lut[lanes[1]];
+  J.su_work[(size_t)i * 3 + 2] = J.lut[lanes[2]];
+}
+
+namespace {
+// _startup_histories (startup.py:74-128) as point accessors.
+struct StartupView {
+  const JobDev& J;
+  int W;
+  float floor;
+  __device__ float score(int y, int x) const {
+    if (x < 0 || x >= W || y < -1) return 0.0f;
+    if (y == -1) return floor;
+    return J.su_score[y * W + x];
+  }
+  __device__ float waux(int y, int x) const {
+    if (x < 0 || x >= W || y < -1) return 0.0f;
+    if (y == -1) return J.su_aux[x] * floor;
+    return J.su_score[y * W + x] * J.su_aux[y * W + x];
+  }
+  __device__ float wrgb(int y, int x, int k) const {
+    if (x < 0 || x >= W || y < -1) return 0.0f;
+    if (y == -1) return J.su_work[x * 3 + k] * floor;
+    return J.su_score[y * W + x] * J.su_work[(y * W + x) * 3 + k];
+  }
+  // Raw decision history: first real row for y < 0; left guard replicates;
+  // the right guard is replicated for real rows and zero for y < 0.
+  __device__ float raw(int y, int x) const {
+    const float* source = J.su_aux + (y < 0 ? 0 : y) * W;
+    if (x < 0) return source[0];
+    if (x >= W) return y >= 0 ? source[W - 1] : 0.0f;
+    return source[x];
+  }
+  // startup.py:131-183: rows at or below the first hidden centre and
+  // out-of-row columns hold allocator zero.
+  __device__ void record(int y, int x, float fallback, float w[4], float f[4]) const {
+    if (y <= -6 || x < 0 || x >= W) {
+      w[0] = w[1] = w[2] = w[3] = 0.0f;
+      f[0] = f[1] = f[2] = f[3] = 0.0f;
+      return;
+    }
+    auto score_at = [&](int dy, int dx) { return score(y - 4 + dy, x - 4 + dx); };
+    auto waux_at = [&](int dy, int dx) { return waux(y - 4 + dy, x - 4 + dx); };
+    const float point_score = score(y, x);
+    feature_record(score_at, waux_at, raw(y, x), point_score, fallback, w, f);
+  }
+};
+}  // namespace
+
+__global__ __launch_bounds__(kThreads) void k_startup_sites(JobDev J, DevParams P) {
+  const int W = J.width;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * W) return;
+  const int stage = i / W + 1, x = i % W;
+  const int center_y = stage - 7;
+  const bool cross = P.cross_neighbor != 0;
+  const int writer_width = (cross && stage <= 2) ? W - 4 : W;
+  J.su_attempted[i] = 0;
+  if (x >= writer_width) return;
+  StartupView V{J, W, P.score_floor};
+  const StageDev& st = J.stages;
+  const bool floor_enabled = P.gate_primary || st.gate_secondary[stage - 1];
+  const int32_t row_gate = st.row_gate[stage - 1];
+  const float combiner_reference = st.coarse[stage - 1];
+  const float fallback_center = st.coarse[max(1, center_y + 6) - 1];
+  const float fallback_up = st.coarse[max(1, center_y + 5) - 1];
+  const float fallback_down = st.coarse[max(1, center_y + 7) - 1];
+  auto aux_at = [&](int dy, int dx) { return V.raw(center_y + dy, x + dx); };
+  const bool fallback = decision_fallback(aux_at, P.decision_threshold, P.radius,
+                                          P.count_limit);
+  float weights[4], features[5][4];
+  V.record(center_y, x, fallback_center, weights, features[0]);
+  if (fallback || row_gate != 0 || (floor_enabled && weights[3] >= 1.0f)) return;
+  int record_count = 1;
+  if (cross) {
+    record_count = 5;
+    float unused[4];
+    V.record(center_y, x - 1, fallback_center, unused, features[1]);
+    V.record(center_y, x + 1, fallback_center, unused, features[2]);
+    V.record(center_y - 1, x, fallback_up, unused, features[3]);
+    V.record(center_y + 1, x, fallback_down, unused, features[4]);
+  }
+  auto wrgb_at = [&](int dy, int dx, int k) {
+    return V.wrgb(center_y - 4 + dy, x - 4 + dx, k);
+  };
+  float candidates[3][3];
+  rgb_candidates(wrgb_at, weights, candidates);
+  const float original[3] = {J.su_work[x * 3 + 0], J.su_work[x * 3 + 1],
+                             J.su_work[x * 3 + 2]};
+  Site site;
+  combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
+                    features, record_count, combiner_reference, P.combine, site.cand);
+  site.orig[0] = original[0];
+  site.orig[1] = original[1];
+  site.orig[2] = original[2];
+  site.pixel = (uint32_t)x | (floor_enabled ? kSiteFloorBit : 0u);
+  J.su_sites[i] = site;
+  J.su_attempted[i] = 1;
+}
+
+// Writes the host writer's changed values into the device output.
+__global__ __launch_bounds__(kThreads) void k_apply_patches(uint16_t* out,
+                                                            const uint32_t* pixels,
+                                                            const uint16_t* values,
+                                                            uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint16_t* o = out + (size_t)pixels[i] * 3;
+  o[0] = values[i * 3 + 0];
+  o[1] = values[i * 3 + 1];
+  o[2] = values[i * 3 + 2];
+}
+
+// ==========================================================================
+// Level-1 test kernels: the device functions above, over host vectors.
+// ==========================================================================
+
+__global__ void t_unscaled_averages(const float* patches, long count, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float* p = patches + i * 81;
+  auto at = [&](int dy, int dx) { return p[dy * 9 + dx]; };
+  double q[3];
+  unscaled_averages(at, q);
+  out[i * 3 + 0] = q[0];
+  out[i * 3 + 1] = q[1];
+  out[i * 3 + 2] = q[2];
+}
+
+__global__ void t_rgb_unscaled_averages(const float* patches, long count, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float* p = patches + i * 243;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    auto lane = [&](int dy, int dx) { return p[(dy * 9 + dx) * 3 + k]; };
+    double q[3];
+    unscaled_averages(lane, q);
+    q[2] = rgb_q16(lane);
+    for (int s = 0; s < 3; ++s) out[(i * 3 + s) * 3 + k] = q[s];
+  }
+}
+
+__global__ void t_dither_chain(const double* values, const float* scales, long count,
+                               uint32_t state_in, float low, float high,
+                               double* deltas, uint32_t* state_out,
+                               unsigned long long* advances_out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  WriterConsts K;
+  K.low = (double)low;
+  K.high = (double)high;
+  K.low_lt_high = K.low < K.high;
+  uint32_t state = state_in;
+  unsigned long long advances = 0;
+  for (long i = 0; i < count; ++i) {
+    uint32_t before = state;
+    deltas[i] = dither_delta(values[i], K, scales[i], state);
+    advances += state != before;
+  }
+  *state_out = state;
+  *advances_out = advances;
+}
+
+__global__ void t_emit(const float* values, long count, uint16_t* out,
+                       const uint32_t* high, const uint32_t* low, uint32_t* flags) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (!__builtin_isfinite(values[i])) atomicOr(flags, 2u);
+  out[i] = emit_value(values[i], high, low);
+}
+
+__global__ void t_planes_aux(const uint16_t* rgbi, long pixels, int width,
+                             const float* lut, DevParams P, const float* alpha,
+                             const uint8_t* is_one, const float* replacement,
+                             const float* offset, float* working, float* aux,
+                             uint32_t* flags) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pixels) return;
+  const int y = (int)(i / width);
+  uint32_t lanes[4];
+  load_pixel(rgbi, (size_t)i, lanes);
+  float w[4];
+  for (int k = 0; k < 4; ++k) {
+    w[k] = lut[lanes[k]];
+    working[i * 4 + k] = w[k];
+  }
+  const float a = is_one[y] ? replacement[y]
+                            : auxiliary(w[P.visible_channel], w[3], alpha[y], offset[y]);
+  if (!__builtin_isfinite(a)) atomicOr(flags, kErrNonfiniteAux);
+  aux[i] = a;
+}
+
+__global__ void t_planes_score(long pixels, int width, DevParams P,
+                               const float* score_base, const float* working,
+                               const float* aux, float* score, float* waux,
+                               float* wrgb) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pixels) return;
+  const int y = (int)(i / width), x = (int)(i % width);
+  float sample = aux[i];
+  if (P.horizontal_minimum && width > 2 && x >= 1 && x <= width - 2)
+    sample = fmin_plain(fmin_plain(aux[i - 1], aux[i]), aux[i + 1]);
+  const float s = score_value(sample, score_base[y], P.score_addend, P.score_scale,
+                              P.score_offset, P.score_floor);
+  score[i] = s;
+  waux[i] = s * aux[i];
+  for (int k = 0; k < 3; ++k) wrgb[i * 3 + k] = s * working[i * 4 + k];
+}
+
+__global__ void t_decision(const float* aux, int height, int width, DevParams P,
+                           uint8_t* fallback) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)height * width) return;
+  const int y = (int)(i / width), x = (int)(i % width);
+  auto aux_at = [&](int dy, int dx) {
+    return aux[(size_t)clampi(y + dy, 0, height - 1) * width + clampi(x + dx, 0, width - 1)];
+  };
+  fallback[i] = decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit);
+}
+
+// ==========================================================================
+// Launchers
+// ==========================================================================
+
+static inline unsigned blocks_for(long n, int threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+hipError_t launch_producer(const JobDev& J, float factor_b, hipStream_t stream) {
+  const int cells = J.width / 8;
+  const long fail_items = (long)((J.height + 7) / 8) * cells;
+  const int epochs = J.height / 8;
+  hipLaunchKernelGGL(k_failpos, dim3(blocks_for(fail_items, kThreads)), dim3(kThreads),
+                     0, stream, J);
+  hipLaunchKernelGGL(k_row_sums, dim3(blocks_for(J.height, kRowSumRows)),
+                     dim3(kRowSumRows), 0, stream, J);
+  if (epochs > 0) {
+    hipLaunchKernelGGL(k_epoch_terms, dim3(blocks_for((long)epochs * cells, kThreads)),
+                       dim3(kThreads), 0, stream, J);
+    hipLaunchKernelGGL(k_epoch_sums, dim3(blocks_for(epochs, kThreads)), dim3(kThreads),
+                       0, stream, J);
+  }
+  hipLaunchKernelGGL(k_producer_chain, dim3(1), dim3(128), 0, stream, J);
+  hipLaunchKernelGGL(k_stage_table, dim3(blocks_for(J.height + 6, kThreads)),
+                     dim3(kThreads), 0, stream, J, factor_b);
+  return hipGetLastError();
+}
+
+hipError_t launch_schedule_rows(const JobDev& J, const ScheduleDev& S,
+                                hipStream_t stream) {
+  hipLaunchKernelGGL(k_schedule_rows, dim3(blocks_for(J.height, kThreads)),
+                     dim3(kThreads), 0, stream, J, S);
+  return hipGetLastError();
+}
+
+template <int TY, int TX>
+static hipError_t launch_scan_shape(const JobDev& J, const DevParams& P, hipStream_t stream) {
+  const int tiles = J.tiles_x * J.tiles_y;
+  hipLaunchKernelGGL((k_scan<TY, TX>), dim3(tiles), dim3(kThreads), 0, stream, J, P);
+  hipLaunchKernelGGL(k_scan_segments, dim3(1), dim3(1024), 0, stream, J);
+  return hipGetLastError();
+}
+
+template <int TY, int TX>
+static hipError_t launch_sites_shape(const JobDev& J, const DevParams& P, hipStream_t stream) {
+  const int tiles = J.tiles_x * J.tiles_y;
+  hipLaunchKernelGGL((k_sites<TY, TX>), dim3(tiles), dim3(kThreads), 0, stream, J, P);
+  return hipGetLastError();
+}
+
+#define DICE_TILE_SHAPES(X) X(16, 64) X(8, 128) X(32, 32) X(16, 32) X(32, 64) X(8, 64)
+
+bool tile_shape_supported(int ty, int tx) {
+#define DICE_MATCH(a, b) if (ty == a && tx == b) return true;
+  DICE_TILE_SHAPES(DICE_MATCH)
+#undef DICE_MATCH
+  return false;
+}
+
+hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream) {
+  const int ty = J.tile_rows, tx = J.tile_cols;
+#define DICE_MATCH(a, b) if (ty == a && tx == b) return launch_scan_shape<a, b>(J, P, stream);
+  DICE_TILE_SHAPES(DICE_MATCH)
+#undef DICE_MATCH
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream) {
+  const int ty = J.tile_rows, tx = J.tile_cols;
+#define DICE_MATCH(a, b) if (ty == a && tx == b) return launch_sites_shape<a, b>(J, P, stream);
+  DICE_TILE_SHAPES(DICE_MATCH)
+#undef DICE_MATCH
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream) {
+  const int rows = J.height < 5 ? J.height : 5;
+  hipLaunchKernelGGL(k_startup_rows, dim3(blocks_for((long)rows * J.width, kThreads)),
+                     dim3(kThreads), 0, stream, J, P, rows);
+  hipLaunchKernelGGL(k_startup_sites, dim3(blocks_for(6L * J.width, kThreads)),
+                     dim3(kThreads), 0, stream, J, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
+                                const uint16_t* values, uint32_t count,
+                                hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_patches, dim3(blocks_for(count, kThreads)), dim3(kThreads),
+                     0, stream, out, pixels, values, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_unscaled_averages(const float* patches, long count, double* out,
+                                      hipStream_t s) {
+  hipLaunchKernelGGL(t_unscaled_averages, dim3(blocks_for(count, 128)), dim3(128), 0, s,
+                     patches, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_rgb_unscaled_averages(const float* patches, long count, double* out,
+                                          hipStream_t s) {
+  hipLaunchKernelGGL(t_rgb_unscaled_averages, dim3(blocks_for(count, 128)), dim3(128), 0,
+                     s, patches, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_dither_chain(const double* values, const float* scales, long count,
+                                 uint32_t state_in, float low, float high, double* deltas,
+                                 uint32_t* state_out, unsigned long long* advances,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(t_dither_chain, dim3(1), dim3(1), 0, s, values, scales, count,
+                     state_in, low, high, deltas, state_out, advances);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_emit(const float* values, long count, uint16_t* out,
+                         const uint32_t* high, const uint32_t* low, uint32_t* flags,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(t_emit, dim3(blocks_for(count, kThreads)), dim3(kThreads), 0, s,
+                     values, count, out, high, low, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_planes(const uint16_t* rgbi, int height, int width, const float* lut,
+                           const DevParams& P, const float* alpha, const uint8_t* is_one,
+                           const float* replacement, const float* offset,
+                           const float* score_base, float* working, float* aux,
+                           float* score, float* waux, float* wrgb, uint32_t* flags,
+                           hipStream_t s) {
+  const long pixels = (long)height * width;
+  hipLaunchKernelGGL(t_planes_aux, dim3(blocks_for(pixels, kThreads)), dim3(kThreads), 0,
+                     s, rgbi, pixels, width, lut, P, alpha, is_one, replacement, offset,
+                     working, aux, flags);
+  hipLaunchKernelGGL(t_planes_score, dim3(blocks_for(pixels, kThreads)), dim3(kThreads),
+                     0, s, pixels, width, P, score_base, working, aux, score, waux, wrgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_t_decision(const float* aux, int height, int width, const DevParams& P,
+                             uint8_t* fallback, hipStream_t s) {
+  hipLaunchKernelGGL(t_decision, dim3(blocks_for((long)height * width, kThreads)),
+                     dim3(kThreads), 0, s, aux, height, width, P, fallback);
+  return hipGetLastError();
+}
+
+}  // namespace dice

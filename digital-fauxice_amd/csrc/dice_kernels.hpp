@@ -1,0 +1,58 @@
+// dice_kernels.hpp -- launchers of the gfx950 kernels (dice_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dice_device.hpp"
+
+namespace dice {
+
+// Device copies of the ProducerRecordSchedule arrays (producer API only).
+struct ScheduleDev {
+  double* weighted_visible_sum;
+  double* weighted_infrared_sum;
+  double* weight_sum;
+  float* mean_visible;
+  float* mean_infrared;
+  uint32_t* accepted;
+  float* scaled_input;
+  float* input_scale;
+  float* scale_denominator;
+  float* scale_numerator;
+  float* scale_add_denominator;
+  float* scale_add_numerator;
+};
+
+hipError_t launch_producer(const JobDev& J, float factor_b, hipStream_t stream);
+hipError_t launch_schedule_rows(const JobDev& J, const ScheduleDev& S,
+                                hipStream_t stream);
+bool tile_shape_supported(int ty, int tx);
+hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream);
+hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream);
+hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream);
+hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
+                                const uint16_t* values, uint32_t count,
+                                hipStream_t stream);
+
+hipError_t launch_t_unscaled_averages(const float* patches, long count, double* out,
+                                      hipStream_t s);
+hipError_t launch_t_rgb_unscaled_averages(const float* patches, long count,
+                                          double* out, hipStream_t s);
+hipError_t launch_t_dither_chain(const double* values, const float* scales, long count,
+                                 uint32_t state_in, float low, float high,
+                                 double* deltas, uint32_t* state_out,
+                                 unsigned long long* advances, hipStream_t s);
+hipError_t launch_t_emit(const float* values, long count, uint16_t* out,
+                         const uint32_t* high, const uint32_t* low, uint32_t* flags,
+                         hipStream_t s);
+hipError_t launch_t_planes(const uint16_t* rgbi, int height, int width,
+                           const float* lut, const DevParams& P, const float* alpha,
+                           const uint8_t* is_one, const float* replacement,
+                           const float* offset, const float* score_base,
+                           float* working, float* aux, float* score, float* waux,
+                           float* wrgb, uint32_t* flags, hipStream_t s);
+hipError_t launch_t_decision(const float* aux, int height, int width,
+                             const DevParams& P, uint8_t* fallback, hipStream_t s);
+
+}  // namespace dice

@@ -160,6 +160,10 @@ class DiceStageTimings(C.Structure):
         ("download_output", C.c_double),
         ("sha256_host", C.c_double),
         ("total", C.c_double),
+        ("producer_kernels_ms", C.c_double),
+        ("scan_kernel_ms", C.c_double),
+        ("sites_kernel_ms", C.c_double),
+        ("startup_kernels_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict[str, float]:

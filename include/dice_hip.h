@@ -195,6 +195,12 @@ typedef struct dice_stage_timings {
   double download_output;
   double sha256_host;
   double total;
+  /* Device durations (milliseconds) measured with HIP events on the
+   * library's stream around each kernel group. */
+  double producer_kernels_ms;
+  double scan_kernel_ms;
+  double sites_kernel_ms;
+  double startup_kernels_ms;
 } dice_stage_timings;
 
 typedef struct dice_device_info {

@@ -1,0 +1,35 @@
+"""pytest configuration: markers, import paths and shared fixtures."""
+
+from __future__ import annotations
+
+import json
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT / "digital-fauxice_amd"))
+sys.path.insert(0, str(ROOT / "tests"))
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+
+
+@pytest.fixture(scope="session")
+def reference_vectors() -> dict:
+    return json.loads((GOLDEN / "reference_vectors.json").read_text())
+
+
+def hip_device_or_skip():
+    """Skip (with the reason) when no HIP device is usable."""
+
+    from dice_amd.hip_backend import HipBackendUnavailable, hip_device_summary
+
+    try:
+        return hip_device_summary()
+    except HipBackendUnavailable as error:
+        pytest.skip(f"HIP device unavailable: {error}")
