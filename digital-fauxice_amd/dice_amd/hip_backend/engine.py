@@ -52,7 +52,7 @@ class HipBackendUnavailable(RuntimeError):
     """HIP execution was requested but cannot run exactly here."""
 
 
-_LOCK = threading.Lock()
+_LOCK = threading.RLock()  # context() -> hip_device_summary() -> library() re-enters
 _LIB: list[C.CDLL] = []
 _CONTEXTS: dict[int, C.c_void_p] = {}
 

@@ -106,3 +106,18 @@ def test_no_device_fails_closed():
         assert str(error)
         return
     assert summary["arch"].startswith("gfx950")
+
+
+def test_context_lock_is_reentrant():
+    """context() calls hip_device_summary() -> library() under the same lock."""
+
+    assert hip_engine._LOCK.acquire(timeout=1)
+    try:
+        assert hip_engine._LOCK.acquire(timeout=1), "module lock must be reentrant"
+        hip_engine._LOCK.release()
+    finally:
+        hip_engine._LOCK.release()
+    try:
+        hip_engine.context()
+    except hip_engine.HipBackendUnavailable:
+        pass
