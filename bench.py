@@ -47,7 +47,22 @@ def parse_args() -> argparse.Namespace:
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=15.0,
                         help="approximate CPU time budget of the baseline sample")
+    parser.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                        help="only run the plain copy kernel over BYTES (counter calibration)")
     return parser.parse_args()
+
+
+def copy_calibration(nbytes: int) -> int:
+    """Plain device copy of a known size: the reference point for HBM counters."""
+
+    from dice_amd.hip_backend import engine
+
+    ms = C.c_double()
+    engine.raise_for(engine.library().dice_t_copy(engine.context(), nbytes, 20, C.byref(ms)))
+    gbs = 2 * nbytes / (ms.value / 1e3) / 1e9
+    print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(ms.value, 4),
+                      "read_plus_write_GBs": round(gbs, 1)}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -56,6 +71,8 @@ def main() -> int:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(jobs shard across GPUs without collectives; see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     from dice_amd import _abi, synth
     from dice_amd._abi import DiceReplayResult, DiceStageTimings, ptr

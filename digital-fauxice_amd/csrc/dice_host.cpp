@@ -1219,6 +1219,26 @@ int32_t dice_t_planes(dice_ctx* ctx, const uint16_t* rgbi, int64_t height, int64
   });
 }
 
+int32_t dice_t_copy(dice_ctx* ctx, uint64_t bytes, int32_t repeats, double* ms_per_copy) {
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    const size_t n = (size_t)(bytes / 16) * 16;
+    Scratch S;
+    uint8_t* src = S.alloc<uint8_t>(n);
+    uint8_t* dst = S.alloc<uint8_t>(n);
+    if (!src || !dst) return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemsetAsync(src, 1, n, ctx->stream));
+    HIP_TRY(launch_copy(src, dst, n, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->events[0], ctx->stream));
+    for (int32_t i = 0; i < repeats; ++i) HIP_TRY(launch_copy(src, dst, n, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->events[1], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->events[1]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    *ms_per_copy = repeats > 0 ? (double)ms / repeats : 0.0;
+    return DICE_OK;
+  });
+}
+
 int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height, int64_t width,
                         const dice_params* params, uint8_t* fallback) {
   return with_device_buffers(ctx, [&]() -> int32_t {

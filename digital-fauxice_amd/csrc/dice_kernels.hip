@@ -1016,9 +1016,24 @@ __global__ void t_decision(const float* aux, int height, int width, DevParams P,
   fallback[i] = decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit);
 }
 
+// Plain 16-byte-vector copy: the calibration kernel for HBM counters and
+// attainable bandwidth.
+__global__ __launch_bounds__(kThreads) void k_copy(const uint4* src, uint4* dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
 // ==========================================================================
 // Launchers
 // ==========================================================================
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(256 * 8 * 4), dim3(kThreads), 0, s,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  return hipGetLastError();
+}
 
 static inline unsigned blocks_for(long n, int threads) {
   return (unsigned)((n + threads - 1) / threads);

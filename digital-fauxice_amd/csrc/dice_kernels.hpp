@@ -35,6 +35,7 @@ hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
                                 const uint16_t* values, uint32_t count,
                                 hipStream_t stream);
 
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_t_unscaled_averages(const float* patches, long count, double* out,
                                       hipStream_t s);
 hipError_t launch_t_rgb_unscaled_averages(const float* patches, long count,

@@ -232,6 +232,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     ),
     "t_decision": (C.c_int32, [VOIDP, f32p, i64, i64, C.POINTER(DiceParams),
                                u8p]),
+    "t_copy": (C.c_int32, [VOIDP, C.c_uint64, C.c_int32, f64p]),
 }
 
 

@@ -295,6 +295,11 @@ int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height,
                         int64_t width, const dice_params* params,
                         uint8_t* fallback);
 
+/* Calibration: `repeats` launches of a plain 16-byte-vector copy kernel over
+ * `bytes` (device buffers owned by the call); average milliseconds per copy. */
+int32_t dice_t_copy(dice_ctx* ctx, uint64_t bytes, int32_t repeats,
+                    double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
