@@ -19,14 +19,123 @@ struct alignas(8) Site {
 };
 static_assert(sizeof(Site) == 40, "Site record must stay 40 bytes");
 
+// The writer's draw schedule of one site in integer form.  For every
+// channel that draws (low < f32(cand) < high), u is the LCG state after the
+// first draw (or its mirror 2^24-1-state when the dither span is negative,
+// so the dither delta is non-decreasing in u); the draw is accepted for
+// u in [accept_lo, accept_hi) and, when accepted, the original-floor test
+// cand + delta <= orig holds for u < floor_end.  Flags: bit 0 valid
+// candidate, bit 1 floor gate; channel c: bit 2+4c draws, bit 3+4c floor
+// test result when the draw is rejected, bit 4+4c mirrored state; bit 31
+// the site is attempted (always for main-frame sites).
+struct WriterRecord {
+  uint32_t flags;
+  uint32_t accept_lo[3];
+  uint32_t accept_hi[3];
+  uint32_t floor_end[3];
+};
+static_assert(sizeof(WriterRecord) == 40, "WriterRecord must stay 40 bytes");
+
 constexpr uint32_t kSitePixelMask = 0x7FFFFFFFu;
 constexpr uint32_t kSiteFloorBit = 0x80000000u;
+constexpr uint32_t kRecordAttempted = 0x80000000u;
+
+// Number of u in [0, 2^24) on the true prefix of a monotone predicate.
+template <class F>
+DICE_HD uint32_t prefix_count(const F& holds) {
+  uint32_t lo = 0, hi = 1u << 24;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (holds(mid))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// The integer draw schedule of one site (reconstruction.py:916-982,
+// dither.py:59-104).  Every operation between the LCG state and the two
+// comparisons the writer makes -- acceptance low < cand32 + delta < high
+// and the original floor cand + delta <= orig -- is a monotone IEEE
+// rounding of a monotone function of the (mirrored) state, so each
+// comparison holds on a prefix of the state range and binary searches
+// recover the exact thresholds.  A non-monotone case (a NaN delta) cannot
+// pass silently: the per-site writer re-run from the recorded states
+// checks the whole chain.
+DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const WriterConsts& K) {
+  WriterRecord r;
+  r.flags = 0;
+  for (int c = 0; c < 3; ++c) r.accept_lo[c] = r.accept_hi[c] = r.floor_end[c] = 0;
+  if (!attempted) return r;
+  const bool valid = finite64(site.cand[0]) && finite64(site.cand[1]) &&
+                     finite64(site.cand[2]) && site.cand[0] > 0.0 && site.cand[1] > 0.0 &&
+                     site.cand[2] > 0.0;
+  if (valid) {
+    r.flags = 1u | ((site.pixel & kSiteFloorBit) ? 2u : 0u);
+    for (int c = 0; c < 3; ++c) {
+      const double cd = (double)(float)site.cand[c];
+      if (!(K.low_lt_high && K.low < cd && cd < K.high)) continue;
+      const double width = K.high - K.low;
+      const double coefficient = 4.0 / (width * width);
+      const double envelope = ((K.high - cd) * (cd - K.low)) * coefficient;
+      const double span = (double)(float)((double)K.scales[c] * cd);
+      const bool mirrored = span < 0.0;
+      auto delta = [&](uint32_t u) {
+        const uint32_t state = mirrored ? kLcgMask - u : u;
+        const double centered = (double)(state + 1u) * kNikonNormalization - 0.5;
+        const double random_value = 0.0 + centered * span;
+        return envelope * random_value;
+      };
+      const double cand = site.cand[c];
+      const double source = (double)site.orig[c];
+      r.accept_lo[c] = prefix_count([&](uint32_t u) { return cd + delta(u) <= K.low; });
+      r.accept_hi[c] = prefix_count([&](uint32_t u) { return cd + delta(u) < K.high; });
+      r.floor_end[c] = prefix_count([&](uint32_t u) { return cand + delta(u) <= source; });
+      const bool floor_when_rejected = cand + 0.0 <= source;
+      r.flags |= (1u | (floor_when_rejected ? 2u : 0u) | (mirrored ? 4u : 0u)) << (2 + 4 * c);
+    }
+  }
+  r.flags |= kRecordAttempted;
+  return r;
+}
+
+DICE_HD uint32_t lcg_step(uint32_t s) { return (125u * s + 1u) & kLcgMask; }
+
+// The serial part of the writer on one record: one LCG step per drawing
+// channel, and a redraw step when the floor gate is on and the floor test
+// fails.  The LCG has no fixed point (124 s = -1 mod 2^24 has no
+// solution), so every draw is one counted advance.
+DICE_HD uint32_t chain_site(const WriterRecord& r, uint32_t s, uint64_t& draws) {
+  const uint32_t f = r.flags;
+  if (!(f & 1u)) return s;
+  const bool floor = (f & 2u) != 0;
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t cf = f >> (2 + 4 * c);
+    if (!(cf & 1u)) continue;
+    const uint32_t t = lcg_step(s);
+    draws += 1;
+    s = t;
+    if (floor) {
+      const uint32_t u = (cf & 4u) ? kLcgMask - t : t;
+      const bool accepted = u >= r.accept_lo[c] && u < r.accept_hi[c];
+      const bool copy = accepted ? u < r.floor_end[c] : (cf & 2u) != 0;
+      if (!copy) {
+        s = lcg_step(t);
+        draws += 1;
+      }
+    }
+  }
+  return s;
+}
 
 // Device error flags (atomicOr): the host maps them to the reference's
 // fail-closed exceptions.
 constexpr uint32_t kErrNonfiniteAux = 1u;    // continuous_score ValueError
 constexpr uint32_t kErrSetterScaleOne = 4u;  // derive_x3a_setter_fields
 constexpr uint32_t kErrStageNonfinite = 8u;  // StageCalibration finiteness
+constexpr uint32_t kErrNonfiniteWork = 16u;  // work_value_indices ValueError
+constexpr uint32_t kErrWriterChain = 32u;    // integer chain disagrees with write_pixel
 
 // Stable replay parameters (dice_params, binary32-quantized).
 struct DevParams {
@@ -67,6 +176,12 @@ struct JobDev {
   uint32_t* seg_count;          // (tiles_y*TY) * tiles_x
   uint32_t* seg_offset;         // (tiles_y*TY) * tiles_x
   Site* sites;                  // compacted, row-major
+  WriterRecord* writer_records; // one per site, row-major
+  WriterRecord* su_writer_records; // 6*W
+  uint32_t* site_state;         // LCG state at the start of each site
+  uint32_t* su_site_state;      // 6*W, the same for the hidden startup rows
+  unsigned long long* writer_counts; // [0] written, [1] changed
+  uint8_t* diag_changed;        // optional H*W
   uint32_t* counters;           // [0] total sites
   uint32_t* error_flags;
   float* diag_score;            // optional H*W

@@ -6,8 +6,11 @@
 //   host    prepass (0x40 record, tiny), LUT/factor tables (hash-pinned)
 //   device  producer schedule incl. the sequential chains and the setter
 //   device  fused analysis tiles + hidden-startup sites + row-major compaction
-//   host    the strictly serial conditional-dither writer over Site records
-//   device/host  patch of changed pixels, SHA-256 of the public output
+//   device  integer draw schedule of every site (exact LCG-state thresholds)
+//   host    the strictly serial LCG state chain over those schedules
+//   device  every site's writer step from its recorded start state, the
+//           output patch, the counts and a check of the chain
+//   host    SHA-256 of the public output
 #include <hip/hip_runtime.h>
 #include <openssl/evp.h>
 
@@ -189,7 +192,7 @@ struct PinnedBuffer {
 
 struct dice_ctx {
   int device = 0;
-  hipEvent_t events[8] = {};
+  hipEvent_t events[12] = {};
   uint64_t hbm_budget = 0;
   hipStream_t stream = nullptr;
   std::mutex mutex;
@@ -205,11 +208,10 @@ struct dice_ctx {
       epoch_wide, epoch_store;
   DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse,
       st_gate, st_row_gate;
-  DeviceBuffer patch_pixels, patch_values;
+  DeviceBuffer writer_records, su_writer_records, site_state, writer_counts, diag_changed;
   DeviceBuffer sched[12];
   // pinned host staging
-  PinnedBuffer h_counters, h_sites, h_su_sites, h_su_attempted, h_patch_pixels,
-      h_patch_values, h_out;
+  PinnedBuffer h_counters, h_records, h_site_state, h_out;
 };
 
 namespace {
@@ -416,21 +418,11 @@ int32_t flag_error(uint32_t flags) {
   return DICE_OK;
 }
 
-// The strictly serial writer over the six hidden stages and then every
-// attempted site in row-major order (streaming.py:476-531,
-// startup.py:231-341, fast_cpu/kernels.py:1139-1204).
-struct WriterOutput {
-  uint64_t attempted = 0, written = 0, advances = 0, changed = 0;
-  uint32_t state = 0;
-  uint32_t patches = 0;
-  bool nonfinite = false;
-};
-
-WriterOutput run_writer(const dice_params& p, uint32_t state, int64_t W,
-                        const Site* su_sites, const uint8_t* su_attempted,
-                        const Site* sites, uint32_t count, uint32_t* patch_pixels,
-                        uint16_t* patch_values, dice_replay_result* result) {
-  const Tables& T = tables();
+// The strictly serial part of the writer (streaming.py:476-531,
+// startup.py:231-341, fast_cpu/kernels.py:1139-1204) is the LCG state
+// chain over the WriterRecord schedules (chain_site); the values follow
+// in parallel on the device from the recorded start states.
+WriterConsts writer_consts(const dice_params& p) {
   WriterConsts K;
   K.low = (double)p.dither_low;
   K.high = (double)p.dither_high;
@@ -438,54 +430,7 @@ WriterOutput run_writer(const dice_params& p, uint32_t state, int64_t W,
   K.scales[0] = p.dither_scales[0];
   K.scales[1] = p.dither_scales[1];
   K.scales[2] = p.dither_scales[2];
-  float values[3];
-  for (int s = 0; s < 6; ++s) {
-    int64_t attempted = 0, advances = 0;
-    for (int64_t x = 0; x < W; ++x) {
-      if (!su_attempted[s * W + x]) continue;
-      const Site& site = su_sites[s * W + x];
-      attempted += 1;
-      advances += write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0,
-                              K, state, values);
-    }
-    result->startup_attempted[s] = attempted;
-    result->startup_advances[s] = advances;
-  }
-  result->startup_final_rng_state = state;
-
-  WriterOutput o;
-  for (uint32_t i = 0; i < count; ++i) {
-    const Site& site = sites[i];
-    o.advances += (uint64_t)write_pixel(site.cand, site.orig,
-                                        (site.pixel & kSiteFloorBit) != 0, K, state,
-                                        values);
-    if (values[0] != site.orig[0] || values[1] != site.orig[1] ||
-        values[2] != site.orig[2])
-      o.written += 1;
-    if (!(std::isfinite(values[0]) && std::isfinite(values[1]) &&
-          std::isfinite(values[2]))) {
-      o.nonfinite = true;
-      continue;
-    }
-    uint16_t rendered[3], noop[3];
-    bool changed = false;
-    for (int c = 0; c < 3; ++c) {
-      rendered[c] = emit_value(values[c], T.high.data(), T.low.data());
-      noop[c] = emit_value(site.orig[c], T.high.data(), T.low.data());
-      changed |= rendered[c] != noop[c];
-    }
-    if (changed) {
-      o.changed += 1;
-      patch_pixels[o.patches] = site.pixel & kSitePixelMask;
-      patch_values[o.patches * 3 + 0] = rendered[0];
-      patch_values[o.patches * 3 + 1] = rendered[1];
-      patch_values[o.patches * 3 + 2] = rendered[2];
-      o.patches += 1;
-    }
-  }
-  o.attempted = count;
-  o.state = state;
-  return o;
+  return K;
 }
 
 // The analysis + writer + output phases of one job whose input is already
@@ -498,6 +443,7 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   const int64_t H = J.height, W = J.width;
   const size_t pixels = (size_t)H * W;
   DevParams P = device_params(p);
+  const WriterConsts K = writer_consts(p);
   cb.progress(3, 0, H, 0, 0);
   double t0 = now_seconds();
   EventTimer ev{ctx, timings != nullptr};
@@ -511,14 +457,42 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   const uint32_t count = hc[0];
   if (int32_t e = flag_error(hc[1])) return e;
   if (cb.cancelled()) return cancelled_error();
-  // Sites go straight to their row-major slot; size the array by the count.
-  HIP_TRY(ctx->sites.ensure((size_t)count * sizeof(Site) + sizeof(Site)));
+
+  // Sites go straight to their row-major slot; size the arrays by the count.
+  const size_t startup_sites = 6 * (size_t)W;
+  HIP_TRY(ctx->sites.ensure(((size_t)count + 1) * sizeof(Site)));
+  HIP_TRY(ctx->writer_records.ensure(((size_t)count + 1) * sizeof(WriterRecord)));
+  HIP_TRY(ctx->su_writer_records.ensure(startup_sites * sizeof(WriterRecord)));
+  HIP_TRY(ctx->site_state.ensure(((size_t)count + startup_sites + 1) * 4));
+  HIP_TRY(ctx->writer_counts.ensure(64));
   J.sites = ctx->sites.as<Site>();
+  J.writer_records = ctx->writer_records.as<WriterRecord>();
+  J.su_writer_records = ctx->su_writer_records.as<WriterRecord>();
+  J.su_site_state = ctx->site_state.as<uint32_t>();
+  J.site_state = J.su_site_state + startup_sites;
+  J.writer_counts = ctx->writer_counts.as<unsigned long long>();
+  J.diag_changed = nullptr;
+  if (diag && diag->changed_mask) {
+    HIP_TRY(ctx->diag_changed.ensure(pixels));
+    J.diag_changed = ctx->diag_changed.as<uint8_t>();
+  }
   if (int32_t e = ev.record(4)) return e;
   HIP_TRY(launch_sites(J, P, s));
   if (int32_t e = ev.record(5)) return e;
   HIP_TRY(launch_startup(J, P, s));
   if (int32_t e = ev.record(6)) return e;
+  HIP_TRY(launch_writer_prep(J.sites, nullptr, J.writer_records, count, K, s));
+  HIP_TRY(launch_writer_prep(J.su_sites, J.su_attempted, J.su_writer_records,
+                             (uint32_t)startup_sites, K, s));
+  if (int32_t e = ev.record(7)) return e;
+  HIP_TRY(ctx->h_records.ensure(((size_t)count + startup_sites) * sizeof(WriterRecord)));
+  WriterRecord* h_su = ctx->h_records.as<WriterRecord>();
+  WriterRecord* h_main = h_su + startup_sites;
+  HIP_TRY(hipMemcpyAsync(h_su, J.su_writer_records, startup_sites * sizeof(WriterRecord),
+                         hipMemcpyDeviceToHost, s));
+  if (count)
+    HIP_TRY(hipMemcpyAsync(h_main, J.writer_records, (size_t)count * sizeof(WriterRecord),
+                           hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   double t1 = now_seconds();
@@ -527,90 +501,97 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
     timings->scan_kernel_ms += ev.elapsed_ms(2, 3);
     timings->sites_kernel_ms += ev.elapsed_ms(4, 5);
     timings->startup_kernels_ms += ev.elapsed_ms(5, 6);
+    timings->writer_prep_kernel_ms += ev.elapsed_ms(6, 7);
   }
   if (int32_t e = flag_error(hc[1])) return e;
+  if (cb.cancelled()) return cancelled_error();
 
-  HIP_TRY(ctx->h_sites.ensure((size_t)count * sizeof(Site) + 64));
-  HIP_TRY(ctx->h_su_sites.ensure(6 * W * sizeof(Site)));
-  HIP_TRY(ctx->h_su_attempted.ensure(6 * W));
-  if (count)
-    HIP_TRY(hipMemcpyAsync(ctx->h_sites.ptr, J.sites, (size_t)count * sizeof(Site),
-                           hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(ctx->h_su_sites.ptr, J.su_sites, 6 * W * sizeof(Site),
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(ctx->h_su_attempted.ptr, J.su_attempted, 6 * W,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  // Serial state chain: the six hidden stages, then every site in order.
+  HIP_TRY(ctx->h_site_state.ensure(((size_t)count + startup_sites + 1) * 4));
+  uint32_t* h_su_state = ctx->h_site_state.as<uint32_t>();
+  uint32_t* h_state = h_su_state + startup_sites;
+  uint32_t state = rng_state;
+  for (int stage = 0; stage < 6; ++stage) {
+    uint64_t draws = 0;
+    int64_t attempted = 0;
+    for (int64_t x = 0; x < W; ++x) {
+      const size_t i = (size_t)stage * W + x;
+      const WriterRecord& r = h_su[i];
+      h_su_state[i] = state;
+      attempted += (r.flags & kRecordAttempted) ? 1 : 0;
+      state = chain_site(r, state, draws);
+    }
+    result->startup_attempted[stage] = attempted;
+    result->startup_advances[stage] = (int64_t)draws;
+  }
+  const uint32_t startup_final = state;
+  result->startup_final_rng_state = startup_final;
+  uint64_t advances = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    h_state[i] = state;
+    state = chain_site(h_main[i], state, advances);
+  }
+  const uint32_t final_state = state;
   double t2 = now_seconds();
-  if (timings) timings->download_sites += t2 - t1;
+  if (timings) timings->writer_host += t2 - t1;
 
-  HIP_TRY(ctx->h_patch_pixels.ensure((size_t)count * 4 + 64));
-  HIP_TRY(ctx->h_patch_values.ensure((size_t)count * 6 + 64));
-  uint32_t* patch_pixels = ctx->h_patch_pixels.as<uint32_t>();
-  uint16_t* patch_values = ctx->h_patch_values.as<uint16_t>();
-  WriterOutput w = run_writer(p, rng_state, W, ctx->h_su_sites.as<Site>(),
-                              ctx->h_su_attempted.as<uint8_t>(), ctx->h_sites.as<Site>(),
-                              count, patch_pixels, patch_values, result);
+  // Parallel values, patch of the output, counts, and the check of both
+  // chains against the per-site writer.
+  HIP_TRY(hipMemcpyAsync(J.su_site_state, h_su_state, (startup_sites + count) * 4,
+                         hipMemcpyHostToDevice, s));
+  if (int32_t e = ev.record(8)) return e;
+  HIP_TRY(launch_writer_values(J, K, count, startup_final, final_state, s));
+  if (int32_t e = ev.record(9)) return e;
+  unsigned long long* h_counts = reinterpret_cast<unsigned long long*>(hc + 4);
+  HIP_TRY(hipMemcpyAsync(h_counts, J.writer_counts, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   double t3 = now_seconds();
-  if (timings) timings->writer_host += t3 - t2;
-  if (w.nonfinite) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
+  if (timings) {
+    timings->finish_device += t3 - t2;
+    timings->writer_values_kernel_ms += ev.elapsed_ms(8, 9);
+  }
+  if (hc[1] & kErrNonfiniteWork) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
+  if (hc[1] & kErrWriterChain)
+    return set_error(DICE_INTERNAL, "writer state chain disagrees with the per-site writer");
+  if (int32_t e = flag_error(hc[1])) return e;
   if (cb.cancelled()) return cancelled_error();
 
   const bool out_on_device = (flags & DICE_OUT_ON_DEVICE) != 0;
   const bool want_sha = (flags & DICE_SKIP_SHA256) == 0;
   result->output_sha256[0] = 0;
+  const void* hash_source = nullptr;
   if (out_on_device) {
-    if (w.patches) {
-      HIP_TRY(ctx->patch_pixels.ensure((size_t)w.patches * 4));
-      HIP_TRY(ctx->patch_values.ensure((size_t)w.patches * 6));
-      HIP_TRY(hipMemcpyAsync(ctx->patch_pixels.ptr, patch_pixels, (size_t)w.patches * 4,
-                             hipMemcpyHostToDevice, s));
-      HIP_TRY(hipMemcpyAsync(ctx->patch_values.ptr, patch_values, (size_t)w.patches * 6,
-                             hipMemcpyHostToDevice, s));
-      HIP_TRY(launch_apply_patches(J.out, ctx->patch_pixels.as<uint32_t>(),
-                                   ctx->patch_values.as<uint16_t>(), w.patches, s));
-    }
     if (want_sha) {
       HIP_TRY(ctx->h_out.ensure(pixels * 6));
       HIP_TRY(hipMemcpyAsync(ctx->h_out.ptr, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      hash_source = ctx->h_out.ptr;
     }
-    HIP_TRY(hipStreamSynchronize(s));
-    double t4 = now_seconds();
-    if (timings) timings->finish_device += t4 - t3;
-    if (want_sha && !sha256_hex(ctx->h_out.ptr, pixels * 6, result->output_sha256))
-      return set_error(DICE_INTERNAL, "SHA-256 failed");
-    if (timings) timings->sha256_host += now_seconds() - t4;
   } else {
     HIP_TRY(hipMemcpyAsync(out_host, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    double t4 = now_seconds();
-    if (timings) timings->download_output += t4 - t3;
-    for (uint32_t i = 0; i < w.patches; ++i) {
-      uint16_t* o = out_host + (size_t)patch_pixels[i] * 3;
-      o[0] = patch_values[i * 3 + 0];
-      o[1] = patch_values[i * 3 + 1];
-      o[2] = patch_values[i * 3 + 2];
-    }
-    if (want_sha && !sha256_hex(out_host, pixels * 6, result->output_sha256))
-      return set_error(DICE_INTERNAL, "SHA-256 failed");
-    if (timings) timings->sha256_host += now_seconds() - t4;
+    hash_source = out_host;
   }
+  double t4 = now_seconds();
+  if (timings) timings->download_output += t4 - t3;
+  if (want_sha && !sha256_hex(hash_source, pixels * 6, result->output_sha256))
+    return set_error(DICE_INTERNAL, "SHA-256 failed");
+  if (timings) timings->sha256_host += now_seconds() - t4;
   if (diag) {
     if (diag->score_plane)
       HIP_TRY(hipMemcpy(diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
     if (diag->at_floor_mask)
       HIP_TRY(hipMemcpy(diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
-    if (diag->changed_mask) {
-      memset(diag->changed_mask, 0, pixels);
-      for (uint32_t i = 0; i < w.patches; ++i) diag->changed_mask[patch_pixels[i]] = 1;
-    }
+    if (diag->changed_mask)
+      HIP_TRY(hipMemcpy(diag->changed_mask, J.diag_changed, pixels, hipMemcpyDeviceToHost));
   }
-  result->attempted_pixels = w.attempted;
-  result->written_pixels = w.written;
-  result->public_rng_advances = w.advances;
-  result->changed_pixels = w.changed;
-  result->final_rng_state = w.state;
-  cb.progress(3, H, H, (int64_t)w.attempted, (int64_t)w.written);
+  result->attempted_pixels = count;
+  result->written_pixels = h_counts[0];
+  result->public_rng_advances = advances;
+  result->changed_pixels = h_counts[1];
+  result->final_rng_state = final_state;
+  cb.progress(3, H, H, (int64_t)count, (int64_t)h_counts[0]);
   return DICE_OK;
 }
 

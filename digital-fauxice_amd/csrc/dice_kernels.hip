@@ -896,6 +896,94 @@ __global__ __launch_bounds__(kThreads) void k_startup_sites(JobDev J, DevParams 
   J.su_attempted[i] = 1;
 }
 
+// ==========================================================================
+// Writer support (reconstruction.py:916-982, dither.py:59-104)
+// ==========================================================================
+
+// Integer draw schedule of every site (writer_record).
+__global__ __launch_bounds__(kThreads) void k_writer_prep(const Site* sites,
+                                                          const uint8_t* attempted,
+                                                          WriterRecord* records,
+                                                          uint32_t count, WriterConsts K) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const bool is_attempted = attempted ? attempted[i] != 0 : true;
+  records[i] = writer_record(sites[i], is_attempted, K);
+}
+
+// The writer itself, every site in parallel from the LCG state the serial
+// integer chain recorded at its start: values, written / changed counts,
+// the patch of the output, and a check that each site ends in the state
+// the chain recorded for the next one.
+__global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, WriterConsts K,
+                                                            uint32_t count,
+                                                            uint32_t final_state) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool written = false, changed = false;
+  if (i < count) {
+    const Site site = J.sites[i];
+    uint32_t state = J.site_state[i];
+    float values[3];
+    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, K, state, values);
+    const uint32_t expected = i + 1 < count ? J.site_state[i + 1] : final_state;
+    uint32_t error = state != expected ? kErrWriterChain : 0u;
+    written = values[0] != site.orig[0] || values[1] != site.orig[1] ||
+              values[2] != site.orig[2];
+    if (!(__builtin_isfinite(values[0]) && __builtin_isfinite(values[1]) &&
+          __builtin_isfinite(values[2]))) {
+      error |= kErrNonfiniteWork;
+    } else {
+      uint16_t rendered[3];
+      for (int c = 0; c < 3; ++c) {
+        rendered[c] = emit_value(values[c], J.factor_high, J.factor_low);
+        changed |= rendered[c] != emit_value(site.orig[c], J.factor_high, J.factor_low);
+      }
+      const uint32_t pixel = site.pixel & kSitePixelMask;
+      if (changed) {
+        uint16_t* o = J.out + (size_t)pixel * 3;
+        o[0] = rendered[0];
+        o[1] = rendered[1];
+        o[2] = rendered[2];
+      }
+      if (J.diag_changed) J.diag_changed[pixel] = changed ? 1 : 0;
+    }
+    if (error) atomicOr(J.error_flags, error);
+  }
+  const unsigned long long written_mask = __ballot(written);
+  const unsigned long long changed_mask = __ballot(changed);
+  if ((threadIdx.x & 63) == 0) {
+    if (written_mask) atomicAdd(&J.writer_counts[0], (unsigned long long)__popcll(written_mask));
+    if (changed_mask) atomicAdd(&J.writer_counts[1], (unsigned long long)__popcll(changed_mask));
+  }
+}
+
+// The same check over the six hidden startup rows, which write no output:
+// each attempted site, run from its recorded start state, must end in the
+// state recorded for the next index (a skipped index passes the state on).
+__global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J, WriterConsts K,
+                                                                   uint32_t count,
+                                                                   uint32_t final_state) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t state = J.su_site_state[i];
+  if (J.su_attempted[i]) {
+    const Site site = J.su_sites[i];
+    float values[3];
+    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, K, state, values);
+  }
+  const uint32_t expected = i + 1 < count ? J.su_site_state[i + 1] : final_state;
+  if (state != expected) atomicOr(J.error_flags, kErrWriterChain);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // Writes the host writer's changed values into the device output.
 __global__ __launch_bounds__(kThreads) void k_apply_patches(uint16_t* out,
                                                             const uint32_t* pixels,
@@ -1112,6 +1200,31 @@ hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t strea
                      dim3(kThreads), 0, stream, J, P, rows);
   hipLaunchKernelGGL(k_startup_sites, dim3(blocks_for(6L * J.width, kThreads)),
                      dim3(kThreads), 0, stream, J, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_writer_prep(const Site* sites, const uint8_t* attempted,
+                              WriterRecord* records, uint32_t count, const WriterConsts& K,
+                              hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_writer_prep, dim3(blocks_for(count, kThreads)), dim3(kThreads), 0,
+                     stream, sites, attempted, records, count, K);
+  return hipGetLastError();
+}
+
+hipError_t launch_writer_values(const JobDev& J, const WriterConsts& K, uint32_t count,
+                                uint32_t startup_final_state, uint32_t final_state,
+                                hipStream_t stream) {
+  const uint32_t startup_count = 6u * (uint32_t)J.width;
+  hipLaunchKernelGGL(k_writer_startup_check, dim3(blocks_for(startup_count, kThreads)),
+                     dim3(kThreads), 0, stream, J, K, startup_count, startup_final_state);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, J.writer_counts, 2);
+  if (J.diag_changed)
+    hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(kThreads), 0, stream, J.diag_changed,
+                       (size_t)J.height * J.width, (uint8_t)0);
+  if (count)
+    hipLaunchKernelGGL(k_writer_values, dim3(blocks_for(count, kThreads)), dim3(kThreads), 0,
+                       stream, J, K, count, final_state);
   return hipGetLastError();
 }
 

@@ -31,6 +31,12 @@ bool tile_shape_supported(int ty, int tx);
 hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream);
 hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream);
 hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream);
+hipError_t launch_writer_prep(const Site* sites, const uint8_t* attempted,
+                              WriterRecord* records, uint32_t count, const WriterConsts& K,
+                              hipStream_t stream);
+hipError_t launch_writer_values(const JobDev& J, const WriterConsts& K, uint32_t count,
+                                uint32_t startup_final_state, uint32_t final_state,
+                                hipStream_t stream);
 hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
                                 const uint16_t* values, uint32_t count,
                                 hipStream_t stream);
