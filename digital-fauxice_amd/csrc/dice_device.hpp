@@ -40,10 +40,18 @@ constexpr uint32_t kSitePixelMask = 0x7FFFFFFFu;
 constexpr uint32_t kSiteFloorBit = 0x80000000u;
 constexpr uint32_t kRecordAttempted = 0x80000000u;
 
-// Number of u in [0, 2^24) on the true prefix of a monotone predicate.
+// Written/changed counters are spread over slots 128 B apart so that the
+// per-workgroup atomics do not all hit one address.
+constexpr int kCountSlots = 64;
+constexpr int kCountStride = 16;  // u64 per slot
+
+// Number of u in [0, 2^24) on the true prefix of a monotone predicate
+// (the two ends first: most predicates hold everywhere or nowhere).
 template <class F>
 DICE_HD uint32_t prefix_count(const F& holds) {
-  uint32_t lo = 0, hi = 1u << 24;
+  if (!holds(0u)) return 0;
+  if (holds(kLcgMask)) return 1u << 24;
+  uint32_t lo = 1, hi = kLcgMask;
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
     if (holds(mid))
@@ -72,10 +80,15 @@ DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const Write
                      finite64(site.cand[2]) && site.cand[0] > 0.0 && site.cand[1] > 0.0 &&
                      site.cand[2] > 0.0;
   if (valid) {
-    r.flags = 1u | ((site.pixel & kSiteFloorBit) ? 2u : 0u);
+    const bool floor = (site.pixel & kSiteFloorBit) != 0;
+    r.flags = 1u | (floor ? 2u : 0u);
     for (int c = 0; c < 3; ++c) {
       const double cd = (double)(float)site.cand[c];
       if (!(K.low_lt_high && K.low < cd && cd < K.high)) continue;
+      if (!floor) {  // one draw, whatever its value
+        r.flags |= 1u << (2 + 4 * c);
+        continue;
+      }
       const double width = K.high - K.low;
       const double coefficient = 4.0 / (width * width);
       const double envelope = ((K.high - cd) * (cd - K.low)) * coefficient;
@@ -175,12 +188,13 @@ struct JobDev {
   uint32_t* tile_count;         // sites per tile
   uint32_t* seg_count;          // (tiles_y*TY) * tiles_x
   uint32_t* seg_offset;         // (tiles_y*TY) * tiles_x
+  uint32_t* seg_block_sums;     // one per 1024 segments
   Site* sites;                  // compacted, row-major
   WriterRecord* writer_records; // one per site, row-major
   WriterRecord* su_writer_records; // 6*W
   uint32_t* site_state;         // LCG state at the start of each site
   uint32_t* su_site_state;      // 6*W, the same for the hidden startup rows
-  unsigned long long* writer_counts; // [0] written, [1] changed
+  unsigned long long* writer_counts; // kCountSlots slots of {written, changed}
   uint8_t* diag_changed;        // optional H*W
   uint32_t* counters;           // [0] total sites
   uint32_t* error_flags;
@@ -196,7 +210,6 @@ struct JobDev {
   uint8_t* failpos;             // ceil(H/8) * (W/8)
   double* row_sums;             // H * 3 (visible, infrared, weight)
   uint32_t* accepted;           // H
-  double* epoch_terms;          // epochs * cells * 8
   double* epoch_adds;           // epochs * 2 (den, num)
   double* cumulative;           // H * 3
   double* epoch_wide;           // epochs * 2 (den_w, num_w)

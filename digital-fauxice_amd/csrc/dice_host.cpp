@@ -204,14 +204,14 @@ struct dice_ctx {
       flags;
   DeviceBuffer diag_score, diag_floor;
   DeviceBuffer su_aux, su_score, su_work, su_sites, su_attempted;
-  DeviceBuffer failpos, row_sums, accepted, epoch_terms, epoch_adds, cumulative,
+  DeviceBuffer failpos, row_sums, accepted, epoch_adds, cumulative, seg_block_sums,
       epoch_wide, epoch_store;
   DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse,
       st_gate, st_row_gate;
   DeviceBuffer writer_records, su_writer_records, site_state, writer_counts, diag_changed;
   DeviceBuffer sched[12];
   // pinned host staging
-  PinnedBuffer h_counters, h_records, h_site_state, h_out;
+  PinnedBuffer h_counters, h_counts, h_records, h_site_state, h_out;
 };
 
 namespace {
@@ -355,7 +355,7 @@ int32_t prepare_job(dice_ctx* ctx, int64_t H, int64_t W, bool want_diag, JobDev&
   HIP_TRY(ctx->failpos.ensure(fail_rows * cells + 8));
   HIP_TRY(ctx->row_sums.ensure(H * 3 * 8));
   HIP_TRY(ctx->accepted.ensure(H * 4));
-  HIP_TRY(ctx->epoch_terms.ensure(epochs * cells * 64 + 64));
+  HIP_TRY(ctx->seg_block_sums.ensure((segments / 1024 + 2) * 4));
   HIP_TRY(ctx->epoch_adds.ensure(epochs * 16 + 16));
   HIP_TRY(ctx->cumulative.ensure(H * 24));
   HIP_TRY(ctx->epoch_wide.ensure(epochs * 16 + 16));
@@ -400,7 +400,7 @@ int32_t prepare_job(dice_ctx* ctx, int64_t H, int64_t W, bool want_diag, JobDev&
   J.failpos = ctx->failpos.as<uint8_t>();
   J.row_sums = ctx->row_sums.as<double>();
   J.accepted = ctx->accepted.as<uint32_t>();
-  J.epoch_terms = ctx->epoch_terms.as<double>();
+  J.seg_block_sums = ctx->seg_block_sums.as<uint32_t>();
   J.epoch_adds = ctx->epoch_adds.as<double>();
   J.cumulative = ctx->cumulative.as<double>();
   J.epoch_wide = ctx->epoch_wide.as<double>();
@@ -450,6 +450,8 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   if (int32_t e = ev.record(2)) return e;
   HIP_TRY(launch_scan(J, P, s));
   if (int32_t e = ev.record(3)) return e;
+  HIP_TRY(launch_scan_segments(J, s));
+  if (int32_t e = ev.record(10)) return e;
   uint32_t* hc = ctx->h_counters.as<uint32_t>();
   HIP_TRY(hipMemcpyAsync(hc, J.counters, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
@@ -464,7 +466,7 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   HIP_TRY(ctx->writer_records.ensure(((size_t)count + 1) * sizeof(WriterRecord)));
   HIP_TRY(ctx->su_writer_records.ensure(startup_sites * sizeof(WriterRecord)));
   HIP_TRY(ctx->site_state.ensure(((size_t)count + startup_sites + 1) * 4));
-  HIP_TRY(ctx->writer_counts.ensure(64));
+  HIP_TRY(ctx->writer_counts.ensure(kCountSlots * kCountStride * 8));
   J.sites = ctx->sites.as<Site>();
   J.writer_records = ctx->writer_records.as<WriterRecord>();
   J.su_writer_records = ctx->su_writer_records.as<WriterRecord>();
@@ -499,6 +501,7 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   if (timings) {
     timings->analysis_device += t1 - t0;
     timings->scan_kernel_ms += ev.elapsed_ms(2, 3);
+    timings->segments_kernel_ms += ev.elapsed_ms(3, 10);
     timings->sites_kernel_ms += ev.elapsed_ms(4, 5);
     timings->startup_kernels_ms += ev.elapsed_ms(5, 6);
     timings->writer_prep_kernel_ms += ev.elapsed_ms(6, 7);
@@ -542,8 +545,10 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
   if (int32_t e = ev.record(8)) return e;
   HIP_TRY(launch_writer_values(J, K, count, startup_final, final_state, s));
   if (int32_t e = ev.record(9)) return e;
-  unsigned long long* h_counts = reinterpret_cast<unsigned long long*>(hc + 4);
-  HIP_TRY(hipMemcpyAsync(h_counts, J.writer_counts, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx->h_counts.ensure(kCountSlots * kCountStride * 8));
+  const unsigned long long* h_slots = ctx->h_counts.as<unsigned long long>();
+  HIP_TRY(hipMemcpyAsync(ctx->h_counts.ptr, J.writer_counts, kCountSlots * kCountStride * 8,
+                         hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   double t3 = now_seconds();
@@ -586,12 +591,17 @@ int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng
     if (diag->changed_mask)
       HIP_TRY(hipMemcpy(diag->changed_mask, J.diag_changed, pixels, hipMemcpyDeviceToHost));
   }
+  uint64_t written = 0, changed = 0;
+  for (int slot = 0; slot < kCountSlots; ++slot) {
+    written += h_slots[slot * kCountStride + 0];
+    changed += h_slots[slot * kCountStride + 1];
+  }
   result->attempted_pixels = count;
-  result->written_pixels = h_counts[0];
+  result->written_pixels = written;
   result->public_rng_advances = advances;
-  result->changed_pixels = h_counts[1];
+  result->changed_pixels = changed;
   result->final_rng_state = final_state;
-  cb.progress(3, H, H, (int64_t)count, (int64_t)h_counts[0]);
+  cb.progress(3, H, H, (int64_t)count, (int64_t)written);
   return DICE_OK;
 }
 

@@ -120,111 +120,39 @@ __device__ __forceinline__ void rgb_candidates(const WrgbAt& at,
 // Producer (producer_parameters.py:266-495)
 // ==========================================================================
 
-// First failing pixel (IR <= threshold) of every 8x8 (block, cell) in
-// row-major order, 64 when none fails: the cell-validity latch.
-__global__ __launch_bounds__(kThreads) void k_failpos(JobDev J) {
-  const int cells = J.width / 8;
-  const int blocks = (J.height + 7) / 8;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)blocks * cells) return;
-  const int b = (int)(idx / cells), c = (int)(idx % cells);
-  const double threshold = (double)kProducerThreshold;
-  int fail = 64;
-  for (int r = 0; r < 8 && fail == 64; ++r) {
-    int y = b * 8 + r;
-    if (y >= J.height) break;
-    for (int k = 0; k < 8; ++k) {
-      uint32_t lanes[4];
-      load_pixel(J.rgbi, (size_t)y * J.width + c * 8 + k, lanes);
-      if (!(threshold < (double)lanes[3])) {
-        fail = r * 8 + k;
-        break;
-      }
-    }
-  }
-  J.failpos[idx] = (uint8_t)fail;
-}
-
-// Per-row column-ordered binary64 sums (one lane per row, rows staged
-// through LDS for coalesced loads).
-constexpr int kRowSumRows = 64;
-constexpr int kRowSumCols = 32;
-
-__global__ __launch_bounds__(64) void k_row_sums(JobDev J) {
-  __shared__ uint32_t tile[kRowSumRows][kRowSumCols + 1];
-  const int lane = threadIdx.x;
-  const int row0 = blockIdx.x * kRowSumRows;
-  const int y = row0 + lane;
-  const int cells = J.width / 8;
-  const int active = cells * 8;
-  double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
-  uint32_t accepted = 0;
-  const uint8_t* fail_row = J.failpos + (size_t)(y / 8) * cells;
-  const int cell_row = (y % 8) * 8;
-  for (int c0 = 0; c0 < active; c0 += kRowSumCols) {
-    for (int k = 0; k < kRowSumRows * kRowSumCols / 64; ++k) {
-      int e = k * 64 + lane;
-      int r = e / kRowSumCols, c = e % kRowSumCols;
-      int yy = row0 + r, xx = c0 + c;
-      uint32_t v = 0;
-      if (yy < J.height && xx < active) {
-        uint32_t lanes[4];
-        load_pixel(J.rgbi, (size_t)yy * J.width + xx, lanes);
-        v = lanes[0] | (lanes[3] << 16);
-      }
-      tile[r][c] = v;
-    }
-    __syncthreads();
-    if (y < J.height) {
-      const int ncols = min(kRowSumCols, active - c0);
-      for (int c = 0; c < ncols; ++c) {
-        const int x = c0 + c;
-        if (cell_row + (x & 7) >= (int)fail_row[x >> 3]) continue;
-        const uint32_t v = tile[lane][c];
-        const uint32_t red = v & 0xFFFFu, infrared = v >> 16;
-        const double weight = (double)((int64_t)infrared * (int64_t)infrared);
-        sum_visible += weight * (double)J.lut[red];
-        sum_infrared += weight * (double)J.lut[infrared];
-        sum_weight += weight;
-        accepted += 1;
-      }
-    }
-    __syncthreads();
-  }
-  if (y < J.height) {
-    J.row_sums[(size_t)y * 3 + 0] = sum_visible;
-    J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
-    J.row_sums[(size_t)y * 3 + 2] = sum_weight;
-    J.accepted[y] = accepted;
-  }
-}
-
 namespace {
-// producer_parameters.py:136-167: the asymmetric 4x4 tree per quadrant.
-__device__ __forceinline__ void producer_quadrants(const double v[8][8],
-                                                   float out[4]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int r0 = (q >> 1) * 4, c0 = (q & 1) * 4;
-    double first = v[r0][c0];
-    first += v[r0][c0 + 1];
-    first += v[r0][c0 + 2];
-    first += v[r0][c0 + 3];
-    first += v[r0 + 1][c0];
-    first += v[r0 + 1][c0 + 1];
-    double second_tail = v[r0 + 1][c0 + 2] + v[r0 + 1][c0 + 3];
-    first += second_tail;
-    double third = v[r0 + 2][c0] + v[r0 + 2][c0 + 1];
-    double third_tail = v[r0 + 2][c0 + 2] + v[r0 + 2][c0 + 3];
-    third += third_tail;
-    double fourth = v[r0 + 3][c0] + v[r0 + 3][c0 + 1];
-    double fourth_tail = v[r0 + 3][c0 + 2] + v[r0 + 3][c0 + 3];
-    fourth += fourth_tail;
-    third += fourth;
-    double total = first + third;
-    out[q] = (float)(total * 0.0625);
+// producer_parameters.py:136-167, evaluated one row of a 4x4 quadrant at a
+// time: rows 0-1 feed `first` (six single adds, then the paired tail),
+// rows 2 and 3 form `third` and `fourth` from pairs; the quadrant total is
+// first + (third + fourth) and its mean f32(total * 0.0625).
+struct QuadrantTree {
+  double first, third;
+  __device__ __forceinline__ void row(int local_row, double a0, double a1, double a2,
+                                      double a3, float* mean) {
+    if (local_row == 0) {
+      first = a0;
+      first += a1;
+      first += a2;
+      first += a3;
+    } else if (local_row == 1) {
+      first += a0;
+      first += a1;
+      const double tail = a2 + a3;
+      first += tail;
+    } else if (local_row == 2) {
+      third = a0 + a1;
+      const double tail = a2 + a3;
+      third += tail;
+    } else {
+      double fourth = a0 + a1;
+      const double tail = a2 + a3;
+      fourth += tail;
+      third += fourth;
+      const double total = first + third;
+      *mean = (float)(total * 0.0625);
+    }
   }
-}
+};
 
 // producer_parameters.py:170-180
 __device__ __forceinline__ void producer_center(const float m[4], float out[4]) {
@@ -238,102 +166,257 @@ __device__ __forceinline__ void producer_center(const float m[4], float out[4]) 
 }
 }  // namespace
 
-// producer_parameters.py:209-262: the four (term, ratio*term) pairs of every
-// whole 8x8 block of every complete epoch; skipped blocks are marked by
-// failpos and contribute nothing.
-__global__ __launch_bounds__(kThreads) void k_epoch_terms(JobDev J) {
-  const int cells = J.width / 8;
-  const int epochs = J.height / 8;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)epochs * cells) return;
-  const int e = (int)(idx / cells), c = (int)(idx % cells);
-  if (J.failpos[(size_t)e * cells + c] != 64) return;
-  double visible[8][8], infrared[8][8];
-  float raw_sum = 0.0f;
-  for (int r = 0; r < 8; ++r)
-    for (int k = 0; k < 8; ++k) {
-      uint32_t lanes[4];
-      load_pixel(J.rgbi, (size_t)(e * 8 + r) * J.width + c * 8 + k, lanes);
-      visible[r][k] = (double)J.lut[lanes[0]];
-      infrared[r][k] = (double)J.lut[lanes[3]];
-      raw_sum = (float)((double)raw_sum + (double)lanes[3]);
-    }
-  float vm[4], im[4], vd[4], id[4];
-  producer_quadrants(visible, vm);
-  producer_quadrants(infrared, im);
-  producer_center(vm, vd);
-  producer_center(im, id);
-  const double low = (double)-0.125f, full_low = (double)0.0f;
-  const double full_high = (double)0.3f, high = (double)0.425f;
-  const double outer_weight = (double)0.70710677f;
-  const double raw_wide = (double)raw_sum;
-  double* out = J.epoch_terms + idx * 8;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    double visible_wide = (double)vd[q];
-    double ratio, weight;
-    if (visible_wide == 0.0) {
-      ratio = 0.0;
-      weight = 0.0;
-    } else {
-      ratio = (double)id[q] / visible_wide;
-      if (ratio < low || ratio > high)
-        weight = 0.0;
-      else if (ratio < full_low || ratio > full_high)
-        weight = visible_wide * outer_weight;
-      else
-        weight = visible_wide;
-    }
-    double term = weight * weight;
-    term *= raw_wide;
-    term *= raw_wide;
-    out[q * 2 + 0] = term;
-    out[q * 2 + 1] = ratio * term;
-  }
-}
+// One workgroup per 8-row block b.  Every thread owns one 8x8 cell (the
+// 64 B of each of its rows are contiguous, so a wave reads 4 KB runs):
+//   * failpos[b][cell]: first pixel with IR <= threshold in row-major order
+//     within the cell, 64 when none -- the 8-row validity latch the row
+//     sums apply (producer_parameters.py:303-340);
+//   * for a complete epoch, the four (term, ratio*term) pairs of a block
+//     whose 64 IR samples all pass (producer_parameters.py:209-262), zero
+//     otherwise -- the sums start at +0.0 and can never become -0.0, so a
+//     +0.0 term is the exact identity of the skipped block;
+//   * thread 0 then adds the pairs in cell-then-quadrant order and narrows
+//     the epoch's two additions once.
+constexpr int kProducerThreads = 256;
 
-// Per-epoch sums in block then quadrant order (one lane per epoch).
-__global__ __launch_bounds__(kThreads) void k_epoch_sums(JobDev J) {
-  const int cells = J.width / 8;
-  const int epochs = J.height / 8;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= epochs) return;
+__global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobDev J) {
+  __shared__ double pairs[kProducerThreads][8];
+  const int H = J.height, W = J.width;
+  const int cells = W / 8;
+  const int b = blockIdx.x;
+  const bool epoch = b < H / 8;
+  const int tid = threadIdx.x;
+  const double threshold = (double)kProducerThreshold;
   double denominator = 0.0, numerator = 0.0;
-  for (int c = 0; c < cells; ++c) {
-    if (J.failpos[(size_t)e * cells + c] != 64) continue;
-    const double* t = J.epoch_terms + ((size_t)e * cells + c) * 8;
+  for (int c0 = 0; c0 < cells; c0 += kProducerThreads) {
+    const int c = c0 + tid;
+    if (c < cells) {
+      int fail = 64;
+      float raw_sum = 0.0f;
+      QuadrantTree vis[2], inf[2];
+      float vm[4] = {0.0f, 0.0f, 0.0f, 0.0f}, im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int r = 0; r < 8; ++r) {
+        const int y = b * 8 + r;
+        if (y >= H) break;
+        const uint2* row = reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + c * 8) * 4);
+        uint2 q[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      denominator += t[q * 2 + 0];
-      numerator += t[q * 2 + 1];
+        for (int k = 0; k < 8; ++k) q[k] = row[k];
+        double v[8], ir[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t red = q[k].x & 0xFFFFu, infrared = q[k].y >> 16;
+          if (fail == 64 && !(threshold < (double)infrared)) fail = r * 8 + k;
+          raw_sum = (float)((double)raw_sum + (double)infrared);
+          v[k] = (double)J.lut[red];
+          ir[k] = (double)J.lut[infrared];
+        }
+        const int local = r & 3, half = (r >> 2) * 2;
+        vis[0].row(local, v[0], v[1], v[2], v[3], &vm[half]);
+        vis[1].row(local, v[4], v[5], v[6], v[7], &vm[half + 1]);
+        inf[0].row(local, ir[0], ir[1], ir[2], ir[3], &im[half]);
+        inf[1].row(local, ir[4], ir[5], ir[6], ir[7], &im[half + 1]);
+      }
+      J.failpos[(size_t)b * cells + c] = (uint8_t)fail;
+      if (epoch) {
+        double* out = pairs[tid];
+        if (fail != 64) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) out[j] = 0.0;
+        } else {
+          float vd[4], id[4];
+          producer_center(vm, vd);
+          producer_center(im, id);
+          const double low = (double)-0.125f, full_low = (double)0.0f;
+          const double full_high = (double)0.3f, high = (double)0.425f;
+          const double outer_weight = (double)0.70710677f;
+          const double raw_wide = (double)raw_sum;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double visible_wide = (double)vd[q];
+            double ratio, weight;
+            if (visible_wide == 0.0) {
+              ratio = 0.0;
+              weight = 0.0;
+            } else {
+              ratio = (double)id[q] / visible_wide;
+              if (ratio < low || ratio > high)
+                weight = 0.0;
+              else if (ratio < full_low || ratio > full_high)
+                weight = visible_wide * outer_weight;
+              else
+                weight = visible_wide;
+            }
+            double term = weight * weight;
+            term *= raw_wide;
+            term *= raw_wide;
+            out[q * 2 + 0] = term;
+            out[q * 2 + 1] = ratio * term;
+          }
+        }
+      }
     }
+    if (!epoch) continue;
+    __syncthreads();
+    if (tid == 0) {
+      const int n = min(kProducerThreads, cells - c0);
+      for (int j = 0; j < n; ++j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          denominator += pairs[j][q * 2 + 0];
+          numerator += pairs[j][q * 2 + 1];
+        }
+      }
+    }
+    __syncthreads();
   }
-  J.epoch_adds[e * 2 + 0] = (double)(float)denominator;
-  J.epoch_adds[e * 2 + 1] = (double)(float)numerator;
+  if (epoch && tid == 0) {
+    J.epoch_adds[b * 2 + 0] = (double)(float)denominator;
+    J.epoch_adds[b * 2 + 1] = (double)(float)numerator;
+  }
 }
 
-// The strictly sequential cross-row chains: cumulative row sums (lanes
-// 0..2 of wave 0) and the epoch accumulators (lane 0 of wave 1).
-__global__ __launch_bounds__(128) void k_producer_chain(JobDev J) {
-  const int t = threadIdx.x;
-  if (t < 3) {
-    double total = 0.0;
-    for (int y = 0; y < J.height; ++y) {
-      total += J.row_sums[(size_t)y * 3 + t];
-      J.cumulative[(size_t)y * 3 + t] = total;
+// Per-row binary64 sums in column order (producer_parameters.py:303-360):
+// a workgroup owns kRowSumRows rows.  Four staging waves load 128-column
+// chunks (coalesced), apply the cell latch and gather the two LUT values;
+// one summing wave runs the row chains (one lane per row) over the
+// previous chunk, so chunk loads overlap the dependent adds.  A rejected
+// pixel is staged as weight 0, whose three products are +0.0: the exact
+// identity of the non-negative sums.  Three LDS buffers keep the staging
+// of chunk k+1 clear of the chunk still being summed.
+constexpr int kRowSumRows = 16;
+constexpr int kRowSumCols = 128;
+constexpr int kRowSumStagers = 256;
+
+__global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobDev J) {
+  __shared__ float s_visible[3][kRowSumRows][kRowSumCols + 1];
+  __shared__ float s_infrared[3][kRowSumRows][kRowSumCols + 1];
+  __shared__ uint32_t s_weight[3][kRowSumRows][kRowSumCols + 1];
+  const int H = J.height, W = J.width;
+  const int cells = W / 8, active = cells * 8;
+  const int row0 = blockIdx.x * kRowSumRows;
+  const int tid = threadIdx.x;
+  const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
+  double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
+  uint32_t accepted = 0;
+  for (int k = 0; k <= chunks; ++k) {
+    if (tid >= 64 && k < chunks) {
+      const int buffer = k % 3, c0 = k * kRowSumCols;
+      constexpr int kPer = kRowSumRows * kRowSumCols / kRowSumStagers;
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const int e = (tid - 64) + i * kRowSumStagers;
+        const int r = e / kRowSumCols, c = e % kRowSumCols;
+        const int y = row0 + r, x = c0 + c;
+        float lut_visible = 0.0f, lut_infrared = 0.0f;
+        uint32_t weight = 0;
+        if (y < H && x < active) {
+          uint32_t lanes[4];
+          load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+          const int position = (y & 7) * 8 + (x & 7);
+          if (position < (int)J.failpos[(size_t)(y >> 3) * cells + (x >> 3)]) {
+            lut_visible = J.lut[lanes[0]];
+            lut_infrared = J.lut[lanes[3]];
+            weight = lanes[3] * lanes[3];
+          }
+        }
+        s_visible[buffer][r][c] = lut_visible;
+        s_infrared[buffer][r][c] = lut_infrared;
+        s_weight[buffer][r][c] = weight;
+      }
     }
-  } else if (t == 64) {
+    __syncthreads();
+    if (tid < kRowSumRows && k >= 1) {
+      const int buffer = (k - 1) % 3;
+#pragma unroll 8
+      for (int c = 0; c < kRowSumCols; ++c) {
+        const uint32_t w = s_weight[buffer][tid][c];
+        const double weight = (double)w;
+        sum_visible += weight * (double)s_visible[buffer][tid][c];
+        sum_infrared += weight * (double)s_infrared[buffer][tid][c];
+        sum_weight += weight;
+        accepted += w != 0u;
+      }
+    }
+  }
+  const int y = row0 + tid;
+  if (tid < kRowSumRows && y < H) {
+    J.row_sums[(size_t)y * 3 + 0] = sum_visible;
+    J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
+    J.row_sums[(size_t)y * 3 + 2] = sum_weight;
+    J.accepted[y] = accepted;
+  }
+}
+
+// The strictly sequential cross-row chains (producer_parameters.py:
+// 358-360, 437-456).  Workgroup 0: the cumulative row sums (lanes 0..2);
+// workgroup 1: the epoch accumulators (lane 0).  Each chain reads its
+// inputs through LDS in chunks the whole wave loads (the next chunk in
+// flight while the current one is summed).
+constexpr int kChainChunk = 512;
+
+__global__ __launch_bounds__(64) void k_producer_chain(JobDev J) {
+  __shared__ double in_buffer[kChainChunk * 3];
+  __shared__ double out_buffer[kChainChunk * 3];
+  __shared__ float out_store[kChainChunk * 2];
+  const int lane = threadIdx.x;
+  constexpr int kPer = kChainChunk * 3 / 64;
+  if (blockIdx.x == 0) {
+    const int H = J.height;
+    const double* __restrict__ source = J.row_sums;
+    double total = 0.0;
+    double next[kPer];
+    auto load = [&](int base) {
+      const int n = min(kChainChunk, H - base) * 3;
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const int i = lane + j * 64;
+        next[j] = i < n ? source[(size_t)base * 3 + i] : 0.0;
+      }
+    };
+    load(0);
+    for (int base = 0; base < H; base += kChainChunk) {
+      const int n = min(kChainChunk, H - base);
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) in_buffer[lane + j * 64] = next[j];
+      __syncthreads();
+      if (base + kChainChunk < H) load(base + kChainChunk);
+      if (lane < 3) {
+        for (int r = 0; r < n; ++r) {
+          total += in_buffer[r * 3 + lane];
+          out_buffer[r * 3 + lane] = total;
+        }
+      }
+      __syncthreads();
+      for (int i = lane; i < n * 3; i += 64) J.cumulative[(size_t)base * 3 + i] = out_buffer[i];
+      __syncthreads();
+    }
+  } else {
     const int epochs = J.height / 8;
+    const double* __restrict__ source = J.epoch_adds;
     float denominator = 0.0f, numerator = 0.0f;
-    for (int e = 0; e < epochs; ++e) {
-      double denominator_wide = (double)denominator + J.epoch_adds[e * 2 + 0];
-      double numerator_wide = (double)numerator + J.epoch_adds[e * 2 + 1];
-      denominator = (float)denominator_wide;
-      numerator = (float)numerator_wide;
-      J.epoch_wide[e * 2 + 0] = denominator_wide;
-      J.epoch_wide[e * 2 + 1] = numerator_wide;
-      J.epoch_store[e * 2 + 0] = denominator;
-      J.epoch_store[e * 2 + 1] = numerator;
+    for (int base = 0; base < epochs; base += kChainChunk) {
+      const int n = min(kChainChunk, epochs - base);
+      for (int i = lane; i < n * 2; i += 64) in_buffer[i] = source[(size_t)base * 2 + i];
+      __syncthreads();
+      if (lane == 0) {
+        for (int e = 0; e < n; ++e) {
+          const double denominator_wide = (double)denominator + in_buffer[e * 2 + 0];
+          const double numerator_wide = (double)numerator + in_buffer[e * 2 + 1];
+          denominator = (float)denominator_wide;
+          numerator = (float)numerator_wide;
+          out_buffer[e * 2 + 0] = denominator_wide;
+          out_buffer[e * 2 + 1] = numerator_wide;
+          out_store[e * 2 + 0] = denominator;
+          out_store[e * 2 + 1] = numerator;
+        }
+      }
+      __syncthreads();
+      for (int i = lane; i < n * 2; i += 64) {
+        J.epoch_wide[(size_t)base * 2 + i] = out_buffer[i];
+        J.epoch_store[(size_t)base * 2 + i] = out_store[i];
+      }
+      __syncthreads();
     }
   }
 }
@@ -737,29 +820,59 @@ __global__ __launch_bounds__(kThreads) void k_sites(JobDev J, DevParams P) {
 
 // Exclusive scan of the (row, tile-column) segment counts in row-major
 // order: the site order the serial writer requires (host_writer.py:73-93).
-__global__ __launch_bounds__(1024) void k_scan_segments(JobDev J) {
-  __shared__ uint32_t partial[1024];
+// Three passes: per-1024-segment sums, one workgroup scanning those sums
+// (and writing the total), and the per-block rescan with its offset.
+constexpr int kSegPerBlock = 1024;
+
+__global__ __launch_bounds__(kThreads) void k_seg_partials(JobDev J) {
+  __shared__ uint32_t wave_sums[kThreads / 64];
   const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
-  const int t = threadIdx.x;
-  const long chunk = (n + 1023) / 1024;
-  const long begin = t * chunk;
-  const long end = min(n, begin + chunk);
+  const long base = (long)blockIdx.x * kSegPerBlock;
   uint32_t sum = 0;
-  for (long i = begin; i < end; ++i) sum += J.seg_count[i];
-  partial[t] = sum;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    uint32_t v = t >= d ? partial[t - d] : 0u;
-    __syncthreads();
-    partial[t] += v;
-    __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSegPerBlock / kThreads; ++j) {
+    const long i = base + threadIdx.x + j * kThreads;
+    if (i < n) sum += J.seg_count[i];
   }
-  uint32_t offset = partial[t] - sum;
-  for (long i = begin; i < end; ++i) {
-    J.seg_offset[i] = offset;
-    offset += J.seg_count[i];
+  uint32_t total;
+  block_exclusive_scan(sum, wave_sums, &total);
+  if (threadIdx.x == 0) J.seg_block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_seg_top(JobDev J, int blocks) {
+  __shared__ uint32_t wave_sums[kThreads / 64];
+  uint32_t carry = 0;
+  for (int base = 0; base < blocks; base += kThreads) {
+    const int i = base + threadIdx.x;
+    const uint32_t value = i < blocks ? J.seg_block_sums[i] : 0u;
+    uint32_t total;
+    const uint32_t offset = block_exclusive_scan(value, wave_sums, &total);
+    if (i < blocks) J.seg_block_sums[i] = carry + offset;
+    carry += total;
   }
-  if (t == 1023) J.counters[0] = partial[1023];
+  if (threadIdx.x == 0) J.counters[0] = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_seg_offsets(JobDev J) {
+  __shared__ uint32_t wave_sums[kThreads / 64];
+  const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+  constexpr int kPer = kSegPerBlock / kThreads;
+  // each thread owns kPer consecutive segments so the rescan stays ordered
+  const long first = (long)blockIdx.x * kSegPerBlock + (long)threadIdx.x * kPer;
+  uint32_t values[kPer];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    values[j] = first + j < n ? J.seg_count[first + j] : 0u;
+    sum += values[j];
+  }
+  uint32_t total;
+  uint32_t offset = J.seg_block_sums[blockIdx.x] + block_exclusive_scan(sum, wave_sums, &total);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    if (first + j < n) J.seg_offset[first + j] = offset;
+    offset += values[j];
+  }
 }
 
 // ==========================================================================
@@ -949,11 +1062,22 @@ __global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, WriterCons
     }
     if (error) atomicOr(J.error_flags, error);
   }
+  __shared__ uint32_t wave_counts[kThreads / 64][2];
   const unsigned long long written_mask = __ballot(written);
   const unsigned long long changed_mask = __ballot(changed);
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    if (written_mask) atomicAdd(&J.writer_counts[0], (unsigned long long)__popcll(written_mask));
-    if (changed_mask) atomicAdd(&J.writer_counts[1], (unsigned long long)__popcll(changed_mask));
+    wave_counts[wave][0] = (uint32_t)__popcll(written_mask);
+    wave_counts[wave][1] = (uint32_t)__popcll(changed_mask);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) total += wave_counts[w][threadIdx.x];
+    if (total)
+      atomicAdd(&J.writer_counts[(blockIdx.x % kCountSlots) * kCountStride + threadIdx.x],
+                (unsigned long long)total);
   }
 }
 
@@ -980,8 +1104,8 @@ __global__ __launch_bounds__(kThreads) void k_fill_u8(uint8_t* p, size_t n, uint
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+__global__ __launch_bounds__(kThreads) void k_zero_u64(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
 // Writes the host writer's changed values into the device output.
@@ -1128,20 +1252,11 @@ static inline unsigned blocks_for(long n, int threads) {
 }
 
 hipError_t launch_producer(const JobDev& J, float factor_b, hipStream_t stream) {
-  const int cells = J.width / 8;
-  const long fail_items = (long)((J.height + 7) / 8) * cells;
-  const int epochs = J.height / 8;
-  hipLaunchKernelGGL(k_failpos, dim3(blocks_for(fail_items, kThreads)), dim3(kThreads),
-                     0, stream, J);
+  const int blocks = (J.height + 7) / 8;
+  hipLaunchKernelGGL(k_producer_blocks, dim3(blocks), dim3(kProducerThreads), 0, stream, J);
   hipLaunchKernelGGL(k_row_sums, dim3(blocks_for(J.height, kRowSumRows)),
-                     dim3(kRowSumRows), 0, stream, J);
-  if (epochs > 0) {
-    hipLaunchKernelGGL(k_epoch_terms, dim3(blocks_for((long)epochs * cells, kThreads)),
-                       dim3(kThreads), 0, stream, J);
-    hipLaunchKernelGGL(k_epoch_sums, dim3(blocks_for(epochs, kThreads)), dim3(kThreads),
-                       0, stream, J);
-  }
-  hipLaunchKernelGGL(k_producer_chain, dim3(1), dim3(128), 0, stream, J);
+                     dim3(64 + kRowSumStagers), 0, stream, J);
+  hipLaunchKernelGGL(k_producer_chain, dim3(2), dim3(64), 0, stream, J);
   hipLaunchKernelGGL(k_stage_table, dim3(blocks_for(J.height + 6, kThreads)),
                      dim3(kThreads), 0, stream, J, factor_b);
   return hipGetLastError();
@@ -1158,7 +1273,15 @@ template <int TY, int TX>
 static hipError_t launch_scan_shape(const JobDev& J, const DevParams& P, hipStream_t stream) {
   const int tiles = J.tiles_x * J.tiles_y;
   hipLaunchKernelGGL((k_scan<TY, TX>), dim3(tiles), dim3(kThreads), 0, stream, J, P);
-  hipLaunchKernelGGL(k_scan_segments, dim3(1), dim3(1024), 0, stream, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_segments(const JobDev& J, hipStream_t stream) {
+  const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+  const int seg_blocks = (int)((segments + kSegPerBlock - 1) / kSegPerBlock);
+  hipLaunchKernelGGL(k_seg_partials, dim3(seg_blocks), dim3(kThreads), 0, stream, J);
+  hipLaunchKernelGGL(k_seg_top, dim3(1), dim3(kThreads), 0, stream, J, seg_blocks);
+  hipLaunchKernelGGL(k_seg_offsets, dim3(seg_blocks), dim3(kThreads), 0, stream, J);
   return hipGetLastError();
 }
 
@@ -1218,7 +1341,8 @@ hipError_t launch_writer_values(const JobDev& J, const WriterConsts& K, uint32_t
   const uint32_t startup_count = 6u * (uint32_t)J.width;
   hipLaunchKernelGGL(k_writer_startup_check, dim3(blocks_for(startup_count, kThreads)),
                      dim3(kThreads), 0, stream, J, K, startup_count, startup_final_state);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, J.writer_counts, 2);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kThreads), 0, stream, J.writer_counts,
+                     kCountSlots * kCountStride);
   if (J.diag_changed)
     hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(kThreads), 0, stream, J.diag_changed,
                        (size_t)J.height * J.width, (uint8_t)0);

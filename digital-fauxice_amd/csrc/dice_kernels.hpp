@@ -29,6 +29,7 @@ hipError_t launch_schedule_rows(const JobDev& J, const ScheduleDev& S,
                                 hipStream_t stream);
 bool tile_shape_supported(int ty, int tx);
 hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream);
+hipError_t launch_scan_segments(const JobDev& J, hipStream_t stream);
 hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream);
 hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream);
 hipError_t launch_writer_prep(const Site* sites, const uint8_t* attempted,

@@ -164,6 +164,7 @@ class DiceStageTimings(C.Structure):
         ("scan_kernel_ms", C.c_double),
         ("sites_kernel_ms", C.c_double),
         ("startup_kernels_ms", C.c_double),
+        ("segments_kernel_ms", C.c_double),
         ("writer_prep_kernel_ms", C.c_double),
         ("writer_values_kernel_ms", C.c_double),
     ]

@@ -201,6 +201,7 @@ typedef struct dice_stage_timings {
   double scan_kernel_ms;
   double sites_kernel_ms;
   double startup_kernels_ms;
+  double segments_kernel_ms;
   double writer_prep_kernel_ms;
   double writer_values_kernel_ms;
 } dice_stage_timings;
