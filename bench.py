@@ -1,20 +1,31 @@
 """Benchmark: bit-exact Mpix/s of the Digital ICE processing core on MI355X.
 
-Default workload (N=1): the plain single-frame HIP path on a synthetic
-5782x3946 RGBI uint16 frame (400 dust disks, 40 scratches <= 400 px).
-One step = one whole job with the main frame already resident in HBM:
-producer schedule, planes / decision / features / candidates / combiner,
-hidden startup, the serial conditional-dither writer (host), output patch,
-and the SHA-256 of the public RGB16 output (which needs the output bytes on
-the host).  Timed: exactly K steps after W warm-up steps, bracketed by
-device synchronization.
+Workload: synthetic 5782x3946 RGBI uint16 frames (400 dust disks, 40
+scratches <= 400 px; a different seed per frame), resident in HBM before
+the timed region.  One job covers:
+* the producer schedule;
+* planes / decision / features / candidates / combiner;
+* the hidden startup and the serial conditional-dither writer chain;
+* the output patch;
+* the SHA-256 of the public RGB16 output, which needs the output bytes on
+  the host.
 
-The JSON line also carries the HBM roofline of the streaming analysis
-kernel (14 algorithmic bytes per pixel: 8 B RGBI in + 6 B RGB out, timed
-with HIP events on the library's stream) and the CPU baseline: the
-compiled CPU oracle (the C restatement of the reference's CPU path) timed
-on this host's cores on the same frame, whose output also checks the GPU
-result bit for bit.
+Three legs:
+  plain    one job per call (dice_process_job), W warm-up + K timed jobs:
+           p50/p99 latency and Mpix/s of the plain single-frame path;
+  batched  the headline `value`: B frames per dice_process_batch call,
+           persistent launches over the batch, device phases replayed from
+           captured hipGraphs, tile shape autotuned on the batch:
+           W warm-up + K timed batches;
+  crops    16 crops of uneven size (sides 512-3000 px, aspect 0.5-2) as one
+           batch (same machinery): Mpix/s and batch latency.
+
+Every timed job's output SHA-256 and counters are compared with the compiled
+CPU oracle (the C restatement of the reference CPU path) run on the same
+frames.  That oracle run is also the CPU baseline, timed on this host's
+cores.  The roofline is that of the streaming analysis kernel k_scan:
+14 algorithmic bytes per pixel (8 B RGBI in + 6 B RGB out) over its
+HIP-event time on the library's stream.
 """
 
 from __future__ import annotations
@@ -41,15 +52,21 @@ BYTES_PER_PIXEL = 14  # 8 B RGBI uint16 in + 6 B RGB uint16 out
 def parse_args() -> argparse.Namespace:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--steps", type=int, default=6, help="timed batches (and plain jobs)")
     parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
+    parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
-    parser.add_argument("--cpu-seconds", type=float, default=15.0,
+    parser.add_argument("--cpu-seconds", type=float, default=20.0,
                         help="approximate CPU time budget of the baseline sample")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                         help="only run the plain copy kernel over BYTES (counter calibration)")
     return parser.parse_args()
+
+
+def log(message: str) -> None:
+    print(f"bench: {message}", file=sys.stderr, flush=True)
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -65,77 +82,158 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
+class Frames:
+    """Synthetic frames, their prepasses and resident device buffers."""
+
+    def __init__(self, specs) -> None:
+        from dice_amd import synth
+        from dice_amd.hip_backend.device import DeviceArray
+
+        self.specs = specs
+        self.mains = [synth.frame(s) for s in specs]
+        self.prepasses = [np.ascontiguousarray(synth.prepass_for(m)) for m in self.mains]
+        self.d_main = [DeviceArray.from_host(m) for m in self.mains]
+        self.d_out = [DeviceArray(s.height * s.width * 6) for s in specs]
+
+    @property
+    def mpix(self) -> list[float]:
+        return [s.height * s.width / 1e6 for s in self.specs]
+
+
+def fingerprint(res) -> tuple:
+    return (res.output_sha256.decode(), res.attempted_pixels, res.written_pixels,
+            res.public_rng_advances, res.final_rng_state, res.changed_pixels)
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(jobs shard across GPUs without collectives; see DESIGN.md)", file=sys.stderr)
+              "(jobs shard across GPUs by job, without collectives; see DESIGN.md)",
+              file=sys.stderr)
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
 
     from dice_amd import _abi, synth
-    from dice_amd._abi import DiceReplayResult, DiceStageTimings, ptr
+    from dice_amd._abi import DiceBatchReport, DiceJob, DiceReplayResult, DiceStageTimings, ptr
     from dice_amd.hip_backend import engine
+    from dice_amd.hip_backend.device import synchronize
     from dice_amd.hip_backend.process import stable_profile_params
     from dice_amd.profile import DEFAULT_PROFILE
-
-    import torch
-
-    spec = synth.config2_full_frame()
-    main_px = synth.frame(spec)
-    prepass = np.ascontiguousarray(synth.prepass_for(main_px))
-    height, width = spec.height, spec.width
-    mpix = height * width / 1e6
-    torch.cuda.set_device(0)
-    d_main = torch.from_numpy(main_px.view(np.int16)).to("cuda")
-    d_out = torch.empty((height, width, 3), dtype=torch.int16, device="cuda")
-    torch.cuda.synchronize()
 
     lib = engine.library()
     ctx = engine.context()
     params = stable_profile_params()
     prepass_params = DEFAULT_PROFILE.prepass_parameters()
-    flags = _abi.DICE_MAIN_ON_DEVICE | _abi.DICE_OUT_ON_DEVICE
+    device_flags = _abi.DICE_MAIN_ON_DEVICE | _abi.DICE_OUT_ON_DEVICE
 
-    def step(timings: DiceStageTimings) -> DiceReplayResult:
+    t0 = time.perf_counter()
+    full = Frames([synth.config2_full_frame(seed=2 + i) for i in range(args.batch)])
+    log(f"{args.batch} full frames generated and resident in {time.perf_counter() - t0:.1f} s")
+    frame_mpix = full.mpix[0]
+    height, width = full.specs[0].height, full.specs[0].width
+
+    # ---- plain leg: one job per call on frame 0
+    def plain_step(timings: DiceStageTimings) -> DiceReplayResult:
         res = DiceReplayResult()
-        code = lib.dice_process_job(ctx, C.c_void_p(d_main.data_ptr()), height, width,
-                                    ptr(prepass, C.c_uint16), prepass.shape[0],
-                                    prepass.shape[1], C.byref(params),
-                                    C.byref(prepass_params), flags,
-                                    C.c_void_p(d_out.data_ptr()), C.byref(res), None, None,
-                                    C.byref(timings))
+        code = lib.dice_process_job(ctx, C.c_void_p(full.d_main[0].address), height, width,
+                                    ptr(full.prepasses[0], C.c_uint16),
+                                    full.prepasses[0].shape[0], full.prepasses[0].shape[1],
+                                    C.byref(params), C.byref(prepass_params), device_flags,
+                                    C.c_void_p(full.d_out[0].address), C.byref(res), None,
+                                    None, C.byref(timings))
         engine.raise_for(code)
         return res
 
-    first = None
-    for i in range(args.warmup):
-        res = step(DiceStageTimings())
-        first = first or res
-        print(f"bench: warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-
-    torch.cuda.synchronize()
-    latencies, stage_rows = [], []
+    for _ in range(args.warmup):
+        plain_step(DiceStageTimings())
+    synchronize()
+    plain_lat, stage_rows, plain_results = [], [], []
     start = time.perf_counter()
     for _ in range(args.steps):
-        t0 = time.perf_counter()
         timings = DiceStageTimings()
-        res = step(timings)
-        latencies.append(time.perf_counter() - t0)
+        a = time.perf_counter()
+        plain_results.append(fingerprint(plain_step(timings)))
+        plain_lat.append((time.perf_counter() - a) * 1e3)
         stage_rows.append(timings.as_dict())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - start
-    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-
-    stage_ms = {key: statistics.mean(row[key] for row in stage_rows) * (1.0 if key.endswith("_ms") else 1e3)
-                for key in stage_rows[0]}
+    synchronize()
+    plain_elapsed = time.perf_counter() - start
+    log(f"plain: {args.steps} jobs in {plain_elapsed:.3f} s")
+    stage_ms = {k: statistics.mean(r[k] for r in stage_rows) * (1.0 if k.endswith("_ms") else 1e3)
+                for k in stage_rows[0]}
     scan_s = stage_ms["scan_kernel_ms"] / 1e3
     achieved = BYTES_PER_PIXEL * height * width / scan_s / 1e9 if scan_s > 0 else 0.0
-    latencies_ms = sorted(v * 1e3 for v in latencies)
-    p50 = float(np.percentile(latencies_ms, 50))
-    p99 = float(np.percentile(latencies_ms, 99))
 
+    # ---- batched leg: B full frames per call
+    def batch_jobs(frames: Frames):
+        array = (DiceJob * len(frames.specs))()
+        results = [DiceReplayResult() for _ in frames.specs]
+        for i, spec in enumerate(frames.specs):
+            e = array[i]
+            e.main_rgbi = frames.d_main[i].address
+            e.height, e.width = spec.height, spec.width
+            e.prepass_rgbi = frames.prepasses[i].ctypes.data
+            e.prepass_height, e.prepass_width = frames.prepasses[i].shape[:2]
+            e.params = C.pointer(params)
+            e.prepass_params = C.pointer(prepass_params)
+            e.flags = device_flags
+            e.out_rgb16 = frames.d_out[i].address
+            e.result = C.pointer(results[i])
+        return array, results
+
+    def run_batch(frames: Frames, flags: int) -> tuple[list, DiceBatchReport]:
+        array, results = batch_jobs(frames)
+        report = DiceBatchReport()
+        code = lib.dice_process_batch(ctx, array, len(frames.specs), flags, C.byref(report))
+        engine.raise_for(code)
+        return [fingerprint(r) for r in results], report
+
+    batch_flags = _abi.DICE_BATCH_GRAPH | _abi.DICE_BATCH_AUTOTUNE
+    for _ in range(max(args.warmup, 2)):  # the second run captures the graphs
+        run_batch(full, batch_flags)
+    synchronize()
+    batch_lat, batch_results = [], []
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        prints, report = run_batch(full, batch_flags)
+        batch_lat.append((time.perf_counter() - a) * 1e3)
+        batch_results.append(prints)
+    synchronize()
+    batch_elapsed = time.perf_counter() - start
+    log(f"batched: {args.steps} batches of {args.batch} in {batch_elapsed:.3f} s "
+        f"(tile {report.tile_rows}x{report.tile_cols}, graph {report.graph_replayed})")
+    batch_mpix = args.steps * sum(full.mpix)
+    value = batch_mpix / batch_elapsed
+
+    # ---- crops leg: 16 uneven crops as one batch
+    crops = None
+    if not args.no_crops:
+        crop_frames = Frames(synth.config3_crops())
+        for _ in range(2):
+            run_batch(crop_frames, batch_flags)
+        synchronize()
+        lat = []
+        crop_prints = None
+        start = time.perf_counter()
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            crop_prints, crop_report = run_batch(crop_frames, batch_flags)
+            lat.append((time.perf_counter() - a) * 1e3)
+        crop_elapsed = time.perf_counter() - start
+        crops = {
+            "jobs": len(crop_frames.specs),
+            "mpix_per_batch": round(sum(crop_frames.mpix), 3),
+            "value": round(args.steps * sum(crop_frames.mpix) / crop_elapsed, 2),
+            "unit": "Mpix/s",
+            "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "tile": [crop_report.tile_rows, crop_report.tile_cols],
+            "graph_replayed": bool(crop_report.graph_replayed),
+        }
+        log(f"crops: {crops}")
+
+    # ---- CPU baseline (the oracle), which also checks every output
     cpu_baseline = None
     bit_exact = None
     if not args.no_cpu_baseline:
@@ -143,64 +241,73 @@ def main() -> int:
         import oracle_lib
 
         threads = min(os.cpu_count() or 1, 16)
-        runs = []
-        oracle = None
+        expected, runs = [], []
         budget_start = time.perf_counter()
-        while True:
-            t0 = time.perf_counter()
-            oracle = oracle_lib.process_job(main_px, prepass, params, prepass_params,
-                                            threads=threads)
-            runs.append(time.perf_counter() - t0)
-            print(f"bench: cpu baseline run {len(runs)} {runs[-1]:.2f} s", file=sys.stderr,
-                  flush=True)
-            if len(runs) >= 4 or time.perf_counter() - budget_start > args.cpu_seconds:
+        for i in range(args.batch):
+            a = time.perf_counter()
+            cpu = oracle_lib.process_job(full.mains[i], full.prepasses[i], params, prepass_params,
+                                         threads=threads)
+            runs.append(time.perf_counter() - a)
+            expected.append((cpu.output_sha256, cpu.attempted_pixels, cpu.written_pixels,
+                             cpu.public_rng_advances, cpu.final_rng_state, cpu.changed_pixels))
+            log(f"cpu oracle frame {i}: {runs[-1]:.2f} s")
+            if i >= 1 and time.perf_counter() - budget_start > args.cpu_seconds:
                 break
-        timed_runs = runs[1:] if len(runs) > 1 else runs
-        median = statistics.median(timed_runs)
-        bit_exact = (
-            res.output_sha256.decode() == oracle.output_sha256
-            and res.attempted_pixels == oracle.attempted_pixels
-            and res.written_pixels == oracle.written_pixels
-            and res.public_rng_advances == oracle.public_rng_advances
-            and res.final_rng_state == oracle.final_rng_state
-            and res.changed_pixels == oracle.changed_pixels
-        )
+        checked = len(expected)
+        bit_exact = (all(p == expected[0] for p in plain_results)
+                     and all(prints[:checked] == expected for prints in batch_results))
+        timed = runs[1:] if len(runs) > 1 else runs
+        median = statistics.median(timed)
         cpu_baseline = {
-            "value": round(mpix / median, 4),
+            "value": round(frame_mpix / median, 4),
             "unit": "Mpix/s",
             "cores": threads,
             "kind": "port",
-            "sample": (f"{len(timed_runs)} timed full-frame jobs (5782x3946, after "
-                       f"{len(runs) - len(timed_runs)} warm-up) of the compiled C oracle, "
-                       f"OpenMP row-parallel analysis + serial writer; median {median:.3f} s"),
+            "sample": (f"{len(timed)} full 5782x3946 frames (after 1 warm-up) of the compiled C "
+                       f"oracle, OpenMP row-parallel analysis + serial writer; median "
+                       f"{median:.3f} s; it also checked {checked} frames of every timed batch "
+                       f"and every plain job bit for bit"),
         }
 
+    plain_value = args.steps * frame_mpix / plain_elapsed
     line = {
         "metric": METRIC,
-        "value": round(args.steps * mpix / elapsed, 4),
+        "value": round(value, 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(batch_elapsed / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "plain single-frame HIP path, 5782x3946 RGBI16, 400 disks + 40 scratches",
-            "global_batch": 1,
+            "workload": (f"batches of {args.batch} full 5782x3946 RGBI16 frames (400 disks + 40 "
+                         "scratches each), resident in HBM; persistent launches over the batch, "
+                         "graph replay, autotuned tiles; SHA-256 of every output"),
+            "global_batch": args.batch,
             "height": height,
             "width": width,
-            "attempted_sites": int(res.attempted_pixels),
+            "attempted_sites_frame0": int(plain_results[0][1]),
             "parallelism": "single GPU",
         },
-        "p50_ms": round(p50, 3),
-        "p99_ms": round(p99, 3),
+        "batch_p50_ms": round(float(np.percentile(batch_lat, 50)), 3),
+        "batch_p99_ms": round(float(np.percentile(batch_lat, 99)), 3),
+        "tile": [report.tile_rows, report.tile_cols],
+        "graph_replayed": bool(report.graph_replayed),
+        "plain": {
+            "value": round(plain_value, 2),
+            "unit": "Mpix/s",
+            "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
+        },
+        "batched_vs_plain": round(value / plain_value, 3),
+        "crops": crops,
         "bit_exact": bit_exact,
-        "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
-        "writer_share": round(stage_ms["writer_host"] / stage_ms["total"], 4)
+        "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
+        "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)
         if stage_ms["total"] else None,
         "roofline": {
             "bound": "hbm",

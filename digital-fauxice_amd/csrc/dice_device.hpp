@@ -175,12 +175,18 @@ struct StageDev {
 };
 
 struct JobDev {
+  DevParams P;                  // stable replay parameters of this job
+  WriterConsts K;               // dither bounds and scales of this job
+  float factor_b;               // auxiliary factor of the setter (stage_parameters.py)
+  int32_t run_producer;         // 1: producer + setter on device; 0: stage table given
+  int32_t skip;                 // 1: the job already failed; later phases leave it
   int32_t height, width;
   int32_t tiles_x, tiles_y;     // analysis tiles (TY x TX)
   int32_t tile_rows, tile_cols; // TY, TX of the launch
   const uint16_t* rgbi;         // H*W*4
   uint16_t* out;                // H*W*3 (noop emit; patched after the writer)
   const float* lut;             // 65536
+  const uint16_t* noop_emit;    // 65536: emit of LUT[i], the output of an unchanged value
   const uint32_t* factor_high;  // 256
   const uint32_t* factor_low;   // 256
   StageDev stages;
@@ -189,6 +195,7 @@ struct JobDev {
   uint32_t* seg_count;          // (tiles_y*TY) * tiles_x
   uint32_t* seg_offset;         // (tiles_y*TY) * tiles_x
   uint32_t* seg_block_sums;     // one per 1024 segments
+  uint32_t* row_start;          // H+1: first site of each row; [H] = total
   Site* sites;                  // compacted, row-major
   WriterRecord* writer_records; // one per site, row-major
   WriterRecord* su_writer_records; // 6*W
@@ -214,6 +221,17 @@ struct JobDev {
   double* cumulative;           // H * 3
   double* epoch_wide;           // epochs * 2 (den_w, num_w)
   float* epoch_store;           // epochs * 2 (den32, num32)
+};
+
+// A batch of jobs for one launch: work unit u of the launch belongs to the
+// job j with base[j] <= u < base[j+1] (empty ranges allowed), and is unit
+// u - base[j] of that job.
+constexpr int kMaxBatch = 64;
+
+struct JobSet {
+  const JobDev* jobs;  // device array of `count` descriptors
+  int32_t count;
+  uint32_t base[kMaxBatch + 1];
 };
 
 }  // namespace dice

@@ -1,101 +1,58 @@
-// dice_host.cpp -- host runtime and C ABI of libdice_hip.so.
+// dice_host.cpp -- C ABI of libdice_hip.so.
 //
-// Orchestrates one job the way process_cuda / run_streaming_replay_cuda do
+// Orchestrates jobs the way process_cuda / run_streaming_replay_cuda do
 // (rohanpandula/digital-fauxice, src/portable_digital_ice/cuda_backend/
 // process.py:25-159, engine.py:183-590), MI355X-first:
-//   host    prepass (0x40 record, tiny), LUT/factor tables (hash-pinned)
+//   host    job validation, prepass 0x40 record (tiny), hash-pinned tables
 //   device  producer schedule incl. the sequential chains and the setter
-//   device  fused analysis tiles + hidden-startup sites + row-major compaction
-//   device  integer draw schedule of every site (exact LCG-state thresholds)
-//   host    the strictly serial LCG state chain over those schedules
-//   device  every site's writer step from its recorded start state, the
-//           output patch, the counts and a check of the chain
-//   host    SHA-256 of the public output
+//   device  analysis tiles, hidden-startup sites, row-major site records,
+//           integer draw schedules of the writer
+//   host    the strictly serial LCG state chain, in row bands
+//   device  every site's writer step from its recorded start state, output
+//           patch, counts, and a check of the chain
+//   host    SHA-256 of the public output, overlapped with the chain
+// A batch (dice_process_batch) runs many jobs through the same phases with
+// one persistent launch per kernel; a single job is a batch of one.
 #include <hip/hip_runtime.h>
-#include <openssl/evp.h>
 
-#include <chrono>
 #include <cmath>
-#include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/dice_hip.h"
+#include "dice_common.hpp"
 #include "dice_device.hpp"
 #include "dice_kernels.hpp"
+#include "dice_runtime.hpp"
 
 #pragma clang fp contract(off)
 
 using namespace dice;
 
+namespace dice {
+
+char* error_buffer() {
+  thread_local char buffer[512];
+  return buffer;
+}
+
 namespace {
-
-thread_local char g_error[512];
-
-int32_t set_error(int32_t code, const char* fmt, ...) {
-  va_list args;
-  va_start(args, fmt);
-  vsnprintf(g_error, sizeof g_error, fmt, args);
-  va_end(args);
-  return code;
-}
-
-#define HIP_TRY(expr)                                                         \
-  do {                                                                        \
-    hipError_t _e = (expr);                                                   \
-    if (_e != hipSuccess)                                                     \
-      return set_error(DICE_INTERNAL, "%s failed: %s", #expr,                  \
-                       hipGetErrorString(_e));                                \
-  } while (0)
-
-double now_seconds() {
-  return std::chrono::duration<double>(
-             std::chrono::steady_clock::now().time_since_epoch())
-      .count();
-}
-
-// --------------------------------------------------------------------------
-// SHA-256 (OpenSSL EVP) and the two hash-pinned tables
-// --------------------------------------------------------------------------
-
-bool sha256_hex(const void* data, size_t bytes, char out[72]) {
-  unsigned char digest[32];
-  unsigned int length = 0;
-  EVP_MD_CTX* md = EVP_MD_CTX_new();
-  if (!md) return false;
-  bool ok = EVP_DigestInit_ex(md, EVP_sha256(), nullptr) == 1 &&
-            EVP_DigestUpdate(md, data, bytes) == 1 &&
-            EVP_DigestFinal_ex(md, digest, &length) == 1;
-  EVP_MD_CTX_free(md);
-  if (!ok) return false;
-  static const char* hex = "0123456789abcdef";
-  for (int i = 0; i < 32; ++i) {
-    out[2 * i] = hex[digest[i] >> 4];
-    out[2 * i + 1] = hex[digest[i] & 15];
-  }
-  out[64] = 0;
-  return true;
-}
-
 constexpr const char* kLutSha256 =
     "5fd225e25719a544df9475315e3017f9b8027d593ac43e392245de59f49f3fcb";  // x3a.py:26-28
 constexpr const char* kHighSha256 =
     "fdc378493f13916e36d3dbd9b69b714e888d7993680509df4550486db6fcf328";  // output.py:19
 constexpr const char* kLowSha256 =
     "b7f3b1b5c0ea816ddb20196e1cf7614d364ea4e164295e0dcb78119c88ef404d";  // output.py:20
-
-struct Tables {
-  std::vector<float> lut;
-  std::vector<uint32_t> high, low;
-  int32_t status = -1;
-};
+}  // namespace
 
 // x3a.py:111-132 and output.py:39-58: the binary64 closed forms, narrowed
 // once, then pinned by SHA-256.  A platform libm that differs in any last
-// bit fails closed.
+// bit fails closed.  The no-op emit table is emit(LUT[i]) (output.py:61-94),
+// the output of every unchanged sample.
 const Tables& tables() {
   static Tables t;
   static std::once_flag once;
@@ -112,16 +69,20 @@ const Tables& tables() {
       t.low[d] = (uint32_t)std::floor(65536.0 * std::pow(10.0, dd * scale) + 0.5);
     }
     char h1[72], h2[72], h3[72];
-    if (!sha256_hex(t.lut.data(), 65536 * 4, h1) ||
-        !sha256_hex(t.high.data(), 256 * 4, h2) ||
+    if (!sha256_hex(t.lut.data(), 65536 * 4, h1) || !sha256_hex(t.high.data(), 256 * 4, h2) ||
         !sha256_hex(t.low.data(), 256 * 4, h3)) {
       t.status = DICE_INTERNAL;
-    } else if (strcmp(h1, kLutSha256) != 0 || strcmp(h2, kHighSha256) != 0 ||
-               strcmp(h3, kLowSha256) != 0) {
-      t.status = DICE_UNAVAILABLE;
-    } else {
-      t.status = DICE_OK;
+      return;
     }
+    if (strcmp(h1, kLutSha256) != 0 || strcmp(h2, kHighSha256) != 0 ||
+        strcmp(h3, kLowSha256) != 0) {
+      t.status = DICE_UNAVAILABLE;
+      return;
+    }
+    t.noop_emit.resize(65536);
+    for (int i = 0; i < 65536; ++i)
+      t.noop_emit[i] = emit_value(t.lut[i], t.high.data(), t.low.data());
+    t.status = DICE_OK;
   });
   return t;
 }
@@ -136,146 +97,28 @@ int32_t require_tables() {
   return DICE_OK;
 }
 
-// --------------------------------------------------------------------------
-// Device buffers
-// --------------------------------------------------------------------------
-
-struct DeviceBuffer {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  ~DeviceBuffer() {
-    if (ptr) (void)hipFree(ptr);
-  }
-  hipError_t ensure(size_t need) {
-    if (need <= bytes) return hipSuccess;
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-    size_t rounded = (need + 0xFFFFF) & ~(size_t)0xFFFFF;
-    hipError_t e = hipMalloc(&ptr, rounded);
-    if (e == hipSuccess) bytes = rounded;
-    return e;
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(ptr);
-  }
-};
-
-struct PinnedBuffer {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  ~PinnedBuffer() {
-    if (ptr) (void)hipHostFree(ptr);
-  }
-  hipError_t ensure(size_t need) {
-    if (need <= bytes) return hipSuccess;
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-    size_t rounded = (need + 0xFFFF) & ~(size_t)0xFFFF;
-    hipError_t e = hipHostMalloc(&ptr, rounded, hipHostMallocDefault);
-    if (e == hipSuccess) bytes = rounded;
-    return e;
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(ptr);
-  }
-};
-
-}  // namespace
+}  // namespace dice
 
 // --------------------------------------------------------------------------
-// Context
+// Context: one device, its resident tables and one batch runner.
 // --------------------------------------------------------------------------
 
 struct dice_ctx {
   int device = 0;
-  hipEvent_t events[12] = {};
   uint64_t hbm_budget = 0;
-  hipStream_t stream = nullptr;
   std::mutex mutex;
-  int tile_rows = 16, tile_cols = 64;
-  // tables on the device
-  DeviceBuffer lut, high, low;
-  // job workspace
-  DeviceBuffer rgbi, out, site_lists, tile_count, seg_count, seg_offset, sites, counters,
-      flags;
-  DeviceBuffer diag_score, diag_floor;
-  DeviceBuffer su_aux, su_score, su_work, su_sites, su_attempted;
-  DeviceBuffer failpos, row_sums, accepted, epoch_adds, cumulative, seg_block_sums,
-      epoch_wide, epoch_store;
-  DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse,
-      st_gate, st_row_gate;
-  DeviceBuffer writer_records, su_writer_records, site_state, writer_counts, diag_changed;
-  DeviceBuffer sched[12];
-  // pinned host staging
-  PinnedBuffer h_counters, h_counts, h_records, h_site_state, h_out;
+  DeviceTables tables;
+  std::unique_ptr<Runner> runner;
+  hipStream_t stream = nullptr;  // the runner's stream (level-1 entry points)
+  hipEvent_t events[2] = {};
 };
 
 namespace {
-
-// Kernel-group timing with HIP events on the context stream (only when the
-// caller asks for stage timings).
-struct EventTimer {
-  dice_ctx* ctx;
-  bool on;
-  int32_t record(int slot) {
-    if (on) HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));
-    return DICE_OK;
-  }
-  double elapsed_ms(int a, int b) {
-    if (!on) return 0.0;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->events[a], ctx->events[b]) != hipSuccess) return 0.0;
-    return (double)ms;
-  }
-};
 
 int32_t activate(dice_ctx* ctx) {
   if (!ctx) return set_error(DICE_INVALID, "null context");
   HIP_TRY(hipSetDevice(ctx->device));
   return DICE_OK;
-}
-
-int32_t upload_tables(dice_ctx* ctx) {
-  const Tables& t = tables();
-  if (ctx->lut.bytes >= 65536 * 4) return DICE_OK;
-  HIP_TRY(ctx->lut.ensure(65536 * 4));
-  HIP_TRY(ctx->high.ensure(256 * 4));
-  HIP_TRY(ctx->low.ensure(256 * 4));
-  HIP_TRY(hipMemcpy(ctx->lut.ptr, t.lut.data(), 65536 * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->high.ptr, t.high.data(), 256 * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->low.ptr, t.low.data(), 256 * 4, hipMemcpyHostToDevice));
-  return DICE_OK;
-}
-
-DevParams device_params(const dice_params& p) {
-  DevParams P;
-  P.visible_channel = p.visible_channel;
-  P.score_addend = p.score_addend;
-  P.score_scale = p.score_scale;
-  P.score_offset = p.score_offset;
-  P.score_floor = p.score_floor;
-  P.horizontal_minimum = p.horizontal_minimum;
-  P.decision_threshold = p.decision_threshold;
-  P.count_limit = p.decision_count_limit;
-  P.radius = p.decision_radius;
-  P.cross_neighbor = p.cross_neighbor;
-  P.gate_primary = p.gate_primary;
-  P.combine.coarse_enabled = p.coarse_enabled;
-  for (int i = 0; i < 3; ++i) {
-    P.combine.coarse_slopes[i] = p.coarse_slopes[i];
-    P.combine.band_enabled[i] = p.band_enabled[i];
-    P.combine.band_scales[i] = p.band_scales[i];
-    P.combine.configured_strengths[i] = p.configured_strengths[i];
-  }
-  for (int i = 0; i < 9; ++i) {
-    P.combine.factors_a[i] = p.factors_a[i];
-    P.combine.factors_b[i] = p.factors_b[i];
-  }
-  return P;
 }
 
 bool all_finite(const float* v, int n) {
@@ -310,327 +153,6 @@ int32_t validate_replay(int64_t H, int64_t W, const dice_params& p) {
     return set_error(DICE_INVALID, "reconstruction parameters must be finite");
   return DICE_OK;
 }
-
-struct Callbacks {
-  const dice_callbacks* cb;
-  void progress(int32_t phase, int64_t completed, int64_t total, int64_t attempted,
-                int64_t written) const {
-    if (cb && cb->progress) cb->progress(cb->user, phase, completed, total, attempted, written);
-  }
-  bool cancelled() const { return cb && cb->cancelled && cb->cancelled(cb->user) != 0; }
-};
-
-int32_t cancelled_error() {
-  return set_error(DICE_CANCELLED, "portable correction was cancelled");
-}
-
-// Sizes the workspace for an H x W job and fills the JobDev view.
-int32_t prepare_job(dice_ctx* ctx, int64_t H, int64_t W, bool want_diag, JobDev& J) {
-  const int TY = ctx->tile_rows, TX = ctx->tile_cols;
-  memset(&J, 0, sizeof J);
-  J.height = (int32_t)H;
-  J.width = (int32_t)W;
-  J.tile_rows = TY;
-  J.tile_cols = TX;
-  J.tiles_x = (int32_t)((W + TX - 1) / TX);
-  J.tiles_y = (int32_t)((H + TY - 1) / TY);
-  const size_t pixels = (size_t)H * W;
-  const size_t tiles = (size_t)J.tiles_x * J.tiles_y;
-  const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
-  const size_t stages = (size_t)H + 6;
-  const size_t cells = (size_t)W / 8;
-  const size_t fail_rows = (size_t)(H + 7) / 8;
-  const size_t epochs = (size_t)H / 8;
-  HIP_TRY(ctx->site_lists.ensure(tiles * TY * TX * sizeof(uint16_t)));
-  HIP_TRY(ctx->tile_count.ensure(tiles * 4));
-  HIP_TRY(ctx->seg_count.ensure(segments * 4));
-  HIP_TRY(ctx->seg_offset.ensure(segments * 4));
-  HIP_TRY(ctx->counters.ensure(64));
-  HIP_TRY(ctx->flags.ensure(64));
-  HIP_TRY(ctx->su_aux.ensure(5 * W * 4));
-  HIP_TRY(ctx->su_score.ensure(5 * W * 4));
-  HIP_TRY(ctx->su_work.ensure(5 * W * 12));
-  HIP_TRY(ctx->su_sites.ensure(6 * W * sizeof(Site)));
-  HIP_TRY(ctx->su_attempted.ensure(6 * W));
-  HIP_TRY(ctx->failpos.ensure(fail_rows * cells + 8));
-  HIP_TRY(ctx->row_sums.ensure(H * 3 * 8));
-  HIP_TRY(ctx->accepted.ensure(H * 4));
-  HIP_TRY(ctx->seg_block_sums.ensure((segments / 1024 + 2) * 4));
-  HIP_TRY(ctx->epoch_adds.ensure(epochs * 16 + 16));
-  HIP_TRY(ctx->cumulative.ensure(H * 24));
-  HIP_TRY(ctx->epoch_wide.ensure(epochs * 16 + 16));
-  HIP_TRY(ctx->epoch_store.ensure(epochs * 8 + 8));
-  HIP_TRY(ctx->st_alpha.ensure(stages * 4));
-  HIP_TRY(ctx->st_offset.ensure(stages * 4));
-  HIP_TRY(ctx->st_replacement.ensure(stages * 4));
-  HIP_TRY(ctx->st_has.ensure(stages));
-  HIP_TRY(ctx->st_score_base.ensure(stages * 4));
-  HIP_TRY(ctx->st_coarse.ensure(stages * 4));
-  HIP_TRY(ctx->st_gate.ensure(stages));
-  HIP_TRY(ctx->st_row_gate.ensure(stages * 4));
-  if (want_diag) {
-    HIP_TRY(ctx->diag_score.ensure(pixels * 4));
-    HIP_TRY(ctx->diag_floor.ensure(pixels));
-  }
-  HIP_TRY(ctx->h_counters.ensure(64));
-  J.lut = ctx->lut.as<float>();
-  J.factor_high = ctx->high.as<uint32_t>();
-  J.factor_low = ctx->low.as<uint32_t>();
-  J.stages.alpha = ctx->st_alpha.as<float>();
-  J.stages.offset = ctx->st_offset.as<float>();
-  J.stages.replacement = ctx->st_replacement.as<float>();
-  J.stages.has_replacement = ctx->st_has.as<uint8_t>();
-  J.stages.score_base = ctx->st_score_base.as<float>();
-  J.stages.coarse = ctx->st_coarse.as<float>();
-  J.stages.gate_secondary = ctx->st_gate.as<uint8_t>();
-  J.stages.row_gate = ctx->st_row_gate.as<int32_t>();
-  J.site_lists = ctx->site_lists.as<uint16_t>();
-  J.tile_count = ctx->tile_count.as<uint32_t>();
-  J.seg_count = ctx->seg_count.as<uint32_t>();
-  J.seg_offset = ctx->seg_offset.as<uint32_t>();
-  J.counters = ctx->counters.as<uint32_t>();
-  J.error_flags = ctx->flags.as<uint32_t>();
-  J.diag_score = want_diag ? ctx->diag_score.as<float>() : nullptr;
-  J.diag_floor = want_diag ? ctx->diag_floor.as<uint8_t>() : nullptr;
-  J.su_aux = ctx->su_aux.as<float>();
-  J.su_score = ctx->su_score.as<float>();
-  J.su_work = ctx->su_work.as<float>();
-  J.su_sites = ctx->su_sites.as<Site>();
-  J.su_attempted = ctx->su_attempted.as<uint8_t>();
-  J.failpos = ctx->failpos.as<uint8_t>();
-  J.row_sums = ctx->row_sums.as<double>();
-  J.accepted = ctx->accepted.as<uint32_t>();
-  J.seg_block_sums = ctx->seg_block_sums.as<uint32_t>();
-  J.epoch_adds = ctx->epoch_adds.as<double>();
-  J.cumulative = ctx->cumulative.as<double>();
-  J.epoch_wide = ctx->epoch_wide.as<double>();
-  J.epoch_store = ctx->epoch_store.as<float>();
-  return DICE_OK;
-}
-
-int32_t flag_error(uint32_t flags) {
-  if (flags & kErrNonfiniteAux)
-    return set_error(DICE_NONFINITE_AUX, "auxiliary score input must be a finite HxW plane");
-  if (flags & kErrSetterScaleOne)
-    return set_error(DICE_INVALID, "X3A setter scale cannot equal one");
-  if (flags & kErrStageNonfinite)
-    return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
-  return DICE_OK;
-}
-
-// The strictly serial part of the writer (streaming.py:476-531,
-// startup.py:231-341, fast_cpu/kernels.py:1139-1204) is the LCG state
-// chain over the WriterRecord schedules (chain_site); the values follow
-// in parallel on the device from the recorded start states.
-WriterConsts writer_consts(const dice_params& p) {
-  WriterConsts K;
-  K.low = (double)p.dither_low;
-  K.high = (double)p.dither_high;
-  K.low_lt_high = K.low < K.high;
-  K.scales[0] = p.dither_scales[0];
-  K.scales[1] = p.dither_scales[1];
-  K.scales[2] = p.dither_scales[2];
-  return K;
-}
-
-// The analysis + writer + output phases of one job whose input is already
-// in `J.rgbi` and whose stage table is on the device.
-int32_t replay_core(dice_ctx* ctx, JobDev& J, const dice_params& p, uint32_t rng_state,
-                    uint32_t flags, uint16_t* out_host, dice_replay_result* result,
-                    dice_diagnostics* diag, const Callbacks& cb,
-                    dice_stage_timings* timings) {
-  hipStream_t s = ctx->stream;
-  const int64_t H = J.height, W = J.width;
-  const size_t pixels = (size_t)H * W;
-  DevParams P = device_params(p);
-  const WriterConsts K = writer_consts(p);
-  cb.progress(3, 0, H, 0, 0);
-  double t0 = now_seconds();
-  EventTimer ev{ctx, timings != nullptr};
-  if (int32_t e = ev.record(2)) return e;
-  HIP_TRY(launch_scan(J, P, s));
-  if (int32_t e = ev.record(3)) return e;
-  HIP_TRY(launch_scan_segments(J, s));
-  if (int32_t e = ev.record(10)) return e;
-  uint32_t* hc = ctx->h_counters.as<uint32_t>();
-  HIP_TRY(hipMemcpyAsync(hc, J.counters, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  const uint32_t count = hc[0];
-  if (int32_t e = flag_error(hc[1])) return e;
-  if (cb.cancelled()) return cancelled_error();
-
-  // Sites go straight to their row-major slot; size the arrays by the count.
-  const size_t startup_sites = 6 * (size_t)W;
-  HIP_TRY(ctx->sites.ensure(((size_t)count + 1) * sizeof(Site)));
-  HIP_TRY(ctx->writer_records.ensure(((size_t)count + 1) * sizeof(WriterRecord)));
-  HIP_TRY(ctx->su_writer_records.ensure(startup_sites * sizeof(WriterRecord)));
-  HIP_TRY(ctx->site_state.ensure(((size_t)count + startup_sites + 1) * 4));
-  HIP_TRY(ctx->writer_counts.ensure(kCountSlots * kCountStride * 8));
-  J.sites = ctx->sites.as<Site>();
-  J.writer_records = ctx->writer_records.as<WriterRecord>();
-  J.su_writer_records = ctx->su_writer_records.as<WriterRecord>();
-  J.su_site_state = ctx->site_state.as<uint32_t>();
-  J.site_state = J.su_site_state + startup_sites;
-  J.writer_counts = ctx->writer_counts.as<unsigned long long>();
-  J.diag_changed = nullptr;
-  if (diag && diag->changed_mask) {
-    HIP_TRY(ctx->diag_changed.ensure(pixels));
-    J.diag_changed = ctx->diag_changed.as<uint8_t>();
-  }
-  if (int32_t e = ev.record(4)) return e;
-  HIP_TRY(launch_sites(J, P, s));
-  if (int32_t e = ev.record(5)) return e;
-  HIP_TRY(launch_startup(J, P, s));
-  if (int32_t e = ev.record(6)) return e;
-  HIP_TRY(launch_writer_prep(J.sites, nullptr, J.writer_records, count, K, s));
-  HIP_TRY(launch_writer_prep(J.su_sites, J.su_attempted, J.su_writer_records,
-                             (uint32_t)startup_sites, K, s));
-  if (int32_t e = ev.record(7)) return e;
-  HIP_TRY(ctx->h_records.ensure(((size_t)count + startup_sites) * sizeof(WriterRecord)));
-  WriterRecord* h_su = ctx->h_records.as<WriterRecord>();
-  WriterRecord* h_main = h_su + startup_sites;
-  HIP_TRY(hipMemcpyAsync(h_su, J.su_writer_records, startup_sites * sizeof(WriterRecord),
-                         hipMemcpyDeviceToHost, s));
-  if (count)
-    HIP_TRY(hipMemcpyAsync(h_main, J.writer_records, (size_t)count * sizeof(WriterRecord),
-                           hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  double t1 = now_seconds();
-  if (timings) {
-    timings->analysis_device += t1 - t0;
-    timings->scan_kernel_ms += ev.elapsed_ms(2, 3);
-    timings->segments_kernel_ms += ev.elapsed_ms(3, 10);
-    timings->sites_kernel_ms += ev.elapsed_ms(4, 5);
-    timings->startup_kernels_ms += ev.elapsed_ms(5, 6);
-    timings->writer_prep_kernel_ms += ev.elapsed_ms(6, 7);
-  }
-  if (int32_t e = flag_error(hc[1])) return e;
-  if (cb.cancelled()) return cancelled_error();
-
-  // Serial state chain: the six hidden stages, then every site in order.
-  HIP_TRY(ctx->h_site_state.ensure(((size_t)count + startup_sites + 1) * 4));
-  uint32_t* h_su_state = ctx->h_site_state.as<uint32_t>();
-  uint32_t* h_state = h_su_state + startup_sites;
-  uint32_t state = rng_state;
-  for (int stage = 0; stage < 6; ++stage) {
-    uint64_t draws = 0;
-    int64_t attempted = 0;
-    for (int64_t x = 0; x < W; ++x) {
-      const size_t i = (size_t)stage * W + x;
-      const WriterRecord& r = h_su[i];
-      h_su_state[i] = state;
-      attempted += (r.flags & kRecordAttempted) ? 1 : 0;
-      state = chain_site(r, state, draws);
-    }
-    result->startup_attempted[stage] = attempted;
-    result->startup_advances[stage] = (int64_t)draws;
-  }
-  const uint32_t startup_final = state;
-  result->startup_final_rng_state = startup_final;
-  uint64_t advances = 0;
-  for (uint32_t i = 0; i < count; ++i) {
-    h_state[i] = state;
-    state = chain_site(h_main[i], state, advances);
-  }
-  const uint32_t final_state = state;
-  double t2 = now_seconds();
-  if (timings) timings->writer_host += t2 - t1;
-
-  // Parallel values, patch of the output, counts, and the check of both
-  // chains against the per-site writer.
-  HIP_TRY(hipMemcpyAsync(J.su_site_state, h_su_state, (startup_sites + count) * 4,
-                         hipMemcpyHostToDevice, s));
-  if (int32_t e = ev.record(8)) return e;
-  HIP_TRY(launch_writer_values(J, K, count, startup_final, final_state, s));
-  if (int32_t e = ev.record(9)) return e;
-  HIP_TRY(ctx->h_counts.ensure(kCountSlots * kCountStride * 8));
-  const unsigned long long* h_slots = ctx->h_counts.as<unsigned long long>();
-  HIP_TRY(hipMemcpyAsync(ctx->h_counts.ptr, J.writer_counts, kCountSlots * kCountStride * 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(hc + 1, J.error_flags, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  double t3 = now_seconds();
-  if (timings) {
-    timings->finish_device += t3 - t2;
-    timings->writer_values_kernel_ms += ev.elapsed_ms(8, 9);
-  }
-  if (hc[1] & kErrNonfiniteWork) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
-  if (hc[1] & kErrWriterChain)
-    return set_error(DICE_INTERNAL, "writer state chain disagrees with the per-site writer");
-  if (int32_t e = flag_error(hc[1])) return e;
-  if (cb.cancelled()) return cancelled_error();
-
-  const bool out_on_device = (flags & DICE_OUT_ON_DEVICE) != 0;
-  const bool want_sha = (flags & DICE_SKIP_SHA256) == 0;
-  result->output_sha256[0] = 0;
-  const void* hash_source = nullptr;
-  if (out_on_device) {
-    if (want_sha) {
-      HIP_TRY(ctx->h_out.ensure(pixels * 6));
-      HIP_TRY(hipMemcpyAsync(ctx->h_out.ptr, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      hash_source = ctx->h_out.ptr;
-    }
-  } else {
-    HIP_TRY(hipMemcpyAsync(out_host, J.out, pixels * 6, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    hash_source = out_host;
-  }
-  double t4 = now_seconds();
-  if (timings) timings->download_output += t4 - t3;
-  if (want_sha && !sha256_hex(hash_source, pixels * 6, result->output_sha256))
-    return set_error(DICE_INTERNAL, "SHA-256 failed");
-  if (timings) timings->sha256_host += now_seconds() - t4;
-  if (diag) {
-    if (diag->score_plane)
-      HIP_TRY(hipMemcpy(diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
-    if (diag->at_floor_mask)
-      HIP_TRY(hipMemcpy(diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
-    if (diag->changed_mask)
-      HIP_TRY(hipMemcpy(diag->changed_mask, J.diag_changed, pixels, hipMemcpyDeviceToHost));
-  }
-  uint64_t written = 0, changed = 0;
-  for (int slot = 0; slot < kCountSlots; ++slot) {
-    written += h_slots[slot * kCountStride + 0];
-    changed += h_slots[slot * kCountStride + 1];
-  }
-  result->attempted_pixels = count;
-  result->written_pixels = written;
-  result->public_rng_advances = advances;
-  result->changed_pixels = changed;
-  result->final_rng_state = final_state;
-  cb.progress(3, H, H, (int64_t)count, (int64_t)written);
-  return DICE_OK;
-}
-
-// Places the main RGBI input (and the output target) for one job.
-int32_t bind_io(dice_ctx* ctx, JobDev& J, const uint16_t* rgbi, uint16_t* out,
-                uint32_t flags, dice_stage_timings* timings) {
-  const size_t pixels = (size_t)J.height * J.width;
-  double t0 = now_seconds();
-  if (flags & DICE_MAIN_ON_DEVICE) {
-    J.rgbi = rgbi;
-  } else {
-    HIP_TRY(ctx->rgbi.ensure(pixels * 8));
-    HIP_TRY(hipMemcpyAsync(ctx->rgbi.ptr, rgbi, pixels * 8, hipMemcpyHostToDevice,
-                           ctx->stream));
-    J.rgbi = ctx->rgbi.as<uint16_t>();
-  }
-  if (flags & DICE_OUT_ON_DEVICE) {
-    J.out = out;
-  } else {
-    HIP_TRY(ctx->out.ensure(pixels * 6));
-    J.out = ctx->out.as<uint16_t>();
-  }
-  if (timings) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    timings->upload += now_seconds() - t0;
-  }
-  return DICE_OK;
-}
-
 
 // ---- prepass (prepass.py:67-236) --------------------------------------
 
@@ -740,14 +262,52 @@ int32_t host_prepass(const uint16_t* px, int64_t H, int64_t W, const float* lut,
   return DICE_OK;
 }
 
-}  // namespace
 
-// Scratch device buffers for the level-1 entry points.
+// profile.py:48-55 and the parameter dataclasses' finiteness checks on the
+// record-derived fields (replaced per row by the provider).
+int32_t check_prepass_record(const dice_prepass_result& prepass) {
+  const float alpha = get32f(prepass.payload, 0x04);
+  const float mean_r = get32f(prepass.payload, 0x28);
+  const float mean_ir = get32f(prepass.payload, 0x2C);
+  const double denominator = 1.0 - (double)alpha;
+  if (denominator == 0.0) return set_error(DICE_INVALID, "prepass alpha cannot equal one");
+  const float base = (float)(((double)mean_ir - (double)alpha * (double)mean_r) / denominator);
+  if (!std::isfinite(alpha)) return set_error(DICE_INVALID, "alpha must be finite");
+  if (!std::isfinite(base)) return set_error(DICE_INVALID, "alpha_one_replacement must be finite");
+  return DICE_OK;
+}
 
-namespace {
+// engine.py:131-145 / profile.py:75-106 (shape part of validate_job), the
+// prepass and its record checks: everything a job needs before the device.
+int32_t admit_job(const uint16_t* prepass_rgbi, int64_t prepass_height,
+                  int64_t prepass_width, int64_t height, int64_t width,
+                  const dice_params& params, const dice_prepass_params& prepass_params,
+                  double* prepass_seconds) {
+  if (height < 5 || width < 8)
+    return set_error(DICE_INVALID, "main RGBI frame must be at least 5 rows by 8 columns");
+  DICE_TRY(validate_replay(height, width, params));
+  const double t0 = now_seconds();
+  dice_prepass_result prepass;
+  DICE_TRY(host_prepass(prepass_rgbi, prepass_height, prepass_width, tables().lut.data(),
+                        prepass_params, &prepass));
+  DICE_TRY(check_prepass_record(prepass));
+  if (prepass_seconds) *prepass_seconds = now_seconds() - t0;
+  return DICE_OK;
+}
+
+// Runs one prepared request as a batch of one and reports its outcome on
+// the calling thread.
+int32_t run_single(dice_ctx* ctx, JobRequest& R, bool timings) {
+  RunOptions options;
+  options.fine_timing = timings;
+  DICE_TRY(ctx->runner->run(&R, 1, options, nullptr));
+  if (R.code != DICE_OK) return set_error(R.code, "%s", R.error.c_str());
+  return DICE_OK;
+}
+
 template <class F>
 int32_t with_device_buffers(dice_ctx* ctx, F&& body) {
-  if (int32_t e = activate(ctx)) return e;
+  DICE_TRY(activate(ctx));
   std::lock_guard<std::mutex> lock(ctx->mutex);
   return body();
 }
@@ -765,6 +325,7 @@ struct Scratch {
     return static_cast<T*>(p);
   }
 };
+
 }  // namespace
 
 // ==========================================================================
@@ -775,7 +336,7 @@ extern "C" {
 
 int32_t dice_abi_version(void) { return DICE_ABI_VERSION; }
 
-const char* dice_last_error(void) { return g_error; }
+const char* dice_last_error(void) { return error_buffer(); }
 
 int32_t dice_probe(dice_device_info* info) {
   memset(info, 0, sizeof *info);
@@ -813,51 +374,47 @@ int32_t dice_probe(dice_device_info* info) {
 
 int32_t dice_ctx_create(int32_t device, uint64_t hbm_budget, dice_ctx** out) {
   *out = nullptr;
-  if (int32_t e = require_tables()) return e;
+  DICE_TRY(require_tables());
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
     return set_error(DICE_UNAVAILABLE, "HIP device %d is not available", device);
-  dice_ctx* ctx = new dice_ctx();
+  std::unique_ptr<dice_ctx> ctx(new dice_ctx());
   ctx->device = device;
   ctx->hbm_budget = hbm_budget;
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete ctx;
-    return set_error(DICE_UNAVAILABLE, "cannot create a HIP stream: %s", hipGetErrorString(e));
-  }
-  for (hipEvent_t& event : ctx->events) {
-    if (hipEventCreate(&event) != hipSuccess) {
-      dice_ctx_destroy(ctx);
+  if (e != hipSuccess)
+    return set_error(DICE_UNAVAILABLE, "cannot select HIP device %d: %s", device,
+                     hipGetErrorString(e));
+  if (int32_t rc = ctx->tables.upload()) return rc == DICE_INTERNAL ? DICE_UNAVAILABLE : rc;
+  ctx->runner.reset(new Runner(device, &ctx->tables));
+  if (ctx->runner->init() != DICE_OK)
+    return set_error(DICE_UNAVAILABLE, "cannot create the HIP stream and events: %s",
+                     error_buffer());
+  ctx->stream = ctx->runner->stream();
+  for (hipEvent_t& event : ctx->events)
+    if (hipEventCreate(&event) != hipSuccess)
       return set_error(DICE_UNAVAILABLE, "cannot create HIP events");
-    }
-  }
-  if (int32_t rc = upload_tables(ctx)) {
-    dice_ctx_destroy(ctx);
-    return rc;
-  }
-  *out = ctx;
+  *out = ctx.release();
   return DICE_OK;
 }
 
 void dice_ctx_destroy(dice_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->runner.reset();
   for (hipEvent_t event : ctx->events)
     if (event) (void)hipEventDestroy(event);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
 int32_t dice_response_lut(float* lut) {
-  if (int32_t e = require_tables()) return e;
+  DICE_TRY(require_tables());
   memcpy(lut, tables().lut.data(), 65536 * 4);
   return DICE_OK;
 }
 
 int32_t dice_inverse_factors(uint32_t* high, uint32_t* low) {
-  if (int32_t e = require_tables()) return e;
+  DICE_TRY(require_tables());
   memcpy(high, tables().high.data(), 256 * 4);
   memcpy(low, tables().low.data(), 256 * 4);
   return DICE_OK;
@@ -909,46 +466,19 @@ int32_t dice_stage_table_from_schedule(const dice_schedule* schedule,
 
 int32_t dice_producer_schedule(dice_ctx* ctx, const uint16_t* rgbi, int64_t height,
                                int64_t width, const float* lut, dice_schedule* out) {
-  if (int32_t e = activate(ctx)) return e;
+  DICE_TRY(activate(ctx));
   std::lock_guard<std::mutex> lock(ctx->mutex);
   if (height <= 0) return set_error(DICE_INVALID, "producer record schedule cannot be empty");
   if (width / 8 <= 0) return set_error(DICE_INVALID, "producer active width is outside the RGBI row");
   if (lut && memcmp(lut, tables().lut.data(), 65536 * 4) != 0)
     return set_error(DICE_INVALID, "response LUT is not the hash-pinned Nikon LUT");
-  JobDev J;
-  if (int32_t e = prepare_job(ctx, height, width, false, J)) return e;
-  if (int32_t e = bind_io(ctx, J, rgbi, nullptr, 0, nullptr)) return e;
-  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, ctx->stream));
-  HIP_TRY(launch_producer(J, 1.0f, ctx->stream));
-  ScheduleDev S;
-  const size_t H = (size_t)height;
-  const size_t sizes[12] = {8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4};
-  for (int i = 0; i < 12; ++i) HIP_TRY(ctx->sched[i].ensure(H * sizes[i]));
-  S.weighted_visible_sum = ctx->sched[0].as<double>();
-  S.weighted_infrared_sum = ctx->sched[1].as<double>();
-  S.weight_sum = ctx->sched[2].as<double>();
-  S.mean_visible = ctx->sched[3].as<float>();
-  S.mean_infrared = ctx->sched[4].as<float>();
-  S.accepted = ctx->sched[5].as<uint32_t>();
-  S.scaled_input = ctx->sched[6].as<float>();
-  S.input_scale = ctx->sched[7].as<float>();
-  S.scale_denominator = ctx->sched[8].as<float>();
-  S.scale_numerator = ctx->sched[9].as<float>();
-  S.scale_add_denominator = ctx->sched[10].as<float>();
-  S.scale_add_numerator = ctx->sched[11].as<float>();
-  HIP_TRY(launch_schedule_rows(J, S, ctx->stream));
-  void* dst[12] = {out->weighted_visible_sum, out->weighted_infrared_sum, out->weight_sum,
-                   out->mean_visible,         out->mean_infrared,         out->accepted,
-                   out->scaled_input,         out->input_scale,           out->scale_denominator,
-                   out->scale_numerator,      out->scale_add_denominator, out->scale_add_numerator};
-  for (int i = 0; i < 12; ++i)
-    HIP_TRY(hipMemcpyAsync(dst[i], ctx->sched[i].ptr, H * sizes[i], hipMemcpyDeviceToHost,
-                           ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  dice_params neutral;
+  memset(&neutral, 0, sizeof neutral);
+  DICE_TRY(ctx->runner->producer_schedule(rgbi, height, width, neutral, out));
   out->rows = height;
   out->active_width = (width / 8) * 8;
   // producer_parameters.py:462-479: the 0x40-byte payload of every row.
-  for (size_t y = 0; y < H; ++y) {
+  for (int64_t y = 0; y < height; ++y) {
     uint8_t* p = out->payloads + y * 64;
     memset(p, 0, 64);
     put32(p, 0x04, &out->input_scale[y]);
@@ -970,13 +500,12 @@ int32_t dice_replay(dice_ctx* ctx, const uint16_t* rgbi, int64_t height, int64_t
                     uint16_t* out_rgb16, dice_replay_result* result,
                     dice_diagnostics* diagnostics, const dice_callbacks* callbacks,
                     dice_stage_timings* timings) {
-  if (int32_t e = activate(ctx)) return e;
+  DICE_TRY(activate(ctx));
   std::lock_guard<std::mutex> lock(ctx->mutex);
   memset(result, 0, sizeof *result);
   if (timings) memset(timings, 0, sizeof *timings);
   const double start = now_seconds();
-  Callbacks cb{callbacks};
-  if (int32_t e = validate_replay(height, width, *params)) return e;
+  DICE_TRY(validate_replay(height, width, *params));
   if (rng_state > kLcgMask) return set_error(DICE_INVALID, "LCG24 state must fit in 24 bits");
   if (stages->count < height + 6)
     return set_error(DICE_INVALID, "stage table must cover stages 1..H+6");
@@ -991,28 +520,20 @@ int32_t dice_replay(dice_ctx* ctx, const uint16_t* rgbi, int64_t height, int64_t
       return set_error(DICE_INVALID,
                        "alpha==1 requires the runtime-configured auxiliary replacement");
   }
-  JobDev J;
-  const bool want_diag = diagnostics && (diagnostics->score_plane || diagnostics->at_floor_mask);
-  if (int32_t e = prepare_job(ctx, height, width, want_diag, J)) return e;
-  if (int32_t e = bind_io(ctx, J, rgbi, out_rgb16, flags, timings)) return e;
-  hipStream_t s = ctx->stream;
-  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.alpha, stages->aux_alpha, count * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.offset, stages->aux_offset, count * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.replacement, stages->aux_replacement, count * 4,
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.has_replacement, stages->aux_has_replacement, count,
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.score_base, stages->score_base, count * 4,
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.coarse, stages->coarse_reference, count * 4,
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.gate_secondary, stages->gate_secondary, count,
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(J.stages.row_gate, stages->row_gate, count * 4,
-                         hipMemcpyHostToDevice, s));
-  int32_t rc = replay_core(ctx, J, *params, rng_state, flags, out_rgb16, result, diagnostics,
-                           cb, timings);
+  JobRequest R;
+  R.main = rgbi;
+  R.height = height;
+  R.width = width;
+  R.flags = flags;
+  R.out = out_rgb16;
+  R.params = *params;
+  R.stages = stages;
+  R.rng_state = rng_state;
+  R.diag = diagnostics;
+  R.cb = Callbacks{callbacks};
+  R.result = result;
+  R.timings = timings;
+  const int32_t rc = run_single(ctx, R, timings != nullptr);
   if (timings) timings->total = now_seconds() - start;
   return rc;
 }
@@ -1025,67 +546,107 @@ int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t heigh
                          uint16_t* out_rgb16, dice_replay_result* result,
                          dice_diagnostics* diagnostics, const dice_callbacks* callbacks,
                          dice_stage_timings* timings) {
-  if (int32_t e = activate(ctx)) return e;
+  DICE_TRY(activate(ctx));
   std::lock_guard<std::mutex> lock(ctx->mutex);
   memset(result, 0, sizeof *result);
   if (timings) memset(timings, 0, sizeof *timings);
   const double start = now_seconds();
   Callbacks cb{callbacks};
-  // engine.py:131-145 / profile.py:75-106 (shape part of validate_job).
   cb.progress(0, 0, 1, 0, 0);
   if (height < 5 || width < 8)
     return set_error(DICE_INVALID, "main RGBI frame must be at least 5 rows by 8 columns");
-  if (int32_t e = validate_replay(height, width, *params)) return e;
+  DICE_TRY(validate_replay(height, width, *params));
   if (cb.cancelled()) return cancelled_error();
   cb.progress(0, 1, 1, 0, 0);
-
   cb.progress(1, 0, 1, 0, 0);
-  double t0 = now_seconds();
-  dice_prepass_result prepass;
-  if (int32_t e = host_prepass(prepass_rgbi, prepass_height, prepass_width,
-                               tables().lut.data(), *prepass_params, &prepass))
-    return e;
-  // profile.py:48-55 and the parameter dataclasses' finiteness checks on
-  // the record-derived fields (replaced per row by the provider).
-  {
-    const float alpha = get32f(prepass.payload, 0x04);
-    const float mean_r = get32f(prepass.payload, 0x28);
-    const float mean_ir = get32f(prepass.payload, 0x2C);
-    const double denominator = 1.0 - (double)alpha;
-    if (denominator == 0.0) return set_error(DICE_INVALID, "prepass alpha cannot equal one");
-    const float base = (float)(((double)mean_ir - (double)alpha * (double)mean_r) / denominator);
-    if (!std::isfinite(alpha)) return set_error(DICE_INVALID, "alpha must be finite");
-    if (!std::isfinite(base)) return set_error(DICE_INVALID, "alpha_one_replacement must be finite");
-  }
-  if (timings) timings->prepass = now_seconds() - t0;
+  double prepass_seconds = 0.0;
+  DICE_TRY(admit_job(prepass_rgbi, prepass_height, prepass_width, height, width, *params,
+                     *prepass_params, &prepass_seconds));
+  if (timings) timings->prepass = prepass_seconds;
   if (cb.cancelled()) return cancelled_error();
   cb.progress(1, 1, 1, 0, 0);
-
-  cb.progress(2, 0, 1, 0, 0);
-  JobDev J;
-  const bool want_diag = diagnostics && (diagnostics->score_plane || diagnostics->at_floor_mask);
-  if (int32_t e = prepare_job(ctx, height, width, want_diag, J)) return e;
-  if (int32_t e = bind_io(ctx, J, main_rgbi, out_rgb16, flags, timings)) return e;
-  double t1 = now_seconds();
-  HIP_TRY(hipMemsetAsync(J.error_flags, 0, 4, ctx->stream));
-  EventTimer ev{ctx, timings != nullptr};
-  if (int32_t e = ev.record(0)) return e;
-  HIP_TRY(launch_producer(J, 1.0f, ctx->stream));
-  if (int32_t e = ev.record(1)) return e;
-  if (timings) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    timings->producer_device = now_seconds() - t1;
-    timings->producer_kernels_ms = ev.elapsed_ms(0, 1);
-  }
-  if (cb.cancelled()) return cancelled_error();
-  cb.progress(2, 1, 1, 0, 0);
-  int32_t rc = replay_core(ctx, J, *params, 0x3045u, flags, out_rgb16, result, diagnostics,
-                           cb, timings);
+  JobRequest R;
+  R.main = main_rgbi;
+  R.height = height;
+  R.width = width;
+  R.flags = flags;
+  R.out = out_rgb16;
+  R.params = *params;
+  R.diag = diagnostics;
+  R.cb = cb;
+  R.result = result;
+  R.timings = timings;
+  const int32_t rc = run_single(ctx, R, timings != nullptr);
   if (rc == DICE_OK) cb.progress(4, 1, 1, 0, 0);
   if (timings) timings->total = now_seconds() - start;
   return rc;
 }
 
+int32_t dice_process_batch(dice_ctx* ctx, dice_job* jobs, int32_t count, uint32_t batch_flags,
+                           dice_batch_report* report) {
+  DICE_TRY(activate(ctx));
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  if (report) memset(report, 0, sizeof *report);
+  if (count < 1 || count > kMaxBatch)
+    return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
+  const double start = now_seconds();
+  std::vector<JobRequest> requests(count);
+  for (int32_t j = 0; j < count; ++j) {
+    dice_job& job = jobs[j];
+    JobRequest& R = requests[j];
+    job.status = DICE_OK;
+    job.error[0] = 0;
+    if (job.result) memset(job.result, 0, sizeof *job.result);
+    if (job.timings) memset(job.timings, 0, sizeof *job.timings);
+    R.main = job.main_rgbi;
+    R.height = job.height;
+    R.width = job.width;
+    R.flags = job.flags;
+    R.out = job.out_rgb16;
+    R.result = job.result;
+    R.timings = job.timings;
+    if (!job.params || !job.prepass_params || !job.result || !job.main_rgbi || !job.out_rgb16) {
+      R.code = set_error(DICE_INVALID, "batch job %d is missing a required pointer", j);
+      R.error = error_buffer();
+      continue;
+    }
+    R.params = *job.params;
+    double prepass_seconds = 0.0;
+    if (int32_t rc = admit_job(job.prepass_rgbi, job.prepass_height, job.prepass_width,
+                               job.height, job.width, *job.params, *job.prepass_params,
+                               &prepass_seconds)) {
+      R.code = rc;
+      R.error = error_buffer();
+    }
+    if (job.timings) job.timings->prepass = prepass_seconds;
+  }
+  RunReport rr;
+  if (batch_flags & DICE_BATCH_AUTOTUNE) DICE_TRY(ctx->runner->autotune(requests.data(), count, &rr));
+  RunOptions options;
+  options.graph = (batch_flags & DICE_BATCH_GRAPH) != 0;
+  DICE_TRY(ctx->runner->run(requests.data(), count, options, &rr));
+  int32_t first_error = DICE_OK;
+  for (int32_t j = 0; j < count; ++j) {
+    dice_job& job = jobs[j];
+    job.status = requests[j].code;
+    snprintf(job.error, sizeof job.error, "%s", requests[j].error.c_str());
+    if (job.timings) job.timings->total = now_seconds() - start;
+    if (job.status != DICE_OK && first_error == DICE_OK) {
+      first_error = job.status;
+      set_error(job.status, "batch job %d: %s", j, job.error);
+    }
+  }
+  if (report) {
+    report->tile_rows = rr.tile_rows;
+    report->tile_cols = rr.tile_cols;
+    report->graph_replayed = rr.graph_replayed ? 1 : 0;
+    report->phase_a_ms = rr.phase_a_ms;
+    report->phase_b_ms = rr.phase_b_ms;
+    report->workspace_bytes = rr.workspace_bytes;
+    report->wall_seconds = now_seconds() - start;
+  }
+  return first_error;
+}
 
 int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches, int64_t count,
                                  double* out3) {
@@ -1151,7 +712,7 @@ int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count, uint16_t*
     if (!dv || !dout || !dflags) return set_error(DICE_INTERNAL, "hipMalloc failed");
     HIP_TRY(hipMemcpy(dv, values, count * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dflags, 0, 4));
-    HIP_TRY(launch_t_emit(dv, count, dout, ctx->high.as<uint32_t>(), ctx->low.as<uint32_t>(),
+    HIP_TRY(launch_t_emit(dv, count, dout, ctx->tables.high.as<uint32_t>(), ctx->tables.low.as<uint32_t>(),
                           dflags, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     uint32_t flags = 0;

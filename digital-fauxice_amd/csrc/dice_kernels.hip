@@ -14,6 +14,11 @@
 // (producer schedule and setter).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "dice_device.hpp"
 #include "dice_kernels.hpp"
 
@@ -72,6 +77,39 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value,
   *total = sum;
   __syncthreads();
   return offset + inclusive - value;
+}
+
+// Job of work unit u: the last j with base[j] <= u (empty ranges skipped).
+__device__ __forceinline__ int job_of(const JobSet& set, uint32_t u) {
+  int lo = 0, hi = set.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (set.base[mid] <= u)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent loop over the work units of a job set.  The grid is sized to
+// the machine, not to the work; each round hands every XCD a contiguous run
+// of units (neighbouring tiles share halo rows in that XCD's L2).  Units are
+// uniform per workgroup, so bodies may synchronize; the barrier after each
+// unit lets the next one reuse LDS.
+template <class Body>
+__device__ __forceinline__ void for_each_unit(const JobSet& set, Body&& body) {
+  const uint32_t total = set.base[set.count];
+  const uint32_t me = (uint32_t)xcd_contiguous_index(blockIdx.x, gridDim.x);
+  for (uint32_t first = 0; first < total; first += gridDim.x) {
+    const uint32_t u = first + me;
+    if (u < total) {
+      const int j = job_of(set, u);
+      const JobDev& J = set.jobs[j];
+      if (!J.skip) body(J, (int)(u - set.base[j]));
+    }
+    __syncthreads();
+  }
 }
 
 // streaming.py:157-198: four nine-sample bars, edge-clamped by `aux_at`.
@@ -179,101 +217,103 @@ __device__ __forceinline__ void producer_center(const float m[4], float out[4]) 
 //     the epoch's two additions once.
 constexpr int kProducerThreads = 256;
 
-__global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobDev J) {
+__global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobSet set) {
   __shared__ double pairs[kProducerThreads][8];
-  const int H = J.height, W = J.width;
-  const int cells = W / 8;
-  const int b = blockIdx.x;
-  const bool epoch = b < H / 8;
-  const int tid = threadIdx.x;
-  const double threshold = (double)kProducerThreshold;
-  double denominator = 0.0, numerator = 0.0;
-  for (int c0 = 0; c0 < cells; c0 += kProducerThreads) {
-    const int c = c0 + tid;
-    if (c < cells) {
-      int fail = 64;
-      float raw_sum = 0.0f;
-      QuadrantTree vis[2], inf[2];
-      float vm[4] = {0.0f, 0.0f, 0.0f, 0.0f}, im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int r = 0; r < 8; ++r) {
-        const int y = b * 8 + r;
-        if (y >= H) break;
-        const uint2* row = reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + c * 8) * 4);
-        uint2 q[8];
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const int H = J.height, W = J.width;
+    const int cells = W / 8;
+    const int b = unit;
+    const bool epoch = b < H / 8;
+    const int tid = threadIdx.x;
+    const double threshold = (double)kProducerThreshold;
+    double denominator = 0.0, numerator = 0.0;
+    for (int c0 = 0; c0 < cells; c0 += kProducerThreads) {
+      const int c = c0 + tid;
+      if (c < cells) {
+        int fail = 64;
+        float raw_sum = 0.0f;
+        QuadrantTree vis[2], inf[2];
+        float vm[4] = {0.0f, 0.0f, 0.0f, 0.0f}, im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int r = 0; r < 8; ++r) {
+          const int y = b * 8 + r;
+          if (y >= H) break;
+          const uint2* row = reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + c * 8) * 4);
+          uint2 q[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = row[k];
-        double v[8], ir[8];
+          for (int k = 0; k < 8; ++k) q[k] = row[k];
+          double v[8], ir[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const uint32_t red = q[k].x & 0xFFFFu, infrared = q[k].y >> 16;
-          if (fail == 64 && !(threshold < (double)infrared)) fail = r * 8 + k;
-          raw_sum = (float)((double)raw_sum + (double)infrared);
-          v[k] = (double)J.lut[red];
-          ir[k] = (double)J.lut[infrared];
+          for (int k = 0; k < 8; ++k) {
+            const uint32_t red = q[k].x & 0xFFFFu, infrared = q[k].y >> 16;
+            if (fail == 64 && !(threshold < (double)infrared)) fail = r * 8 + k;
+            raw_sum = (float)((double)raw_sum + (double)infrared);
+            v[k] = (double)J.lut[red];
+            ir[k] = (double)J.lut[infrared];
+          }
+          const int local = r & 3, half = (r >> 2) * 2;
+          vis[0].row(local, v[0], v[1], v[2], v[3], &vm[half]);
+          vis[1].row(local, v[4], v[5], v[6], v[7], &vm[half + 1]);
+          inf[0].row(local, ir[0], ir[1], ir[2], ir[3], &im[half]);
+          inf[1].row(local, ir[4], ir[5], ir[6], ir[7], &im[half + 1]);
         }
-        const int local = r & 3, half = (r >> 2) * 2;
-        vis[0].row(local, v[0], v[1], v[2], v[3], &vm[half]);
-        vis[1].row(local, v[4], v[5], v[6], v[7], &vm[half + 1]);
-        inf[0].row(local, ir[0], ir[1], ir[2], ir[3], &im[half]);
-        inf[1].row(local, ir[4], ir[5], ir[6], ir[7], &im[half + 1]);
-      }
-      J.failpos[(size_t)b * cells + c] = (uint8_t)fail;
-      if (epoch) {
-        double* out = pairs[tid];
-        if (fail != 64) {
+        J.failpos[(size_t)b * cells + c] = (uint8_t)fail;
+        if (epoch) {
+          double* out = pairs[tid];
+          if (fail != 64) {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) out[j] = 0.0;
-        } else {
-          float vd[4], id[4];
-          producer_center(vm, vd);
-          producer_center(im, id);
-          const double low = (double)-0.125f, full_low = (double)0.0f;
-          const double full_high = (double)0.3f, high = (double)0.425f;
-          const double outer_weight = (double)0.70710677f;
-          const double raw_wide = (double)raw_sum;
+            for (int j = 0; j < 8; ++j) out[j] = 0.0;
+          } else {
+            float vd[4], id[4];
+            producer_center(vm, vd);
+            producer_center(im, id);
+            const double low = (double)-0.125f, full_low = (double)0.0f;
+            const double full_high = (double)0.3f, high = (double)0.425f;
+            const double outer_weight = (double)0.70710677f;
+            const double raw_wide = (double)raw_sum;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double visible_wide = (double)vd[q];
-            double ratio, weight;
-            if (visible_wide == 0.0) {
-              ratio = 0.0;
-              weight = 0.0;
-            } else {
-              ratio = (double)id[q] / visible_wide;
-              if (ratio < low || ratio > high)
+            for (int q = 0; q < 4; ++q) {
+              const double visible_wide = (double)vd[q];
+              double ratio, weight;
+              if (visible_wide == 0.0) {
+                ratio = 0.0;
                 weight = 0.0;
-              else if (ratio < full_low || ratio > full_high)
-                weight = visible_wide * outer_weight;
-              else
-                weight = visible_wide;
+              } else {
+                ratio = (double)id[q] / visible_wide;
+                if (ratio < low || ratio > high)
+                  weight = 0.0;
+                else if (ratio < full_low || ratio > full_high)
+                  weight = visible_wide * outer_weight;
+                else
+                  weight = visible_wide;
+              }
+              double term = weight * weight;
+              term *= raw_wide;
+              term *= raw_wide;
+              out[q * 2 + 0] = term;
+              out[q * 2 + 1] = ratio * term;
             }
-            double term = weight * weight;
-            term *= raw_wide;
-            term *= raw_wide;
-            out[q * 2 + 0] = term;
-            out[q * 2 + 1] = ratio * term;
           }
         }
       }
-    }
-    if (!epoch) continue;
-    __syncthreads();
-    if (tid == 0) {
-      const int n = min(kProducerThreads, cells - c0);
-      for (int j = 0; j < n; ++j) {
+      if (!epoch) continue;
+      __syncthreads();
+      if (tid == 0) {
+        const int n = min(kProducerThreads, cells - c0);
+        for (int j = 0; j < n; ++j) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          denominator += pairs[j][q * 2 + 0];
-          numerator += pairs[j][q * 2 + 1];
+          for (int q = 0; q < 4; ++q) {
+            denominator += pairs[j][q * 2 + 0];
+            numerator += pairs[j][q * 2 + 1];
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if (epoch && tid == 0) {
-    J.epoch_adds[b * 2 + 0] = (double)(float)denominator;
-    J.epoch_adds[b * 2 + 1] = (double)(float)numerator;
-  }
+    if (epoch && tid == 0) {
+      J.epoch_adds[b * 2 + 0] = (double)(float)denominator;
+      J.epoch_adds[b * 2 + 1] = (double)(float)numerator;
+    }
+  });
 }
 
 // Per-row binary64 sums in column order (producer_parameters.py:303-360):
@@ -288,64 +328,66 @@ constexpr int kRowSumRows = 16;
 constexpr int kRowSumCols = 128;
 constexpr int kRowSumStagers = 256;
 
-__global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobDev J) {
+__global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobSet set) {
   __shared__ float s_visible[3][kRowSumRows][kRowSumCols + 1];
   __shared__ float s_infrared[3][kRowSumRows][kRowSumCols + 1];
   __shared__ uint32_t s_weight[3][kRowSumRows][kRowSumCols + 1];
-  const int H = J.height, W = J.width;
-  const int cells = W / 8, active = cells * 8;
-  const int row0 = blockIdx.x * kRowSumRows;
-  const int tid = threadIdx.x;
-  const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
-  double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
-  uint32_t accepted = 0;
-  for (int k = 0; k <= chunks; ++k) {
-    if (tid >= 64 && k < chunks) {
-      const int buffer = k % 3, c0 = k * kRowSumCols;
-      constexpr int kPer = kRowSumRows * kRowSumCols / kRowSumStagers;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const int H = J.height, W = J.width;
+    const int cells = W / 8, active = cells * 8;
+    const int row0 = unit * kRowSumRows;
+    const int tid = threadIdx.x;
+    const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
+    double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
+    uint32_t accepted = 0;
+    for (int k = 0; k <= chunks; ++k) {
+      if (tid >= 64 && k < chunks) {
+        const int buffer = k % 3, c0 = k * kRowSumCols;
+        constexpr int kPer = kRowSumRows * kRowSumCols / kRowSumStagers;
 #pragma unroll
-      for (int i = 0; i < kPer; ++i) {
-        const int e = (tid - 64) + i * kRowSumStagers;
-        const int r = e / kRowSumCols, c = e % kRowSumCols;
-        const int y = row0 + r, x = c0 + c;
-        float lut_visible = 0.0f, lut_infrared = 0.0f;
-        uint32_t weight = 0;
-        if (y < H && x < active) {
-          uint32_t lanes[4];
-          load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-          const int position = (y & 7) * 8 + (x & 7);
-          if (position < (int)J.failpos[(size_t)(y >> 3) * cells + (x >> 3)]) {
-            lut_visible = J.lut[lanes[0]];
-            lut_infrared = J.lut[lanes[3]];
-            weight = lanes[3] * lanes[3];
+        for (int i = 0; i < kPer; ++i) {
+          const int e = (tid - 64) + i * kRowSumStagers;
+          const int r = e / kRowSumCols, c = e % kRowSumCols;
+          const int y = row0 + r, x = c0 + c;
+          float lut_visible = 0.0f, lut_infrared = 0.0f;
+          uint32_t weight = 0;
+          if (y < H && x < active) {
+            uint32_t lanes[4];
+            load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+            const int position = (y & 7) * 8 + (x & 7);
+            if (position < (int)J.failpos[(size_t)(y >> 3) * cells + (x >> 3)]) {
+              lut_visible = J.lut[lanes[0]];
+              lut_infrared = J.lut[lanes[3]];
+              weight = lanes[3] * lanes[3];
+            }
           }
+          s_visible[buffer][r][c] = lut_visible;
+          s_infrared[buffer][r][c] = lut_infrared;
+          s_weight[buffer][r][c] = weight;
         }
-        s_visible[buffer][r][c] = lut_visible;
-        s_infrared[buffer][r][c] = lut_infrared;
-        s_weight[buffer][r][c] = weight;
       }
-    }
-    __syncthreads();
-    if (tid < kRowSumRows && k >= 1) {
-      const int buffer = (k - 1) % 3;
+      __syncthreads();
+      if (tid < kRowSumRows && k >= 1) {
+        const int buffer = (k - 1) % 3;
 #pragma unroll 8
-      for (int c = 0; c < kRowSumCols; ++c) {
-        const uint32_t w = s_weight[buffer][tid][c];
-        const double weight = (double)w;
-        sum_visible += weight * (double)s_visible[buffer][tid][c];
-        sum_infrared += weight * (double)s_infrared[buffer][tid][c];
-        sum_weight += weight;
-        accepted += w != 0u;
+        for (int c = 0; c < kRowSumCols; ++c) {
+          const uint32_t w = s_weight[buffer][tid][c];
+          const double weight = (double)w;
+          sum_visible += weight * (double)s_visible[buffer][tid][c];
+          sum_infrared += weight * (double)s_infrared[buffer][tid][c];
+          sum_weight += weight;
+          accepted += w != 0u;
+        }
       }
     }
-  }
-  const int y = row0 + tid;
-  if (tid < kRowSumRows && y < H) {
-    J.row_sums[(size_t)y * 3 + 0] = sum_visible;
-    J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
-    J.row_sums[(size_t)y * 3 + 2] = sum_weight;
-    J.accepted[y] = accepted;
-  }
+    const int y = row0 + tid;
+    if (tid < kRowSumRows && y < H) {
+      J.row_sums[(size_t)y * 3 + 0] = sum_visible;
+      J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
+      J.row_sums[(size_t)y * 3 + 2] = sum_weight;
+      J.accepted[y] = accepted;
+    }
+  });
 }
 
 // The strictly sequential cross-row chains (producer_parameters.py:
@@ -355,70 +397,72 @@ __global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobDev J) {
 // flight while the current one is summed).
 constexpr int kChainChunk = 512;
 
-__global__ __launch_bounds__(64) void k_producer_chain(JobDev J) {
+__global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
   __shared__ double in_buffer[kChainChunk * 3];
   __shared__ double out_buffer[kChainChunk * 3];
   __shared__ float out_store[kChainChunk * 2];
-  const int lane = threadIdx.x;
-  constexpr int kPer = kChainChunk * 3 / 64;
-  if (blockIdx.x == 0) {
-    const int H = J.height;
-    const double* __restrict__ source = J.row_sums;
-    double total = 0.0;
-    double next[kPer];
-    auto load = [&](int base) {
-      const int n = min(kChainChunk, H - base) * 3;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const int lane = threadIdx.x;
+    constexpr int kPer = kChainChunk * 3 / 64;
+    if (unit == 0) {
+      const int H = J.height;
+      const double* __restrict__ source = J.row_sums;
+      double total = 0.0;
+      double next[kPer];
+      auto load = [&](int base) {
+        const int n = min(kChainChunk, H - base) * 3;
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        const int i = lane + j * 64;
-        next[j] = i < n ? source[(size_t)base * 3 + i] : 0.0;
-      }
-    };
-    load(0);
-    for (int base = 0; base < H; base += kChainChunk) {
-      const int n = min(kChainChunk, H - base);
+        for (int j = 0; j < kPer; ++j) {
+          const int i = lane + j * 64;
+          next[j] = i < n ? source[(size_t)base * 3 + i] : 0.0;
+        }
+      };
+      load(0);
+      for (int base = 0; base < H; base += kChainChunk) {
+        const int n = min(kChainChunk, H - base);
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) in_buffer[lane + j * 64] = next[j];
-      __syncthreads();
-      if (base + kChainChunk < H) load(base + kChainChunk);
-      if (lane < 3) {
-        for (int r = 0; r < n; ++r) {
-          total += in_buffer[r * 3 + lane];
-          out_buffer[r * 3 + lane] = total;
+        for (int j = 0; j < kPer; ++j) in_buffer[lane + j * 64] = next[j];
+        __syncthreads();
+        if (base + kChainChunk < H) load(base + kChainChunk);
+        if (lane < 3) {
+          for (int r = 0; r < n; ++r) {
+            total += in_buffer[r * 3 + lane];
+            out_buffer[r * 3 + lane] = total;
+          }
         }
+        __syncthreads();
+        for (int i = lane; i < n * 3; i += 64) J.cumulative[(size_t)base * 3 + i] = out_buffer[i];
+        __syncthreads();
       }
-      __syncthreads();
-      for (int i = lane; i < n * 3; i += 64) J.cumulative[(size_t)base * 3 + i] = out_buffer[i];
-      __syncthreads();
-    }
-  } else {
-    const int epochs = J.height / 8;
-    const double* __restrict__ source = J.epoch_adds;
-    float denominator = 0.0f, numerator = 0.0f;
-    for (int base = 0; base < epochs; base += kChainChunk) {
-      const int n = min(kChainChunk, epochs - base);
-      for (int i = lane; i < n * 2; i += 64) in_buffer[i] = source[(size_t)base * 2 + i];
-      __syncthreads();
-      if (lane == 0) {
-        for (int e = 0; e < n; ++e) {
-          const double denominator_wide = (double)denominator + in_buffer[e * 2 + 0];
-          const double numerator_wide = (double)numerator + in_buffer[e * 2 + 1];
-          denominator = (float)denominator_wide;
-          numerator = (float)numerator_wide;
-          out_buffer[e * 2 + 0] = denominator_wide;
-          out_buffer[e * 2 + 1] = numerator_wide;
-          out_store[e * 2 + 0] = denominator;
-          out_store[e * 2 + 1] = numerator;
+    } else {
+      const int epochs = J.height / 8;
+      const double* __restrict__ source = J.epoch_adds;
+      float denominator = 0.0f, numerator = 0.0f;
+      for (int base = 0; base < epochs; base += kChainChunk) {
+        const int n = min(kChainChunk, epochs - base);
+        for (int i = lane; i < n * 2; i += 64) in_buffer[i] = source[(size_t)base * 2 + i];
+        __syncthreads();
+        if (lane == 0) {
+          for (int e = 0; e < n; ++e) {
+            const double denominator_wide = (double)denominator + in_buffer[e * 2 + 0];
+            const double numerator_wide = (double)numerator + in_buffer[e * 2 + 1];
+            denominator = (float)denominator_wide;
+            numerator = (float)numerator_wide;
+            out_buffer[e * 2 + 0] = denominator_wide;
+            out_buffer[e * 2 + 1] = numerator_wide;
+            out_store[e * 2 + 0] = denominator;
+            out_store[e * 2 + 1] = numerator;
+          }
         }
+        __syncthreads();
+        for (int i = lane; i < n * 2; i += 64) {
+          J.epoch_wide[(size_t)base * 2 + i] = out_buffer[i];
+          J.epoch_store[(size_t)base * 2 + i] = out_store[i];
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      for (int i = lane; i < n * 2; i += 64) {
-        J.epoch_wide[(size_t)base * 2 + i] = out_buffer[i];
-        J.epoch_store[(size_t)base * 2 + i] = out_store[i];
-      }
-      __syncthreads();
     }
-  }
+  });
 }
 
 namespace {
@@ -481,38 +525,40 @@ __global__ __launch_bounds__(kThreads) void k_schedule_rows(JobDev J,
 // ContentDerivedStageParameterProvider (producer_parameters.py:498-541) +
 // derive_x3a_setter_fields / StageCalibration (stage_parameters.py:93-197):
 // stage s reads record row min(s-1, H-1).
-__global__ __launch_bounds__(kThreads) void k_stage_table(JobDev J,
-                                                          float factor_b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int count = J.height + 6;
-  if (i >= count) return;
-  const int row = i < J.height - 1 ? i : J.height - 1;
-  RowRecord r = row_record(J, row);
-  float p1 = r.p1, p2 = r.p2, p3 = r.mean_visible, p4 = r.mean_infrared;
-  uint32_t error = 0;
-  if (!(__builtin_isfinite(p1) && __builtin_isfinite(p2) &&
-        __builtin_isfinite(p3) && __builtin_isfinite(p4)))
-    error |= kErrStageNonfinite;
-  if (p2 < 0.0f) {
-    p1 = -p1;
-    p2 = -p2;
-  }
-  const double denominator_p1 = 1.0 - (double)p1;
-  const double denominator_p2 = 1.0 - (double)p2;
-  if (denominator_p1 == 0.0 || denominator_p2 == 0.0) error |= kErrSetterScaleOne;
-  const float base = (float)(((double)p4 - (double)p3 * (double)p2) / denominator_p2);
-  const bool gate = !(p3 != 0.0f && p2 <= 0.075f);
-  const float alpha = p2 * factor_b;
-  if (!(__builtin_isfinite(alpha) && __builtin_isfinite(base))) error |= kErrStageNonfinite;
-  if (error) atomicOr(J.error_flags, error);
-  J.stages.alpha[i] = alpha;
-  J.stages.offset[i] = 1.0f;
-  J.stages.replacement[i] = base;
-  J.stages.has_replacement[i] = 1;
-  J.stages.score_base[i] = base;
-  J.stages.coarse[i] = base;
-  J.stages.gate_secondary[i] = gate ? 1 : 0;
-  J.stages.row_gate[i] = 0;
+__global__ __launch_bounds__(kThreads) void k_stage_table(JobSet set) {
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const float factor_b = J.factor_b;
+    const int i = unit * blockDim.x + threadIdx.x;
+    const int count = J.height + 6;
+    if (i >= count) return;
+    const int row = i < J.height - 1 ? i : J.height - 1;
+    RowRecord r = row_record(J, row);
+    float p1 = r.p1, p2 = r.p2, p3 = r.mean_visible, p4 = r.mean_infrared;
+    uint32_t error = 0;
+    if (!(__builtin_isfinite(p1) && __builtin_isfinite(p2) &&
+          __builtin_isfinite(p3) && __builtin_isfinite(p4)))
+      error |= kErrStageNonfinite;
+    if (p2 < 0.0f) {
+      p1 = -p1;
+      p2 = -p2;
+    }
+    const double denominator_p1 = 1.0 - (double)p1;
+    const double denominator_p2 = 1.0 - (double)p2;
+    if (denominator_p1 == 0.0 || denominator_p2 == 0.0) error |= kErrSetterScaleOne;
+    const float base = (float)(((double)p4 - (double)p3 * (double)p2) / denominator_p2);
+    const bool gate = !(p3 != 0.0f && p2 <= 0.075f);
+    const float alpha = p2 * factor_b;
+    if (!(__builtin_isfinite(alpha) && __builtin_isfinite(base))) error |= kErrStageNonfinite;
+    if (error) atomicOr(J.error_flags, error);
+    J.stages.alpha[i] = alpha;
+    J.stages.offset[i] = 1.0f;
+    J.stages.replacement[i] = base;
+    J.stages.has_replacement[i] = 1;
+    J.stages.score_base[i] = base;
+    J.stages.coarse[i] = base;
+    J.stages.gate_secondary[i] = gate ? 1 : 0;
+    J.stages.row_gate[i] = 0;
+  });
 }
 
 // ==========================================================================
@@ -555,99 +601,100 @@ struct ScanShared {
 };
 
 template <int TY, int TX>
-__global__ __launch_bounds__(kThreads) void k_scan(JobDev J, DevParams P) {
+__global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
   using Sh = ScanShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC;
   static_assert((TY * TX) % kThreads == 0, "tile must be a multiple of the block");
   __shared__ Sh S;
-  const int tiles = J.tiles_x * J.tiles_y;
-  const int t = xcd_contiguous_index(blockIdx.x, gridDim.x);
-  if (t >= tiles) return;
-  const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
-  const int H = J.height, W = J.width;
-  const int y0 = tile_y * TY, x0 = tile_x * TX;
-  const int tid = threadIdx.x;
-  const float floor = P.score_floor;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const DevParams& P = J.P;
+    const int t = unit;
+    const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
+    const int H = J.height, W = J.width;
+    const int y0 = tile_y * TY, x0 = tile_x * TX;
+    const int tid = threadIdx.x;
+    const float floor = P.score_floor;
 
-  for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
-  for (int i = tid; i < TY; i += kThreads) {
-    const int y = min(y0 + i, H - 1);
-    S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
-    S.writer_gate[i] = J.stages.row_gate[y + 6];
-    S.row_count[i] = 0;
-  }
-  __syncthreads();
-
-  // x3a.py:193-221 on the clamped halo; no-op emit (output.py:76-94) of the
-  // centres.
-  uint32_t aux_error = 0;
-  for (int i = tid; i < AR * AC; i += kThreads) {
-    const int r = i / AC, c = i % AC;
-    const int ly = y0 - 4 + r, lx = x0 - 4 + c;
-    uint32_t lanes[4];
-    load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
-    const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
-    const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
-    const float visible =
-        P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
-    const RowParams& rp = S.rows[r];
-    const float a = rp.is_one ? rp.replacement : auxiliary(visible, work3, rp.alpha, rp.offset);
-    if (!__builtin_isfinite(a)) aux_error = kErrNonfiniteAux;
-    S.aux[r][c] = a;
-    const int ty = r - 4, tx = c - 4;
-    if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
-      uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
-      o[0] = emit_value(work0, J.factor_high, J.factor_low);
-      o[1] = emit_value(work1, J.factor_high, J.factor_low);
-      o[2] = emit_value(work2, J.factor_high, J.factor_low);
+    for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
+    for (int i = tid; i < TY; i += kThreads) {
+      const int y = min(y0 + i, H - 1);
+      S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
+      S.writer_gate[i] = J.stages.row_gate[y + 6];
+      S.row_count[i] = 0;
     }
-  }
-  if (aux_error) atomicOr(J.error_flags, aux_error);
-  __syncthreads();
+    __syncthreads();
 
-  // Centre score (x3a.py:245-286), decision bars (streaming.py:157-198) and
-  // eligibility (streaming.py:461-466); each thread owns a contiguous run of
-  // centres so the queue comes out row-major.
-  constexpr int kPer = TY * TX / kThreads;
-  uint32_t eligible_bits = 0, eligible_count = 0;
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int li = tid * kPer + k;
-    const int ty = li / TX, tx = li % TX;
-    const int y = y0 + ty, x = x0 + tx;
-    if (y >= H || x >= W) continue;
-    const int ar = ty + 4, ac = tx + 4;
-    float sample = S.aux[ar][ac];
-    if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
-      sample = fmin_plain(fmin_plain(S.aux[ar][ac - 1], S.aux[ar][ac]), S.aux[ar][ac + 1]);
-    const float sc = score_value(sample, S.rows[ar].score_base, P.score_addend,
-                                 P.score_scale, P.score_offset, floor);
-    if (J.diag_score) {
-      J.diag_score[(size_t)y * W + x] = sc;
-      J.diag_floor[(size_t)y * W + x] = sc == floor ? 1 : 0;
+    // x3a.py:193-221 on the clamped halo; no-op emit (output.py:76-94) of the
+    // centres.
+    uint32_t aux_error = 0;
+    for (int i = tid; i < AR * AC; i += kThreads) {
+      const int r = i / AC, c = i % AC;
+      const int ly = y0 - 4 + r, lx = x0 - 4 + c;
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
+      const uint32_t raw_visible =
+          P.visible_channel == 0 ? lanes[0] : (P.visible_channel == 1 ? lanes[1] : lanes[2]);
+      const float visible = J.lut[raw_visible], infrared = J.lut[lanes[3]];
+      const RowParams& rp = S.rows[r];
+      const float a = rp.is_one ? rp.replacement : auxiliary(visible, infrared, rp.alpha, rp.offset);
+      if (!__builtin_isfinite(a)) aux_error = kErrNonfiniteAux;
+      S.aux[r][c] = a;
+      const int ty = r - 4, tx = c - 4;
+      if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
+        // emit(LUT[raw]) is a function of the raw sample alone
+        uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
+        o[0] = J.noop_emit[lanes[0]];
+        o[1] = J.noop_emit[lanes[1]];
+        o[2] = J.noop_emit[lanes[2]];
+      }
     }
-    if (S.writer_gate[ty] != 0) continue;
-    if (S.writer_floor[ty] && sc >= 1.0f) continue;
-    auto aux_at = [&](int dy, int dx) { return S.aux[ar + dy][ac + dx]; };
-    if (decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit)) continue;
-    eligible_bits |= 1u << k;
-    eligible_count += 1;
-  }
-  uint32_t total;
-  uint32_t base = block_exclusive_scan(eligible_count, S.wave_sums, &total);
-  uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+    if (aux_error) atomicOr(J.error_flags, aux_error);
+    __syncthreads();
+
+    // Centre score (x3a.py:245-286), decision bars (streaming.py:157-198) and
+    // eligibility (streaming.py:461-466); each thread owns a contiguous run of
+    // centres so the queue comes out row-major.
+    constexpr int kPer = TY * TX / kThreads;
+    uint32_t eligible_bits = 0, eligible_count = 0;
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    if (eligible_bits & (1u << k)) {
+    for (int k = 0; k < kPer; ++k) {
       const int li = tid * kPer + k;
-      list[base++] = (uint16_t)li;
-      atomicAdd(&S.row_count[li / TX], 1u);
+      const int ty = li / TX, tx = li % TX;
+      const int y = y0 + ty, x = x0 + tx;
+      if (y >= H || x >= W) continue;
+      const int ar = ty + 4, ac = tx + 4;
+      float sample = S.aux[ar][ac];
+      if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
+        sample = fmin_plain(fmin_plain(S.aux[ar][ac - 1], S.aux[ar][ac]), S.aux[ar][ac + 1]);
+      const float sc = score_value(sample, S.rows[ar].score_base, P.score_addend,
+                                   P.score_scale, P.score_offset, floor);
+      if (J.diag_score) {
+        J.diag_score[(size_t)y * W + x] = sc;
+        J.diag_floor[(size_t)y * W + x] = sc == floor ? 1 : 0;
+      }
+      if (S.writer_gate[ty] != 0) continue;
+      if (S.writer_floor[ty] && sc >= 1.0f) continue;
+      auto aux_at = [&](int dy, int dx) { return S.aux[ar + dy][ac + dx]; };
+      if (decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit)) continue;
+      eligible_bits |= 1u << k;
+      eligible_count += 1;
     }
-  }
-  __syncthreads();
-  for (int i = tid; i < TY; i += kThreads)
-    J.seg_count[(size_t)(y0 + i) * J.tiles_x + tile_x] = S.row_count[i];
-  if (tid == 0) J.tile_count[t] = total;
+    uint32_t total;
+    uint32_t base = block_exclusive_scan(eligible_count, S.wave_sums, &total);
+    uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      if (eligible_bits & (1u << k)) {
+        const int li = tid * kPer + k;
+        list[base++] = (uint16_t)li;
+        atomicAdd(&S.row_count[li / TX], 1u);
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < TY; i += kThreads)
+      J.seg_count[(size_t)(y0 + i) * J.tiles_x + tile_x] = S.row_count[i];
+    if (tid == 0) J.tile_count[t] = total;
+  });
 }
 
 template <int TY, int TX>
@@ -667,155 +714,156 @@ struct SitesShared {
 };
 
 template <int TY, int TX>
-__global__ __launch_bounds__(kThreads) void k_sites(JobDev J, DevParams P) {
+__global__ __launch_bounds__(kThreads) void k_sites(JobSet set) {
   using Sh = SitesShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC;
   __shared__ Sh S;
-  const int tiles = J.tiles_x * J.tiles_y;
-  const int t = xcd_contiguous_index(blockIdx.x, gridDim.x);
-  if (t >= tiles) return;
-  const uint32_t total = J.tile_count[t];
-  if (total == 0) return;
-  const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
-  const int H = J.height, W = J.width;
-  const int y0 = tile_y * TY, x0 = tile_x * TX;
-  const int tid = threadIdx.x;
-  const float floor = P.score_floor;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const DevParams& P = J.P;
+    const int t = unit;
+    const uint32_t total = J.tile_count[t];
+    if (total == 0) return;
+    const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
+    const int H = J.height, W = J.width;
+    const int y0 = tile_y * TY, x0 = tile_x * TX;
+    const int tid = threadIdx.x;
+    const float floor = P.score_floor;
 
-  for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
-  if (tid == 0) {
-    uint32_t start = 0;
-    for (int i = 0; i < TY; ++i) {
-      const size_t seg = (size_t)(y0 + i) * J.tiles_x + tile_x;
-      const int y = min(y0 + i, H - 1);
-      S.row_start[i] = start;
-      S.row_offset[i] = J.seg_offset[seg];
-      start += J.seg_count[seg];
-      S.writer_coarse[i] = J.stages.coarse[y + 6];
-      S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
-    }
-  }
-  __syncthreads();
-
-  // Working lanes and aux on the clamped halo; working RGB staged for the
-  // weighted-RGB history.
-  for (int i = tid; i < AR * AC; i += kThreads) {
-    const int r = i / AC, c = i % AC;
-    const int ly = y0 - 5 + r, lx = x0 - 6 + c;
-    uint32_t lanes[4];
-    load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
-    const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
-    const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
-    const float visible =
-        P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
-    const RowParams& rp = S.rows[r];
-    S.aux[r][c] = rp.is_one ? rp.replacement : auxiliary(visible, work3, rp.alpha, rp.offset);
-    const int rr = r - 1, rc = c - 2;
-    if (rr >= 0 && rr < RR && rc >= 0 && rc < RC) {
-      S.wrgb[0][rr][rc] = work0;
-      S.wrgb[1][rr][rc] = work1;
-      S.wrgb[2][rr][rc] = work2;
-    }
-  }
-  __syncthreads();
-
-  // Score / weighted-aux history with the asymmetric guards
-  // (streaming.py:201-233): zero outside the row and above row -1, the
-  // floor pseudo-row at -1, the last row repeated below.
-  for (int i = tid; i < HR * HC; i += kThreads) {
-    const int r = i / HC, c = i % HC;
-    const int ly = y0 - 5 + r, lx = x0 - 5 + c;
-    float s, wa;
-    if (lx < 0 || lx >= W || ly < -1) {
-      s = 0.0f;
-      wa = 0.0f;
-    } else if (ly == -1) {
-      s = floor;
-      wa = S.aux[r][c + 1] * floor;
-    } else {
-      const float a = S.aux[r][c + 1];
-      float sample = a;
-      if (P.horizontal_minimum && W > 2 && lx >= 1 && lx <= W - 2)
-        sample = fmin_plain(fmin_plain(S.aux[r][c], a), S.aux[r][c + 2]);
-      s = score_value(sample, S.rows[r].score_base, P.score_addend, P.score_scale,
-                      P.score_offset, floor);
-      wa = s * a;
-    }
-    S.score[r][c] = s;
-    S.waux[r][c] = wa;
-  }
-  __syncthreads();
-  for (int i = tid; i < RR * RC; i += kThreads) {
-    const int rr = i / RC, rc = i % RC;
-    const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
-    if (lx < 0 || lx >= W || ly < -1) {
-      S.wrgb[0][rr][rc] = 0.0f;
-      S.wrgb[1][rr][rc] = 0.0f;
-      S.wrgb[2][rr][rc] = 0.0f;
-    } else {
-      const float s = S.score[rr + 1][rc + 1];
-      S.wrgb[0][rr][rc] = s * S.wrgb[0][rr][rc];
-      S.wrgb[1][rr][rc] = s * S.wrgb[1][rr][rc];
-      S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
-    }
-  }
-  __syncthreads();
-
-  // One site per thread.  Eligibility already implies the driver gate
-  // passes (reconstruction.py:890-904 re-tests the same row gate and the
-  // same point score), so every listed centre is attempted.
-  const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
-  for (uint32_t q = tid; q < total; q += kThreads) {
-    const int li = list[q];
-    const int ty = li / TX, tx = li % TX;
-    const int y = y0 + ty, x = x0 + tx;
-    const int cr = ty + 5, cc = tx + 5;  // history index of the centre
-    const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
-    const float fallback = S.writer_coarse[ty];
-    auto record = [&](int r, int c, float point_aux, float point_score, float w[4],
-                      float f[4]) {
-      auto score_at = [&](int dy, int dx) { return S.score[r - 4 + dy][c - 4 + dx]; };
-      auto waux_at = [&](int dy, int dx) { return S.waux[r - 4 + dy][c - 4 + dx]; };
-      feature_record(score_at, waux_at, point_aux, point_score, fallback, w, f);
-    };
-    float weights[4], features[5][4];
-    record(cr, cc, S.aux[ar][ac], S.score[cr][cc], weights, features[0]);
-    int record_count = 1;
-    if (P.cross_neighbor) {
-      // streaming.py:259-308: neighbours share the centre row's fallback;
-      // out-of-row neighbours are all-zero records; rows -1 and H take the
-      // pseudo-row / repeated-row point values already in the halo.
-      record_count = 5;
-      float unused[4];
-      if (x > 0) {
-        record(cr, cc - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], unused, features[1]);
-      } else {
-        features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+    for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
+    if (tid == 0) {
+      uint32_t start = 0;
+      for (int i = 0; i < TY; ++i) {
+        const size_t seg = (size_t)(y0 + i) * J.tiles_x + tile_x;
+        const int y = min(y0 + i, H - 1);
+        S.row_start[i] = start;
+        S.row_offset[i] = J.seg_offset[seg];
+        start += J.seg_count[seg];
+        S.writer_coarse[i] = J.stages.coarse[y + 6];
+        S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
       }
-      if (x < W - 1) {
-        record(cr, cc + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], unused, features[2]);
-      } else {
-        features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
-      }
-      record(cr - 1, cc, S.aux[ar - 1][ac], S.score[cr - 1][cc], unused, features[3]);
-      record(cr + 1, cc, S.aux[ar + 1][ac], S.score[cr + 1][cc], unused, features[4]);
     }
-    auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
-    float candidates[3][3];
-    rgb_candidates(wrgb_at, weights, candidates);
-    uint32_t lanes[4];
-    load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-    const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
-    Site site;
-    combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
-                      features, record_count, fallback, P.combine, site.cand);
-    site.orig[0] = original[0];
-    site.orig[1] = original[1];
-    site.orig[2] = original[2];
-    site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
-    J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
-  }
+    __syncthreads();
+
+    // Working lanes and aux on the clamped halo; working RGB staged for the
+    // weighted-RGB history.
+    for (int i = tid; i < AR * AC; i += kThreads) {
+      const int r = i / AC, c = i % AC;
+      const int ly = y0 - 5 + r, lx = x0 - 6 + c;
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
+      const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
+      const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
+      const float visible =
+          P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
+      const RowParams& rp = S.rows[r];
+      S.aux[r][c] = rp.is_one ? rp.replacement : auxiliary(visible, work3, rp.alpha, rp.offset);
+      const int rr = r - 1, rc = c - 2;
+      if (rr >= 0 && rr < RR && rc >= 0 && rc < RC) {
+        S.wrgb[0][rr][rc] = work0;
+        S.wrgb[1][rr][rc] = work1;
+        S.wrgb[2][rr][rc] = work2;
+      }
+    }
+    __syncthreads();
+
+    // Score / weighted-aux history with the asymmetric guards
+    // (streaming.py:201-233): zero outside the row and above row -1, the
+    // floor pseudo-row at -1, the last row repeated below.
+    for (int i = tid; i < HR * HC; i += kThreads) {
+      const int r = i / HC, c = i % HC;
+      const int ly = y0 - 5 + r, lx = x0 - 5 + c;
+      float s, wa;
+      if (lx < 0 || lx >= W || ly < -1) {
+        s = 0.0f;
+        wa = 0.0f;
+      } else if (ly == -1) {
+        s = floor;
+        wa = S.aux[r][c + 1] * floor;
+      } else {
+        const float a = S.aux[r][c + 1];
+        float sample = a;
+        if (P.horizontal_minimum && W > 2 && lx >= 1 && lx <= W - 2)
+          sample = fmin_plain(fmin_plain(S.aux[r][c], a), S.aux[r][c + 2]);
+        s = score_value(sample, S.rows[r].score_base, P.score_addend, P.score_scale,
+                        P.score_offset, floor);
+        wa = s * a;
+      }
+      S.score[r][c] = s;
+      S.waux[r][c] = wa;
+    }
+    __syncthreads();
+    for (int i = tid; i < RR * RC; i += kThreads) {
+      const int rr = i / RC, rc = i % RC;
+      const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
+      if (lx < 0 || lx >= W || ly < -1) {
+        S.wrgb[0][rr][rc] = 0.0f;
+        S.wrgb[1][rr][rc] = 0.0f;
+        S.wrgb[2][rr][rc] = 0.0f;
+      } else {
+        const float s = S.score[rr + 1][rc + 1];
+        S.wrgb[0][rr][rc] = s * S.wrgb[0][rr][rc];
+        S.wrgb[1][rr][rc] = s * S.wrgb[1][rr][rc];
+        S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
+      }
+    }
+    __syncthreads();
+
+    // One site per thread.  Eligibility already implies the driver gate
+    // passes (reconstruction.py:890-904 re-tests the same row gate and the
+    // same point score), so every listed centre is attempted.
+    const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+    for (uint32_t q = tid; q < total; q += kThreads) {
+      const int li = list[q];
+      const int ty = li / TX, tx = li % TX;
+      const int y = y0 + ty, x = x0 + tx;
+      const int cr = ty + 5, cc = tx + 5;  // history index of the centre
+      const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
+      const float fallback = S.writer_coarse[ty];
+      auto record = [&](int r, int c, float point_aux, float point_score, float w[4],
+                        float f[4]) {
+        auto score_at = [&](int dy, int dx) { return S.score[r - 4 + dy][c - 4 + dx]; };
+        auto waux_at = [&](int dy, int dx) { return S.waux[r - 4 + dy][c - 4 + dx]; };
+        feature_record(score_at, waux_at, point_aux, point_score, fallback, w, f);
+      };
+      float weights[4], features[5][4];
+      record(cr, cc, S.aux[ar][ac], S.score[cr][cc], weights, features[0]);
+      int record_count = 1;
+      if (P.cross_neighbor) {
+        // streaming.py:259-308: neighbours share the centre row's fallback;
+        // out-of-row neighbours are all-zero records; rows -1 and H take the
+        // pseudo-row / repeated-row point values already in the halo.
+        record_count = 5;
+        float unused[4];
+        if (x > 0) {
+          record(cr, cc - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], unused, features[1]);
+        } else {
+          features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+        }
+        if (x < W - 1) {
+          record(cr, cc + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], unused, features[2]);
+        } else {
+          features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
+        }
+        record(cr - 1, cc, S.aux[ar - 1][ac], S.score[cr - 1][cc], unused, features[3]);
+        record(cr + 1, cc, S.aux[ar + 1][ac], S.score[cr + 1][cc], unused, features[4]);
+      }
+      auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
+      float candidates[3][3];
+      rgb_candidates(wrgb_at, weights, candidates);
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+      const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
+      Site site;
+      combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
+                        features, record_count, fallback, P.combine, site.cand);
+      site.orig[0] = original[0];
+      site.orig[1] = original[1];
+      site.orig[2] = original[2];
+      site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
+      J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
+    }
+  });
 }
 
 // Exclusive scan of the (row, tile-column) segment counts in row-major
@@ -824,55 +872,72 @@ __global__ __launch_bounds__(kThreads) void k_sites(JobDev J, DevParams P) {
 // (and writing the total), and the per-block rescan with its offset.
 constexpr int kSegPerBlock = 1024;
 
-__global__ __launch_bounds__(kThreads) void k_seg_partials(JobDev J) {
+__global__ __launch_bounds__(kThreads) void k_seg_partials(JobSet set) {
   __shared__ uint32_t wave_sums[kThreads / 64];
-  const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
-  const long base = (long)blockIdx.x * kSegPerBlock;
-  uint32_t sum = 0;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+    const long base = (long)unit * kSegPerBlock;
+    uint32_t sum = 0;
 #pragma unroll
-  for (int j = 0; j < kSegPerBlock / kThreads; ++j) {
-    const long i = base + threadIdx.x + j * kThreads;
-    if (i < n) sum += J.seg_count[i];
-  }
-  uint32_t total;
-  block_exclusive_scan(sum, wave_sums, &total);
-  if (threadIdx.x == 0) J.seg_block_sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kThreads) void k_seg_top(JobDev J, int blocks) {
-  __shared__ uint32_t wave_sums[kThreads / 64];
-  uint32_t carry = 0;
-  for (int base = 0; base < blocks; base += kThreads) {
-    const int i = base + threadIdx.x;
-    const uint32_t value = i < blocks ? J.seg_block_sums[i] : 0u;
+    for (int j = 0; j < kSegPerBlock / kThreads; ++j) {
+      const long i = base + threadIdx.x + j * kThreads;
+      if (i < n) sum += J.seg_count[i];
+    }
     uint32_t total;
-    const uint32_t offset = block_exclusive_scan(value, wave_sums, &total);
-    if (i < blocks) J.seg_block_sums[i] = carry + offset;
-    carry += total;
-  }
-  if (threadIdx.x == 0) J.counters[0] = carry;
+    block_exclusive_scan(sum, wave_sums, &total);
+    if (threadIdx.x == 0) J.seg_block_sums[unit] = total;
+  });
 }
 
-__global__ __launch_bounds__(kThreads) void k_seg_offsets(JobDev J) {
+__global__ __launch_bounds__(kThreads) void k_seg_top(JobSet set) {
   __shared__ uint32_t wave_sums[kThreads / 64];
-  const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
-  constexpr int kPer = kSegPerBlock / kThreads;
-  // each thread owns kPer consecutive segments so the rescan stays ordered
-  const long first = (long)blockIdx.x * kSegPerBlock + (long)threadIdx.x * kPer;
-  uint32_t values[kPer];
-  uint32_t sum = 0;
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+    const int blocks = (int)((segments + kSegPerBlock - 1) / kSegPerBlock);
+    uint32_t carry = 0;
+    for (int base = 0; base < blocks; base += kThreads) {
+      const int i = base + threadIdx.x;
+      const uint32_t value = i < blocks ? J.seg_block_sums[i] : 0u;
+      uint32_t total;
+      const uint32_t offset = block_exclusive_scan(value, wave_sums, &total);
+      if (i < blocks) J.seg_block_sums[i] = carry + offset;
+      carry += total;
+    }
+    if (threadIdx.x == 0) {
+      J.counters[0] = carry;
+      J.row_start[J.height] = carry;
+    }
+  });
+}
+
+__global__ __launch_bounds__(kThreads) void k_seg_offsets(JobSet set) {
+  __shared__ uint32_t wave_sums[kThreads / 64];
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+    constexpr int kPer = kSegPerBlock / kThreads;
+    // each thread owns kPer consecutive segments so the rescan stays ordered
+    const long first = (long)unit * kSegPerBlock + (long)threadIdx.x * kPer;
+    uint32_t values[kPer];
+    uint32_t sum = 0;
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    values[j] = first + j < n ? J.seg_count[first + j] : 0u;
-    sum += values[j];
-  }
-  uint32_t total;
-  uint32_t offset = J.seg_block_sums[blockIdx.x] + block_exclusive_scan(sum, wave_sums, &total);
+    for (int j = 0; j < kPer; ++j) {
+      values[j] = first + j < n ? J.seg_count[first + j] : 0u;
+      sum += values[j];
+    }
+    uint32_t total;
+    uint32_t offset = J.seg_block_sums[unit] + block_exclusive_scan(sum, wave_sums, &total);
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    if (first + j < n) J.seg_offset[first + j] = offset;
-    offset += values[j];
-  }
+    for (int j = 0; j < kPer; ++j) {
+      const long i = first + j;
+      if (i < n) {
+        J.seg_offset[i] = offset;
+        // first site of row y: its tile-column-0 segment
+        const long y = i / J.tiles_x;
+        if (i % J.tiles_x == 0 && y < J.height) J.row_start[y] = offset;
+      }
+      offset += values[j];
+    }
+  });
 }
 
 // ==========================================================================
@@ -881,36 +946,39 @@ __global__ __launch_bounds__(kThreads) void k_seg_offsets(JobDev J) {
 
 // Planes of the first rows (5 in cross-neighbour mode) for the six hidden
 // writer stages.
-__global__ __launch_bounds__(kThreads) void k_startup_rows(JobDev J, DevParams P,
-                                                           int rows) {
-  const int W = J.width;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * W) return;
-  const int y = i / W, x = i % W;
-  const float alpha = J.stages.alpha[y];
-  const bool is_one = alpha == 1.0f;
-  float aux3[3];
+__global__ __launch_bounds__(kThreads) void k_startup_rows(JobSet set) {
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const DevParams& P = J.P;
+    const int rows = J.height < 5 ? J.height : 5;
+    const int W = J.width;
+    const int i = unit * blockDim.x + threadIdx.x;
+    if (i >= rows * W) return;
+    const int y = i / W, x = i % W;
+    const float alpha = J.stages.alpha[y];
+    const bool is_one = alpha == 1.0f;
+    float aux3[3];
 #pragma unroll
-  for (int d = -1; d <= 1; ++d) {
-    const int xx = clampi(x + d, 0, W - 1);
+    for (int d = -1; d <= 1; ++d) {
+      const int xx = clampi(x + d, 0, W - 1);
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)y * W + xx, lanes);
+      const float visible = J.lut[lanes[P.visible_channel]];
+      aux3[d + 1] = is_one ? J.stages.replacement[y]
+                           : auxiliary(visible, J.lut[lanes[3]], alpha, J.stages.offset[y]);
+    }
+    float sample = aux3[1];
+    if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
+      sample = fmin_plain(fmin_plain(aux3[0], aux3[1]), aux3[2]);
+    const float s = score_value(sample, J.stages.score_base[y + 1], P.score_addend,
+                                P.score_scale, P.score_offset, P.score_floor);
     uint32_t lanes[4];
-    load_pixel(J.rgbi, (size_t)y * W + xx, lanes);
-    const float visible = J.lut[lanes[P.visible_channel]];
-    aux3[d + 1] = is_one ? J.stages.replacement[y]
-                         : auxiliary(visible, J.lut[lanes[3]], alpha, J.stages.offset[y]);
-  }
-  float sample = aux3[1];
-  if (P.horizontal_minimum && W > 2 && x >= 1 && x <= W - 2)
-    sample = fmin_plain(fmin_plain(aux3[0], aux3[1]), aux3[2]);
-  const float s = score_value(sample, J.stages.score_base[y + 1], P.score_addend,
-                              P.score_scale, P.score_offset, P.score_floor);
-  uint32_t lanes[4];
-  load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-  J.su_aux[i] = aux3[1];
-  J.su_score[i] = s;
-  J.su_work[(size_t)i * 3 + 0] = J.lut[lanes[0]];
-  J.su_work[(size_t)i * 3 + 1] = J.lut[lanes[1]];
-  J.su_work[(size_t)i * 3 + 2] = J.lut[lanes[2]];
+    load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+    J.su_aux[i] = aux3[1];
+    J.su_score[i] = s;
+    J.su_work[(size_t)i * 3 + 0] = J.lut[lanes[0]];
+    J.su_work[(size_t)i * 3 + 1] = J.lut[lanes[1]];
+    J.su_work[(size_t)i * 3 + 2] = J.lut[lanes[2]];
+  });
 }
 
 namespace {
@@ -958,87 +1026,113 @@ struct StartupView {
 };
 }  // namespace
 
-__global__ __launch_bounds__(kThreads) void k_startup_sites(JobDev J, DevParams P) {
-  const int W = J.width;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 6 * W) return;
-  const int stage = i / W + 1, x = i % W;
-  const int center_y = stage - 7;
-  const bool cross = P.cross_neighbor != 0;
-  const int writer_width = (cross && stage <= 2) ? W - 4 : W;
-  J.su_attempted[i] = 0;
-  if (x >= writer_width) return;
-  StartupView V{J, W, P.score_floor};
-  const StageDev& st = J.stages;
-  const bool floor_enabled = P.gate_primary || st.gate_secondary[stage - 1];
-  const int32_t row_gate = st.row_gate[stage - 1];
-  const float combiner_reference = st.coarse[stage - 1];
-  const float fallback_center = st.coarse[max(1, center_y + 6) - 1];
-  const float fallback_up = st.coarse[max(1, center_y + 5) - 1];
-  const float fallback_down = st.coarse[max(1, center_y + 7) - 1];
-  auto aux_at = [&](int dy, int dx) { return V.raw(center_y + dy, x + dx); };
-  const bool fallback = decision_fallback(aux_at, P.decision_threshold, P.radius,
-                                          P.count_limit);
-  float weights[4], features[5][4];
-  V.record(center_y, x, fallback_center, weights, features[0]);
-  if (fallback || row_gate != 0 || (floor_enabled && weights[3] >= 1.0f)) return;
-  int record_count = 1;
-  if (cross) {
-    record_count = 5;
-    float unused[4];
-    V.record(center_y, x - 1, fallback_center, unused, features[1]);
-    V.record(center_y, x + 1, fallback_center, unused, features[2]);
-    V.record(center_y - 1, x, fallback_up, unused, features[3]);
-    V.record(center_y + 1, x, fallback_down, unused, features[4]);
-  }
-  auto wrgb_at = [&](int dy, int dx, int k) {
-    return V.wrgb(center_y - 4 + dy, x - 4 + dx, k);
-  };
-  float candidates[3][3];
-  rgb_candidates(wrgb_at, weights, candidates);
-  const float original[3] = {J.su_work[x * 3 + 0], J.su_work[x * 3 + 1],
-                             J.su_work[x * 3 + 2]};
-  Site site;
-  combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
-                    features, record_count, combiner_reference, P.combine, site.cand);
-  site.orig[0] = original[0];
-  site.orig[1] = original[1];
-  site.orig[2] = original[2];
-  site.pixel = (uint32_t)x | (floor_enabled ? kSiteFloorBit : 0u);
-  J.su_sites[i] = site;
-  J.su_attempted[i] = 1;
+__global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const DevParams& P = J.P;
+    const int W = J.width;
+    const int i = unit * blockDim.x + threadIdx.x;
+    if (i >= 6 * W) return;
+    const int stage = i / W + 1, x = i % W;
+    const int center_y = stage - 7;
+    const bool cross = P.cross_neighbor != 0;
+    const int writer_width = (cross && stage <= 2) ? W - 4 : W;
+    J.su_attempted[i] = 0;
+    if (x >= writer_width) return;
+    StartupView V{J, W, P.score_floor};
+    const StageDev& st = J.stages;
+    const bool floor_enabled = P.gate_primary || st.gate_secondary[stage - 1];
+    const int32_t row_gate = st.row_gate[stage - 1];
+    const float combiner_reference = st.coarse[stage - 1];
+    const float fallback_center = st.coarse[max(1, center_y + 6) - 1];
+    const float fallback_up = st.coarse[max(1, center_y + 5) - 1];
+    const float fallback_down = st.coarse[max(1, center_y + 7) - 1];
+    auto aux_at = [&](int dy, int dx) { return V.raw(center_y + dy, x + dx); };
+    const bool fallback = decision_fallback(aux_at, P.decision_threshold, P.radius,
+                                            P.count_limit);
+    float weights[4], features[5][4];
+    V.record(center_y, x, fallback_center, weights, features[0]);
+    if (fallback || row_gate != 0 || (floor_enabled && weights[3] >= 1.0f)) return;
+    int record_count = 1;
+    if (cross) {
+      record_count = 5;
+      float unused[4];
+      V.record(center_y, x - 1, fallback_center, unused, features[1]);
+      V.record(center_y, x + 1, fallback_center, unused, features[2]);
+      V.record(center_y - 1, x, fallback_up, unused, features[3]);
+      V.record(center_y + 1, x, fallback_down, unused, features[4]);
+    }
+    auto wrgb_at = [&](int dy, int dx, int k) {
+      return V.wrgb(center_y - 4 + dy, x - 4 + dx, k);
+    };
+    float candidates[3][3];
+    rgb_candidates(wrgb_at, weights, candidates);
+    const float original[3] = {J.su_work[x * 3 + 0], J.su_work[x * 3 + 1],
+                               J.su_work[x * 3 + 2]};
+    Site site;
+    combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
+                      features, record_count, combiner_reference, P.combine, site.cand);
+    site.orig[0] = original[0];
+    site.orig[1] = original[1];
+    site.orig[2] = original[2];
+    site.pixel = (uint32_t)x | (floor_enabled ? kSiteFloorBit : 0u);
+    J.su_sites[i] = site;
+    J.su_attempted[i] = 1;
+  });
 }
 
 // ==========================================================================
 // Writer support (reconstruction.py:916-982, dither.py:59-104)
 // ==========================================================================
 
-// Integer draw schedule of every site (writer_record).
-__global__ __launch_bounds__(kThreads) void k_writer_prep(const Site* sites,
-                                                          const uint8_t* attempted,
-                                                          WriterRecord* records,
-                                                          uint32_t count, WriterConsts K) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const bool is_attempted = attempted ? attempted[i] != 0 : true;
-  records[i] = writer_record(sites[i], is_attempted, K);
+// Per-job flags and counters to zero (one unit per job).  A kernel, not a
+// memset, so that it can sit inside a captured graph.
+__global__ __launch_bounds__(kThreads) void k_zero_jobs(JobSet set) {
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    for (int i = threadIdx.x; i < kCountSlots * kCountStride; i += blockDim.x)
+      J.writer_counts[i] = 0ull;
+    if (threadIdx.x < 16) J.counters[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *J.error_flags = 0u;
+  });
 }
 
-// The writer itself, every site in parallel from the LCG state the serial
-// integer chain recorded at its start: values, written / changed counts,
-// the patch of the output, and a check that each site ends in the state
-// the chain recorded for the next one.
-__global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, WriterConsts K,
-                                                            uint32_t count,
-                                                            uint32_t final_state) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Integer draw schedule (writer_record) of every main site and every
+// hidden-startup slot of every job; the main site count is read on the
+// device, so the launch geometry does not depend on it.
+__global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = 0; j < set.count; ++j) {
+    const JobDev J = set.jobs[j];
+    if (J.skip) continue;
+    const uint32_t count = J.counters[0];
+    const uint32_t total = count + 6u * (uint32_t)J.width;
+    for (uint32_t i = first; i < total; i += stride) {
+      if (i < count) {
+        J.writer_records[i] = writer_record(J.sites[i], true, J.K);
+      } else {
+        const uint32_t k = i - count;
+        J.su_writer_records[k] = writer_record(J.su_sites[k], J.su_attempted[k] != 0, J.K);
+      }
+    }
+  }
+}
+
+// The writer itself over sites [begin, end) of one job, every site in
+// parallel from the LCG state the serial integer chain recorded at its
+// start: values, written / changed counts, the patch of the output, and a
+// check that each site ends in the state the chain recorded for the next
+// one (`end_state` after the last site of the range).
+__global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, uint32_t begin,
+                                                            uint32_t end,
+                                                            uint32_t end_state) {
+  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   bool written = false, changed = false;
-  if (i < count) {
+  if (i < end) {
     const Site site = J.sites[i];
     uint32_t state = J.site_state[i];
     float values[3];
-    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, K, state, values);
-    const uint32_t expected = i + 1 < count ? J.site_state[i + 1] : final_state;
+    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, J.K, state, values);
+    const uint32_t expected = i + 1 < end ? J.site_state[i + 1] : end_state;
     uint32_t error = state != expected ? kErrWriterChain : 0u;
     written = values[0] != site.orig[0] || values[1] != site.orig[1] ||
               values[2] != site.orig[2];
@@ -1084,16 +1178,16 @@ __global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, WriterCons
 // The same check over the six hidden startup rows, which write no output:
 // each attempted site, run from its recorded start state, must end in the
 // state recorded for the next index (a skipped index passes the state on).
-__global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J, WriterConsts K,
-                                                                   uint32_t count,
+__global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J,
                                                                    uint32_t final_state) {
+  const uint32_t count = 6u * (uint32_t)J.width;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   uint32_t state = J.su_site_state[i];
   if (J.su_attempted[i]) {
     const Site site = J.su_sites[i];
     float values[3];
-    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, K, state, values);
+    write_pixel(site.cand, site.orig, (site.pixel & kSiteFloorBit) != 0, J.K, state, values);
   }
   const uint32_t expected = i + 1 < count ? J.su_site_state[i + 1] : final_state;
   if (state != expected) atomicOr(J.error_flags, kErrWriterChain);
@@ -1104,22 +1198,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_u8(uint8_t* p, size_t n, uint
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-__global__ __launch_bounds__(kThreads) void k_zero_u64(unsigned long long* p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
-}
 
-// Writes the host writer's changed values into the device output.
-__global__ __launch_bounds__(kThreads) void k_apply_patches(uint16_t* out,
-                                                            const uint32_t* pixels,
-                                                            const uint16_t* values,
-                                                            uint32_t count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint16_t* o = out + (size_t)pixels[i] * 3;
-  o[0] = values[i * 3 + 0];
-  o[1] = values[i * 3 + 1];
-  o[2] = values[i * 3 + 2];
-}
 
 // ==========================================================================
 // Level-1 test kernels: the device functions above, over host vectors.
@@ -1251,14 +1330,82 @@ static inline unsigned blocks_for(long n, int threads) {
   return (unsigned)((n + threads - 1) / threads);
 }
 
-hipError_t launch_producer(const JobDev& J, float factor_b, hipStream_t stream) {
-  const int blocks = (J.height + 7) / 8;
-  hipLaunchKernelGGL(k_producer_blocks, dim3(blocks), dim3(kProducerThreads), 0, stream, J);
-  hipLaunchKernelGGL(k_row_sums, dim3(blocks_for(J.height, kRowSumRows)),
-                     dim3(64 + kRowSumStagers), 0, stream, J);
-  hipLaunchKernelGGL(k_producer_chain, dim3(2), dim3(64), 0, stream, J);
-  hipLaunchKernelGGL(k_stage_table, dim3(blocks_for(J.height + 6, kThreads)),
-                     dim3(kThreads), 0, stream, J, factor_b);
+namespace {
+// Workgroups that stay resident on the whole device for one kernel: the
+// grid of a persistent launch (never more than the work needs).
+unsigned resident_blocks(const void* kernel, int threads) {
+  static std::mutex mutex;
+  static std::map<std::pair<int, const void*>, unsigned> cache;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  std::lock_guard<std::mutex> lock(mutex);
+  auto key = std::make_pair(device, kernel);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1 || cus < 1) {
+    per_cu = 1;
+    cus = 256;
+  }
+  const unsigned blocks = (unsigned)(per_cu * cus);
+  cache[key] = blocks;
+  return blocks;
+}
+
+template <class Units>
+JobSet make_set(const JobDev* d_jobs, const JobDev* h_jobs, int n, Units&& units) {
+  JobSet set;
+  set.jobs = d_jobs;
+  set.count = n;
+  set.base[0] = 0;
+  for (int j = 0; j < n; ++j) {
+    const uint32_t u = h_jobs[j].skip ? 0u : (uint32_t)units(h_jobs[j]);
+    set.base[j + 1] = set.base[j] + u;
+  }
+  for (int j = n + 1; j <= kMaxBatch; ++j) set.base[j] = set.base[n];
+  return set;
+}
+
+template <class Kernel, class... Args>
+void launch_persistent(Kernel kernel, const JobSet& set, int threads, hipStream_t stream,
+                       Args... args) {
+  const uint32_t total = set.base[set.count];
+  if (total == 0) return;
+  const unsigned grid =
+      std::min<unsigned>(total, resident_blocks(reinterpret_cast<const void*>(kernel), threads));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, stream, set, args...);
+}
+
+uint32_t segment_blocks(const JobDev& J) {
+  const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+  return (uint32_t)((segments + kSegPerBlock - 1) / kSegPerBlock);
+}
+}  // namespace
+
+hipError_t launch_zero_jobs(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                            hipStream_t stream) {
+  const JobSet set = make_set(d_jobs, h_jobs, n, [](const JobDev&) { return 1; });
+  launch_persistent(k_zero_jobs, set, kThreads, stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_producer(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                           hipStream_t stream) {
+  auto when = [](const JobDev& J, long units) { return J.run_producer ? units : 0L; };
+  launch_persistent(k_producer_blocks, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
+                      return when(J, (J.height + 7) / 8);
+                    }), kProducerThreads, stream);
+  launch_persistent(k_row_sums, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
+                      return when(J, blocks_for(J.height, kRowSumRows));
+                    }), 64 + kRowSumStagers, stream);
+  launch_persistent(k_producer_chain, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
+                      return when(J, 2);
+                    }), 64, stream);
+  launch_persistent(k_stage_table, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
+                      return when(J, blocks_for(J.height + 6, kThreads));
+                    }), kThreads, stream);
   return hipGetLastError();
 }
 
@@ -1269,30 +1416,20 @@ hipError_t launch_schedule_rows(const JobDev& J, const ScheduleDev& S,
   return hipGetLastError();
 }
 
-template <int TY, int TX>
-static hipError_t launch_scan_shape(const JobDev& J, const DevParams& P, hipStream_t stream) {
-  const int tiles = J.tiles_x * J.tiles_y;
-  hipLaunchKernelGGL((k_scan<TY, TX>), dim3(tiles), dim3(kThreads), 0, stream, J, P);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_segments(const JobDev& J, hipStream_t stream) {
-  const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
-  const int seg_blocks = (int)((segments + kSegPerBlock - 1) / kSegPerBlock);
-  hipLaunchKernelGGL(k_seg_partials, dim3(seg_blocks), dim3(kThreads), 0, stream, J);
-  hipLaunchKernelGGL(k_seg_top, dim3(1), dim3(kThreads), 0, stream, J, seg_blocks);
-  hipLaunchKernelGGL(k_seg_offsets, dim3(seg_blocks), dim3(kThreads), 0, stream, J);
-  return hipGetLastError();
-}
-
-template <int TY, int TX>
-static hipError_t launch_sites_shape(const JobDev& J, const DevParams& P, hipStream_t stream) {
-  const int tiles = J.tiles_x * J.tiles_y;
-  hipLaunchKernelGGL((k_sites<TY, TX>), dim3(tiles), dim3(kThreads), 0, stream, J, P);
-  return hipGetLastError();
-}
-
 #define DICE_TILE_SHAPES(X) X(16, 64) X(8, 128) X(32, 32) X(16, 32) X(32, 64) X(8, 64)
+
+int tile_shapes(int (*out)[2], int max) {
+  int n = 0;
+#define DICE_LIST(a, b)   \
+  if (n < max) {          \
+    out[n][0] = a;        \
+    out[n][1] = b;        \
+  }                       \
+  ++n;
+  DICE_TILE_SHAPES(DICE_LIST)
+#undef DICE_LIST
+  return n;
+}
 
 bool tile_shape_supported(int ty, int tx) {
 #define DICE_MATCH(a, b) if (ty == a && tx == b) return true;
@@ -1301,63 +1438,120 @@ bool tile_shape_supported(int ty, int tx) {
   return false;
 }
 
-hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream) {
-  const int ty = J.tile_rows, tx = J.tile_cols;
-#define DICE_MATCH(a, b) if (ty == a && tx == b) return launch_scan_shape<a, b>(J, P, stream);
+namespace {
+template <int TY, int TX>
+void launch_scan_shape(const JobSet& set, hipStream_t stream) {
+  launch_persistent(k_scan<TY, TX>, set, kThreads, stream);
+}
+template <int TY, int TX>
+void launch_sites_shape(const JobSet& set, hipStream_t stream) {
+  launch_persistent(k_sites<TY, TX>, set, kThreads, stream);
+}
+}  // namespace
+
+namespace {
+// The tile shape every live job of the batch shares (false: none is live,
+// or two live jobs disagree).
+bool batch_tile(const JobDev* h_jobs, int n, int* ty, int* tx, bool* any) {
+  *any = false;
+  for (int j = 0; j < n; ++j) {
+    if (h_jobs[j].skip) continue;
+    if (!*any) {
+      *ty = h_jobs[j].tile_rows;
+      *tx = h_jobs[j].tile_cols;
+      *any = true;
+    } else if (h_jobs[j].tile_rows != *ty || h_jobs[j].tile_cols != *tx) {
+      return false;
+    }
+  }
+  return true;
+}
+}  // namespace
+
+hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream) {
+  int ty = 0, tx = 0;
+  bool any = false;
+  if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
+  if (!any) return hipSuccess;
+  const JobSet set = make_set(d_jobs, h_jobs, n,
+                              [](const JobDev& J) { return J.tiles_x * J.tiles_y; });
+#define DICE_MATCH(a, b)                 \
+  if (ty == a && tx == b) {              \
+    launch_scan_shape<a, b>(set, stream); \
+    return hipGetLastError();            \
+  }
   DICE_TILE_SHAPES(DICE_MATCH)
 #undef DICE_MATCH
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream) {
-  const int ty = J.tile_rows, tx = J.tile_cols;
-#define DICE_MATCH(a, b) if (ty == a && tx == b) return launch_sites_shape<a, b>(J, P, stream);
+hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                                hipStream_t stream) {
+  const JobSet blocks = make_set(d_jobs, h_jobs, n, segment_blocks);
+  launch_persistent(k_seg_partials, blocks, kThreads, stream);
+  launch_persistent(k_seg_top, make_set(d_jobs, h_jobs, n, [](const JobDev&) { return 1; }),
+                    kThreads, stream);
+  launch_persistent(k_seg_offsets, blocks, kThreads, stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream) {
+  int ty = 0, tx = 0;
+  bool any = false;
+  if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
+  if (!any) return hipSuccess;
+  const JobSet set = make_set(d_jobs, h_jobs, n,
+                              [](const JobDev& J) { return J.tiles_x * J.tiles_y; });
+#define DICE_MATCH(a, b)                  \
+  if (ty == a && tx == b) {               \
+    launch_sites_shape<a, b>(set, stream); \
+    return hipGetLastError();             \
+  }
   DICE_TILE_SHAPES(DICE_MATCH)
 #undef DICE_MATCH
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream) {
-  const int rows = J.height < 5 ? J.height : 5;
-  hipLaunchKernelGGL(k_startup_rows, dim3(blocks_for((long)rows * J.width, kThreads)),
-                     dim3(kThreads), 0, stream, J, P, rows);
-  hipLaunchKernelGGL(k_startup_sites, dim3(blocks_for(6L * J.width, kThreads)),
-                     dim3(kThreads), 0, stream, J, P);
+hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                          hipStream_t stream) {
+  launch_persistent(k_startup_rows, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
+                      return blocks_for((long)(J.height < 5 ? J.height : 5) * J.width, kThreads);
+                    }), kThreads, stream);
+  launch_persistent(k_startup_sites, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
+                      return blocks_for(6L * J.width, kThreads);
+                    }), kThreads, stream);
   return hipGetLastError();
 }
 
-hipError_t launch_writer_prep(const Site* sites, const uint8_t* attempted,
-                              WriterRecord* records, uint32_t count, const WriterConsts& K,
-                              hipStream_t stream) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_writer_prep, dim3(blocks_for(count, kThreads)), dim3(kThreads), 0,
-                     stream, sites, attempted, records, count, K);
+hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream) {
+  JobSet set;
+  set.jobs = d_jobs;
+  set.count = n;
+  for (int j = 0; j <= kMaxBatch; ++j) set.base[j] = 0;
+  const unsigned grid = resident_blocks(reinterpret_cast<const void*>(k_writer_prep), kThreads);
+  hipLaunchKernelGGL(k_writer_prep, dim3(grid), dim3(kThreads), 0, stream, set);
   return hipGetLastError();
 }
 
-hipError_t launch_writer_values(const JobDev& J, const WriterConsts& K, uint32_t count,
-                                uint32_t startup_final_state, uint32_t final_state,
-                                hipStream_t stream) {
-  const uint32_t startup_count = 6u * (uint32_t)J.width;
-  hipLaunchKernelGGL(k_writer_startup_check, dim3(blocks_for(startup_count, kThreads)),
-                     dim3(kThreads), 0, stream, J, K, startup_count, startup_final_state);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kThreads), 0, stream, J.writer_counts,
-                     kCountSlots * kCountStride);
-  if (J.diag_changed)
-    hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(kThreads), 0, stream, J.diag_changed,
-                       (size_t)J.height * J.width, (uint8_t)0);
-  if (count)
-    hipLaunchKernelGGL(k_writer_values, dim3(blocks_for(count, kThreads)), dim3(kThreads), 0,
-                       stream, J, K, count, final_state);
+hipError_t launch_writer_values(const JobDev& J, uint32_t begin, uint32_t end,
+                                uint32_t end_state, hipStream_t stream) {
+  if (end <= begin) return hipSuccess;
+  hipLaunchKernelGGL(k_writer_values, dim3(blocks_for(end - begin, kThreads)), dim3(kThreads),
+                     0, stream, J, begin, end, end_state);
   return hipGetLastError();
 }
 
-hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
-                                const uint16_t* values, uint32_t count,
-                                hipStream_t stream) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_apply_patches, dim3(blocks_for(count, kThreads)), dim3(kThreads),
-                     0, stream, out, pixels, values, count);
+hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
+                                       hipStream_t stream) {
+  hipLaunchKernelGGL(k_writer_startup_check, dim3(blocks_for(6L * J.width, kThreads)),
+                     dim3(kThreads), 0, stream, J, final_state);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u8, dim3(std::min<size_t>(blocks_for((long)n, kThreads), 4096)),
+                     dim3(kThreads), 0, stream, p, n, value);
   return hipGetLastError();
 }
 

@@ -24,23 +24,31 @@ struct ScheduleDev {
   float* scale_add_numerator;
 };
 
-hipError_t launch_producer(const JobDev& J, float factor_b, hipStream_t stream);
+// Batched launches: d_jobs is the device array of n descriptors, h_jobs its
+// host copy (launch geometry only).  Every job of one launch uses the same
+// analysis tile shape.
+hipError_t launch_zero_jobs(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                            hipStream_t stream);
+hipError_t launch_producer(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                           hipStream_t stream);
 hipError_t launch_schedule_rows(const JobDev& J, const ScheduleDev& S,
                                 hipStream_t stream);
 bool tile_shape_supported(int ty, int tx);
-hipError_t launch_scan(const JobDev& J, const DevParams& P, hipStream_t stream);
-hipError_t launch_scan_segments(const JobDev& J, hipStream_t stream);
-hipError_t launch_sites(const JobDev& J, const DevParams& P, hipStream_t stream);
-hipError_t launch_startup(const JobDev& J, const DevParams& P, hipStream_t stream);
-hipError_t launch_writer_prep(const Site* sites, const uint8_t* attempted,
-                              WriterRecord* records, uint32_t count, const WriterConsts& K,
-                              hipStream_t stream);
-hipError_t launch_writer_values(const JobDev& J, const WriterConsts& K, uint32_t count,
-                                uint32_t startup_final_state, uint32_t final_state,
+// The compiled analysis tile shapes {rows, cols}; returns how many exist.
+int tile_shapes(int (*out)[2], int max);
+hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream);
+hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                 hipStream_t stream);
-hipError_t launch_apply_patches(uint16_t* out, const uint32_t* pixels,
-                                const uint16_t* values, uint32_t count,
-                                hipStream_t stream);
+hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream);
+hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
+                          hipStream_t stream);
+hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream);
+// Per job: the writer over sites [begin, end) and the startup check.
+hipError_t launch_writer_values(const JobDev& J, uint32_t begin, uint32_t end,
+                                uint32_t end_state, hipStream_t stream);
+hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
+                                       hipStream_t stream);
+hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t stream);
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_t_unscaled_averages(const float* patches, long count, double* out,

@@ -1,0 +1,874 @@
+// dice_runtime.cpp -- batch execution of Digital ICE jobs on one MI355X.
+//
+// The reference runs one job at a time (engine.py:110-267,
+// cuda_backend/engine.py:183-590); here a Runner takes 1..kMaxBatch
+// independent jobs and runs every device phase as one persistent launch
+// over all of them, so small crops fill the 256 CUs together.  The strictly
+// serial writer (host_writer.py:73-93) stays one chain per job, on one host
+// core; chains of different jobs run on different cores, and each job's
+// SHA-256 runs beside its chain, band by band.
+#include "dice_runtime.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+#include "dice_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace dice {
+
+namespace {
+constexpr int kMaxBands = 64;
+}  // namespace
+
+DevParams device_params(const dice_params& p) {
+  DevParams P;
+  P.visible_channel = p.visible_channel;
+  P.score_addend = p.score_addend;
+  P.score_scale = p.score_scale;
+  P.score_offset = p.score_offset;
+  P.score_floor = p.score_floor;
+  P.horizontal_minimum = p.horizontal_minimum;
+  P.decision_threshold = p.decision_threshold;
+  P.count_limit = p.decision_count_limit;
+  P.radius = p.decision_radius;
+  P.cross_neighbor = p.cross_neighbor;
+  P.gate_primary = p.gate_primary;
+  P.combine.coarse_enabled = p.coarse_enabled;
+  for (int i = 0; i < 3; ++i) {
+    P.combine.coarse_slopes[i] = p.coarse_slopes[i];
+    P.combine.band_enabled[i] = p.band_enabled[i];
+    P.combine.band_scales[i] = p.band_scales[i];
+    P.combine.configured_strengths[i] = p.configured_strengths[i];
+  }
+  for (int i = 0; i < 9; ++i) {
+    P.combine.factors_a[i] = p.factors_a[i];
+    P.combine.factors_b[i] = p.factors_b[i];
+  }
+  return P;
+}
+
+WriterConsts writer_consts(const dice_params& p) {
+  WriterConsts K;
+  K.low = (double)p.dither_low;
+  K.high = (double)p.dither_high;
+  K.low_lt_high = K.low < K.high;
+  K.scales[0] = p.dither_scales[0];
+  K.scales[1] = p.dither_scales[1];
+  K.scales[2] = p.dither_scales[2];
+  return K;
+}
+
+namespace {
+
+int32_t flag_error(uint32_t flags) {
+  if (flags & kErrNonfiniteAux)
+    return set_error(DICE_NONFINITE_AUX, "auxiliary score input must be a finite HxW plane");
+  if (flags & kErrSetterScaleOne) return set_error(DICE_INVALID, "X3A setter scale cannot equal one");
+  if (flags & kErrStageNonfinite)
+    return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
+  if (flags & kErrNonfiniteWork) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
+  if (flags & kErrWriterChain)
+    return set_error(DICE_INTERNAL, "writer state chain disagrees with the per-site writer");
+  return DICE_OK;
+}
+
+// Band b of H rows in `bands` equal parts.
+inline int64_t band_row(int64_t H, int bands, int b) { return H * b / bands; }
+
+}  // namespace
+
+int32_t DeviceTables::upload() {
+  DICE_TRY(require_tables());
+  const Tables& t = tables();
+  if (noop.bytes >= 65536 * 2) return DICE_OK;
+  HIP_TRY(lut.ensure(65536 * 4));
+  HIP_TRY(high.ensure(256 * 4));
+  HIP_TRY(low.ensure(256 * 4));
+  HIP_TRY(noop.ensure(65536 * 2));
+  HIP_TRY(hipMemcpy(lut.ptr, t.lut.data(), 65536 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(high.ptr, t.high.data(), 256 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(low.ptr, t.low.data(), 256 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(noop.ptr, t.noop_emit.data(), 65536 * 2, hipMemcpyHostToDevice));
+  return DICE_OK;
+}
+
+// --------------------------------------------------------------------------
+// Workspace: the device and pinned buffers of one job slot, grown on demand
+// and kept for the next batch.
+// --------------------------------------------------------------------------
+
+struct Workspace {
+  DeviceBuffer rgbi, out, site_lists, tile_count, seg_count, seg_offset, seg_block_sums,
+      row_start, counters, flags;
+  DeviceBuffer diag_score, diag_floor, diag_changed;
+  DeviceBuffer su_aux, su_score, su_work, su_sites, su_attempted;
+  DeviceBuffer failpos, row_sums, accepted, epoch_adds, cumulative, epoch_wide, epoch_store;
+  DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse, st_gate,
+      st_row_gate;
+  DeviceBuffer sites, records, su_records, states, writer_counts;
+  PinnedBuffer h_small;      // [0,64) counters, [64,128) flags, [128, ...) writer slots
+  PinnedBuffer h_records;    // 6W startup records, then the main records
+  PinnedBuffer h_row_start;  // H+1
+  PinnedBuffer h_states;     // 6W startup states, then the main states
+  PinnedBuffer h_out;        // output bytes to hash when the output stays on the device
+  hipEvent_t band[kMaxBands] = {};
+  bool events = false;
+  // chain -> hash hand-off
+  std::mutex mutex;
+  std::condition_variable cv;
+  int published = 0;
+  bool aborted = false;
+
+  ~Workspace() {
+    for (hipEvent_t& e : band)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int32_t init_events() {
+    if (events) return DICE_OK;
+    for (hipEvent_t& e : band) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    events = true;
+    return DICE_OK;
+  }
+  uint64_t bytes() const {
+    const DeviceBuffer* all[] = {&rgbi, &out, &site_lists, &tile_count, &seg_count, &seg_offset,
+                                 &seg_block_sums, &row_start, &counters, &flags, &diag_score,
+                                 &diag_floor, &diag_changed, &su_aux, &su_score, &su_work,
+                                 &su_sites, &su_attempted, &failpos, &row_sums, &accepted,
+                                 &epoch_adds, &cumulative, &epoch_wide, &epoch_store,
+                                 &st_alpha, &st_offset, &st_replacement, &st_has,
+                                 &st_score_base, &st_coarse, &st_gate, &st_row_gate, &sites,
+                                 &records, &su_records, &states, &writer_counts};
+    uint64_t total = 0;
+    for (const DeviceBuffer* b : all) total += b->bytes;
+    return total;
+  }
+};
+
+// --------------------------------------------------------------------------
+// Runner
+// --------------------------------------------------------------------------
+
+Runner::Runner(int device, const DeviceTables* tables) : device_(device), tables_(tables) {}
+
+Runner::~Runner() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (Graph* g : {&graph_a_, &graph_b_})
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  for (hipEvent_t e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : fine_)
+    if (e) (void)hipEventDestroy(e);
+  ws_.clear();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int32_t Runner::init() {
+  HIP_TRY(hipSetDevice(device_));
+  HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (hipEvent_t& e : ev_) HIP_TRY(hipEventCreate(&e));
+  for (hipEvent_t& e : fine_) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(d_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
+  HIP_TRY(h_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
+  return DICE_OK;
+}
+
+uint64_t Runner::workspace_bytes() const {
+  uint64_t total = 0;
+  for (const auto& w : ws_) total += w->bytes();
+  return total;
+}
+
+// Sizes every job's workspace, places its input, stage table and output,
+// and uploads the descriptors.
+int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
+  (void)any_host_work;
+  while ((int)ws_.size() < n) ws_.push_back(std::make_unique<Workspace>());
+  JobDev* H_jobs = h_jobs_.as<JobDev>();
+  const int TY = tile_rows_, TX = tile_cols_;
+  for (int j = 0; j < n; ++j) {
+    JobRequest& R = jobs[j];
+    Workspace& w = *ws_[j];
+    DICE_TRY(w.init_events());
+    JobDev& J = H_jobs[j];
+    memset(&J, 0, sizeof J);
+    if (R.code != DICE_OK) {  // failed before the device: no work, no buffers
+      J.skip = 1;
+      continue;
+    }
+    const int64_t H = R.height, W = R.width;
+    J.P = device_params(R.params);
+    J.K = writer_consts(R.params);
+    J.factor_b = R.factor_b;
+    J.run_producer = R.stages ? 0 : 1;
+    J.height = (int32_t)H;
+    J.width = (int32_t)W;
+    J.tile_rows = TY;
+    J.tile_cols = TX;
+    J.tiles_x = (int32_t)((W + TX - 1) / TX);
+    J.tiles_y = (int32_t)((H + TY - 1) / TY);
+    const size_t pixels = (size_t)H * W;
+    const size_t tiles = (size_t)J.tiles_x * J.tiles_y;
+    const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
+    const size_t stages = (size_t)H + 6;
+    const size_t cells = (size_t)W / 8;
+    HIP_TRY(w.site_lists.ensure(tiles * TY * TX * sizeof(uint16_t)));
+    HIP_TRY(w.tile_count.ensure(tiles * 4));
+    HIP_TRY(w.seg_count.ensure(segments * 4));
+    HIP_TRY(w.seg_offset.ensure(segments * 4));
+    HIP_TRY(w.seg_block_sums.ensure((segments / 1024 + 2) * 4));
+    HIP_TRY(w.row_start.ensure((H + 1) * 4));
+    HIP_TRY(w.counters.ensure(64));
+    HIP_TRY(w.flags.ensure(64));
+    HIP_TRY(w.writer_counts.ensure(kCountSlots * kCountStride * 8));
+    HIP_TRY(w.su_aux.ensure(5 * W * 4));
+    HIP_TRY(w.su_score.ensure(5 * W * 4));
+    HIP_TRY(w.su_work.ensure(5 * W * 12));
+    HIP_TRY(w.su_sites.ensure(6 * W * sizeof(Site)));
+    HIP_TRY(w.su_attempted.ensure(6 * W));
+    HIP_TRY(w.failpos.ensure(((H + 7) / 8) * cells + 8));
+    HIP_TRY(w.row_sums.ensure(H * 3 * 8));
+    HIP_TRY(w.accepted.ensure(H * 4));
+    HIP_TRY(w.epoch_adds.ensure((H / 8) * 16 + 16));
+    HIP_TRY(w.cumulative.ensure(H * 24));
+    HIP_TRY(w.epoch_wide.ensure((H / 8) * 16 + 16));
+    HIP_TRY(w.epoch_store.ensure((H / 8) * 8 + 8));
+    HIP_TRY(w.st_alpha.ensure(stages * 4));
+    HIP_TRY(w.st_offset.ensure(stages * 4));
+    HIP_TRY(w.st_replacement.ensure(stages * 4));
+    HIP_TRY(w.st_has.ensure(stages));
+    HIP_TRY(w.st_score_base.ensure(stages * 4));
+    HIP_TRY(w.st_coarse.ensure(stages * 4));
+    HIP_TRY(w.st_gate.ensure(stages));
+    HIP_TRY(w.st_row_gate.ensure(stages * 4));
+    HIP_TRY(w.su_records.ensure(6 * W * sizeof(WriterRecord)));
+    HIP_TRY(w.h_small.ensure(128 + kCountSlots * kCountStride * 8));
+    HIP_TRY(w.h_row_start.ensure((H + 1) * 4));
+    const bool want_scores = R.diag && (R.diag->score_plane || R.diag->at_floor_mask);
+    if (want_scores) {
+      HIP_TRY(w.diag_score.ensure(pixels * 4));
+      HIP_TRY(w.diag_floor.ensure(pixels));
+    }
+    const bool want_changed = R.diag && R.diag->changed_mask;
+    if (want_changed) HIP_TRY(w.diag_changed.ensure(pixels));
+    J.lut = tables_->lut.as<float>();
+    J.noop_emit = tables_->noop.as<uint16_t>();
+    J.factor_high = tables_->high.as<uint32_t>();
+    J.factor_low = tables_->low.as<uint32_t>();
+    J.stages.alpha = w.st_alpha.as<float>();
+    J.stages.offset = w.st_offset.as<float>();
+    J.stages.replacement = w.st_replacement.as<float>();
+    J.stages.has_replacement = w.st_has.as<uint8_t>();
+    J.stages.score_base = w.st_score_base.as<float>();
+    J.stages.coarse = w.st_coarse.as<float>();
+    J.stages.gate_secondary = w.st_gate.as<uint8_t>();
+    J.stages.row_gate = w.st_row_gate.as<int32_t>();
+    J.site_lists = w.site_lists.as<uint16_t>();
+    J.tile_count = w.tile_count.as<uint32_t>();
+    J.seg_count = w.seg_count.as<uint32_t>();
+    J.seg_offset = w.seg_offset.as<uint32_t>();
+    J.seg_block_sums = w.seg_block_sums.as<uint32_t>();
+    J.row_start = w.row_start.as<uint32_t>();
+    J.counters = w.counters.as<uint32_t>();
+    J.error_flags = w.flags.as<uint32_t>();
+    J.writer_counts = w.writer_counts.as<unsigned long long>();
+    J.diag_score = want_scores ? w.diag_score.as<float>() : nullptr;
+    J.diag_floor = want_scores ? w.diag_floor.as<uint8_t>() : nullptr;
+    J.diag_changed = want_changed ? w.diag_changed.as<uint8_t>() : nullptr;
+    J.su_aux = w.su_aux.as<float>();
+    J.su_score = w.su_score.as<float>();
+    J.su_work = w.su_work.as<float>();
+    J.su_sites = w.su_sites.as<Site>();
+    J.su_attempted = w.su_attempted.as<uint8_t>();
+    J.su_writer_records = w.su_records.as<WriterRecord>();
+    J.failpos = w.failpos.as<uint8_t>();
+    J.row_sums = w.row_sums.as<double>();
+    J.accepted = w.accepted.as<uint32_t>();
+    J.epoch_adds = w.epoch_adds.as<double>();
+    J.cumulative = w.cumulative.as<double>();
+    J.epoch_wide = w.epoch_wide.as<double>();
+    J.epoch_store = w.epoch_store.as<float>();
+    // input and output
+    if (R.flags & DICE_MAIN_ON_DEVICE) {
+      J.rgbi = R.main;
+    } else {
+      HIP_TRY(w.rgbi.ensure(pixels * 8));
+      HIP_TRY(hipMemcpyAsync(w.rgbi.ptr, R.main, pixels * 8, hipMemcpyHostToDevice, stream_));
+      J.rgbi = w.rgbi.as<uint16_t>();
+    }
+    if (R.flags & DICE_OUT_ON_DEVICE) {
+      J.out = R.out;
+    } else {
+      HIP_TRY(w.out.ensure(pixels * 6));
+      J.out = w.out.as<uint16_t>();
+    }
+    if (R.stages) {
+      const dice_stage_table* st = R.stages;
+      const size_t c = (size_t)H + 6;
+      HIP_TRY(hipMemcpyAsync(J.stages.alpha, st->aux_alpha, c * 4, hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.offset, st->aux_offset, c * 4, hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.replacement, st->aux_replacement, c * 4,
+                             hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.has_replacement, st->aux_has_replacement, c,
+                             hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.score_base, st->score_base, c * 4, hipMemcpyHostToDevice,
+                             stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.coarse, st->coarse_reference, c * 4,
+                             hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.gate_secondary, st->gate_secondary, c,
+                             hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(J.stages.row_gate, st->row_gate, c * 4, hipMemcpyHostToDevice,
+                             stream_));
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
+                         stream_));
+  return DICE_OK;
+}
+
+int32_t Runner::mark(int slot) {
+  if (fine_on_) HIP_TRY(hipEventRecord(fine_[slot], stream_));
+  return DICE_OK;
+}
+
+int32_t Runner::launch_a(int n) {
+  const JobDev* d = d_jobs_.as<JobDev>();
+  const JobDev* h = h_jobs_.as<JobDev>();
+  HIP_TRY(launch_zero_jobs(d, h, n, stream_));
+  DICE_TRY(mark(0));
+  HIP_TRY(launch_producer(d, h, n, stream_));
+  DICE_TRY(mark(1));
+  HIP_TRY(launch_scan(d, h, n, stream_));
+  DICE_TRY(mark(2));
+  HIP_TRY(launch_scan_segments(d, h, n, stream_));
+  DICE_TRY(mark(3));
+  return DICE_OK;
+}
+
+int32_t Runner::launch_b(int n) {
+  const JobDev* d = d_jobs_.as<JobDev>();
+  const JobDev* h = h_jobs_.as<JobDev>();
+  DICE_TRY(mark(4));
+  HIP_TRY(launch_sites(d, h, n, stream_));
+  DICE_TRY(mark(5));
+  HIP_TRY(launch_startup(d, h, n, stream_));
+  DICE_TRY(mark(6));
+  HIP_TRY(launch_writer_prep(d, n, stream_));
+  DICE_TRY(mark(7));
+  return DICE_OK;
+}
+
+double Runner::fine_ms(int a, int b) const {
+  float ms = 0.0f;
+  if (!fine_on_ || hipEventElapsedTime(&ms, fine_[a], fine_[b]) != hipSuccess) return 0.0;
+  return (double)ms;
+}
+
+std::string Runner::geometry_key(int n, char phase) const {
+  const JobDev* h = h_jobs_.as<JobDev>();
+  std::string key(1, phase);
+  char part[96];
+  snprintf(part, sizeof part, "|%d|%dx%d", n, tile_rows_, tile_cols_);
+  key += part;
+  for (int j = 0; j < n; ++j) {
+    snprintf(part, sizeof part, "|%dx%d:%d%d%d%d", h[j].height, h[j].width, h[j].run_producer,
+             h[j].skip, h[j].diag_score ? 1 : 0, h[j].diag_changed ? 1 : 0);
+    key += part;
+  }
+  return key;
+}
+
+// Launch directly, or replay the graph captured for this geometry (the
+// first run of a geometry is direct; the second captures it).
+int32_t Runner::replay_or_capture(Graph& g, const std::string& key, bool graph, bool* replayed,
+                                  int32_t (Runner::*launch)(int), int n) {
+  if (!graph) return (this->*launch)(n);
+  if (g.key != key) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key = key;
+    g.seen = 0;
+  }
+  g.seen += 1;
+  if (!g.exec) {
+    if (g.seen < 2) return (this->*launch)(n);
+    HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    const int32_t rc = (this->*launch)(n);
+    hipGraph_t captured = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream_, &captured);
+    if (rc != DICE_OK) {
+      if (captured) (void)hipGraphDestroy(captured);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return set_error(DICE_INTERNAL, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    const hipError_t ie = hipGraphInstantiate(&g.exec, captured, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(captured);
+    if (ie != hipSuccess) {
+      g.exec = nullptr;
+      return set_error(DICE_INTERNAL, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+    }
+  }
+  HIP_TRY(hipGraphLaunch(g.exec, stream_));
+  *replayed = true;
+  return DICE_OK;
+}
+
+int32_t Runner::phase_a(int n, bool graph, bool* replayed) {
+  return replay_or_capture(graph_a_, geometry_key(n, 'A'), graph, replayed, &Runner::launch_a, n);
+}
+
+int32_t Runner::phase_b(int n, bool graph, bool* replayed) {
+  return replay_or_capture(graph_b_, geometry_key(n, 'B'), graph, replayed, &Runner::launch_b, n);
+}
+
+namespace {
+
+struct JobThreads {
+  int32_t code = DICE_OK;
+  std::string error;
+  void fail(int32_t rc) {
+    if (code == DICE_OK) {
+      code = rc;
+      error = error_buffer();
+    }
+  }
+};
+
+}  // namespace
+
+int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunReport* report) {
+  if (n < 1 || n > kMaxBatch) return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
+  HIP_TRY(hipSetDevice(device_));
+  for (int j = 0; j < n; ++j) {
+    if (jobs[j].result) memset(jobs[j].result, 0, sizeof *jobs[j].result);
+  }
+  DICE_TRY(prepare(jobs, n, nullptr));
+  JobDev* H_jobs = h_jobs_.as<JobDev>();
+  const double t_start = now_seconds();
+  fine_on_ = options.fine_timing && !options.graph;
+  bool replayed_a = false, replayed_b = false;
+
+  // ---- phase A: producer, analysis scan, segment scan
+  for (int j = 0; j < n; ++j)
+    if (jobs[j].code == DICE_OK) jobs[j].cb.progress(2, 0, 1, 0, 0);
+  HIP_TRY(hipEventRecord(ev_[0], stream_));
+  DICE_TRY(phase_a(n, options.graph, &replayed_a));
+  HIP_TRY(hipEventRecord(ev_[1], stream_));
+  for (int j = 0; j < n; ++j) {
+    if (H_jobs[j].skip) continue;  // failed before the device: no workspace
+    Workspace& w = *ws_[j];
+    HIP_TRY(hipMemcpyAsync(w.h_small.ptr, w.counters.ptr, 64, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, w.flags.ptr, 4, hipMemcpyDeviceToHost,
+                           stream_));
+  }
+  HIP_TRY(hipStreamSynchronize(stream_));
+  const double t_a = now_seconds();
+  float ms_a = 0.0f;
+  (void)hipEventElapsedTime(&ms_a, ev_[0], ev_[1]);
+
+  // ---- size the site arrays
+  for (int j = 0; j < n; ++j) {
+    JobRequest& R = jobs[j];
+    JobDev& J = H_jobs[j];
+    Workspace& w = *ws_[j];
+    if (R.code != DICE_OK) {
+      J.skip = 1;
+      continue;
+    }
+    const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
+    if (int32_t rc = flag_error(flags)) {
+      R.code = rc;
+      R.error = error_buffer();
+      J.skip = 1;
+      continue;
+    }
+    if (R.cb.cancelled()) {
+      R.code = cancelled_error();
+      R.error = error_buffer();
+      J.skip = 1;
+      continue;
+    }
+    R.cb.progress(2, 1, 1, 0, 0);
+    R.cb.progress(3, 0, R.height, 0, 0);
+    const size_t count = w.h_small.as<uint32_t>()[0];
+    const size_t su = 6 * (size_t)R.width;
+    HIP_TRY(w.sites.ensure((count + 1) * sizeof(Site)));
+    HIP_TRY(w.records.ensure((count + 1) * sizeof(WriterRecord)));
+    HIP_TRY(w.states.ensure((count + su + 1) * 4));
+    HIP_TRY(w.h_records.ensure((count + su + 1) * sizeof(WriterRecord)));
+    HIP_TRY(w.h_states.ensure((count + su + 1) * 4));
+    J.sites = w.sites.as<Site>();
+    J.writer_records = w.records.as<WriterRecord>();
+    J.su_site_state = w.states.as<uint32_t>();
+    J.site_state = J.su_site_state + su;
+  }
+  HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
+                         stream_));
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip && H_jobs[j].diag_changed)
+      HIP_TRY(launch_fill_u8(H_jobs[j].diag_changed, (size_t)jobs[j].height * jobs[j].width, 0,
+                             stream_));
+
+  // ---- phase B: sites, hidden startup, writer draw schedules
+  HIP_TRY(hipEventRecord(ev_[2], stream_));
+  DICE_TRY(phase_b(n, options.graph, &replayed_b));
+  HIP_TRY(hipEventRecord(ev_[3], stream_));
+  for (int j = 0; j < n; ++j) {
+    const JobDev& J = H_jobs[j];
+    if (J.skip) continue;
+    Workspace& w = *ws_[j];
+    const size_t count = w.h_small.as<uint32_t>()[0];
+    const size_t su = 6 * (size_t)J.width;
+    WriterRecord* hr = w.h_records.as<WriterRecord>();
+    HIP_TRY(hipMemcpyAsync(hr, J.su_writer_records, su * sizeof(WriterRecord),
+                           hipMemcpyDeviceToHost, stream_));
+    if (count)
+      HIP_TRY(hipMemcpyAsync(hr + su, J.writer_records, count * sizeof(WriterRecord),
+                             hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(w.h_row_start.ptr, J.row_start, ((size_t)J.height + 1) * 4,
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, w.flags.ptr, 4, hipMemcpyDeviceToHost,
+                           stream_));
+  }
+  HIP_TRY(hipStreamSynchronize(stream_));
+  const double t_b = now_seconds();
+  float ms_b = 0.0f;
+  (void)hipEventElapsedTime(&ms_b, ev_[2], ev_[3]);
+  for (int j = 0; j < n; ++j) {
+    JobDev& J = H_jobs[j];
+    if (J.skip) continue;
+    const uint32_t flags = *reinterpret_cast<uint32_t*>(ws_[j]->h_small.as<uint8_t>() + 64);
+    if (int32_t rc = flag_error(flags)) {
+      jobs[j].code = rc;
+      jobs[j].error = error_buffer();
+      J.skip = 1;
+    }
+  }
+
+  // ---- host: per job, the serial chain and the hash, overlapped by band
+  std::vector<JobThreads> outcome(n);
+  std::vector<std::thread> threads;
+  std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    if (H_jobs[j].skip) continue;
+    Workspace& w = *ws_[j];
+    {
+      std::lock_guard<std::mutex> lock(w.mutex);
+      w.published = 0;
+      w.aborted = false;
+    }
+    const int bands = std::max(1, std::min<int>(std::min(options.bands, kMaxBands),
+                                                (int)jobs[j].height));
+    // chain thread
+    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds]() {
+      Workspace& w = *ws_[j];
+      JobRequest& R = jobs[j];
+      const JobDev& J = H_jobs[j];
+      JobThreads& out = outcome[j];
+      const double t0 = now_seconds();
+      auto body = [&]() -> int32_t {
+        HIP_TRY(hipSetDevice(device_));
+        const int64_t H = R.height, W = R.width;
+        const size_t su = 6 * (size_t)W;
+        const WriterRecord* records = w.h_records.as<WriterRecord>();
+        const uint32_t* row_start = w.h_row_start.as<uint32_t>();
+        uint32_t* states = w.h_states.as<uint32_t>();
+        dice_replay_result* res = R.result;
+        uint32_t state = R.rng_state;
+        for (int stage = 0; stage < 6; ++stage) {
+          uint64_t draws = 0;
+          int64_t attempted = 0;
+          for (int64_t x = 0; x < W; ++x) {
+            const size_t i = (size_t)stage * W + x;
+            states[i] = state;
+            attempted += (records[i].flags & kRecordAttempted) ? 1 : 0;
+            state = chain_site(records[i], state, draws);
+          }
+          res->startup_attempted[stage] = attempted;
+          res->startup_advances[stage] = (int64_t)draws;
+        }
+        res->startup_final_rng_state = state;
+        HIP_TRY(hipMemcpyAsync(J.su_site_state, states, su * 4, hipMemcpyHostToDevice, stream_));
+        HIP_TRY(launch_writer_startup_check(J, state, stream_));
+        uint64_t advances = 0;
+        const WriterRecord* main = records + su;
+        uint32_t* main_states = states + su;
+        for (int b = 0; b < bands; ++b) {
+          const uint32_t s0 = row_start[band_row(H, bands, b)];
+          const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
+          for (uint32_t i = s0; i < s1; ++i) {
+            main_states[i] = state;
+            state = chain_site(main[i], state, advances);
+          }
+          if (s1 > s0) {
+            HIP_TRY(hipMemcpyAsync(J.site_state + s0, main_states + s0, (size_t)(s1 - s0) * 4,
+                                   hipMemcpyHostToDevice, stream_));
+            HIP_TRY(launch_writer_values(J, s0, s1, state, stream_));
+          }
+          HIP_TRY(hipEventRecord(w.band[b], stream_));
+          {
+            std::lock_guard<std::mutex> lock(w.mutex);
+            w.published = b + 1;
+          }
+          w.cv.notify_all();
+        }
+        res->attempted_pixels = row_start[H];
+        res->public_rng_advances = advances;
+        res->final_rng_state = state;
+        HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 128, J.writer_counts,
+                               kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, stream_));
+        HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
+                               hipMemcpyDeviceToHost, stream_));
+        return DICE_OK;
+      };
+      if (int32_t rc = body()) {
+        out.fail(rc);
+        std::lock_guard<std::mutex> lock(w.mutex);
+        w.aborted = true;
+      }
+      w.cv.notify_all();
+      chain_seconds[j] = now_seconds() - t0;
+    });
+    // hash / download thread
+    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &hash_seconds,
+                          &copy_seconds]() {
+      Workspace& w = *ws_[j];
+      JobRequest& R = jobs[j];
+      const JobDev& J = H_jobs[j];
+      const bool want_sha = (R.flags & DICE_SKIP_SHA256) == 0;
+      const bool host_out = (R.flags & DICE_OUT_ON_DEVICE) == 0;
+      if (!want_sha && !host_out) return;
+      double hashing = 0.0, copying = 0.0;
+      auto body = [&]() -> int32_t {
+        HIP_TRY(hipSetDevice(device_));
+        const int64_t H = R.height, W = R.width;
+        uint8_t* target;
+        if (host_out) {
+          target = reinterpret_cast<uint8_t*>(R.out);
+        } else {
+          HIP_TRY(w.h_out.ensure((size_t)H * W * 6));
+          target = w.h_out.as<uint8_t>();
+        }
+        const uint8_t* source = reinterpret_cast<const uint8_t*>(J.out);
+        Sha256 sha;
+        for (int b = 0; b < bands; ++b) {
+          {
+            std::unique_lock<std::mutex> lock(w.mutex);
+            w.cv.wait(lock, [&] { return w.published > b || w.aborted; });
+            if (w.published <= b) return DICE_OK;  // the chain failed; it reports
+          }
+          HIP_TRY(hipEventSynchronize(w.band[b]));
+          const size_t begin = (size_t)band_row(H, bands, b) * W * 6;
+          const size_t end = (size_t)band_row(H, bands, b + 1) * W * 6;
+          const double c0 = now_seconds();
+          HIP_TRY(hipMemcpy(target + begin, source + begin, end - begin, hipMemcpyDeviceToHost));
+          const double c1 = now_seconds();
+          if (want_sha) sha.update(target + begin, end - begin);
+          copying += c1 - c0;
+          hashing += now_seconds() - c1;
+        }
+        if (want_sha && !sha.hex(R.result->output_sha256))
+          return set_error(DICE_INTERNAL, "SHA-256 failed");
+        return DICE_OK;
+      };
+      if (int32_t rc = body()) outcome[j].fail(rc);
+      hash_seconds[j] = hashing;
+      copy_seconds[j] = copying;
+    });
+  }
+  for (std::thread& t : threads) t.join();
+  HIP_TRY(hipStreamSynchronize(stream_));
+  const double t_host = now_seconds();
+
+  // ---- counts, checks, diagnostics, results
+  for (int j = 0; j < n; ++j) {
+    JobRequest& R = jobs[j];
+    const JobDev& J = H_jobs[j];
+    if (J.skip) continue;
+    Workspace& w = *ws_[j];
+    if (outcome[j].code != DICE_OK) {
+      R.code = outcome[j].code;
+      R.error = outcome[j].error;
+      continue;
+    }
+    const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
+    if (int32_t rc = flag_error(flags)) {
+      R.code = rc;
+      R.error = error_buffer();
+      continue;
+    }
+    const unsigned long long* slots =
+        reinterpret_cast<const unsigned long long*>(w.h_small.as<uint8_t>() + 128);
+    uint64_t written = 0, changed = 0;
+    for (int slot = 0; slot < kCountSlots; ++slot) {
+      written += slots[slot * kCountStride + 0];
+      changed += slots[slot * kCountStride + 1];
+    }
+    R.result->written_pixels = written;
+    R.result->changed_pixels = changed;
+    const size_t pixels = (size_t)R.height * R.width;
+    if (R.diag) {
+      if (R.diag->score_plane)
+        HIP_TRY(hipMemcpy(R.diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
+      if (R.diag->at_floor_mask)
+        HIP_TRY(hipMemcpy(R.diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
+      if (R.diag->changed_mask)
+        HIP_TRY(hipMemcpy(R.diag->changed_mask, J.diag_changed, pixels, hipMemcpyDeviceToHost));
+    }
+    R.cb.progress(3, R.height, R.height, (int64_t)R.result->attempted_pixels, (int64_t)written);
+    if (R.timings) {
+      dice_stage_timings& T = *R.timings;
+      T.producer_device += t_a - t_start;
+      T.analysis_device += t_b - t_a;
+      T.writer_host += chain_seconds[j];
+      T.download_output += copy_seconds[j];
+      T.sha256_host += hash_seconds[j];
+      T.finish_device += now_seconds() - t_host;
+      if (fine_on_) {
+        T.producer_kernels_ms += fine_ms(0, 1);
+        T.scan_kernel_ms += fine_ms(1, 2);
+        T.segments_kernel_ms += fine_ms(2, 3);
+        T.sites_kernel_ms += fine_ms(4, 5);
+        T.startup_kernels_ms += fine_ms(5, 6);
+        T.writer_prep_kernel_ms += fine_ms(6, 7);
+      } else {
+        T.producer_kernels_ms += ms_a;
+        T.sites_kernel_ms += ms_b;
+      }
+    }
+  }
+  if (report) {
+    report->tile_rows = tile_rows_;
+    report->tile_cols = tile_cols_;
+    report->graph_replayed = replayed_a && replayed_b;
+    report->phase_a_ms = ms_a;
+    report->phase_b_ms = ms_b;
+    report->workspace_bytes = workspace_bytes();
+  }
+  return DICE_OK;
+}
+
+int32_t Runner::producer_schedule(const uint16_t* rgbi, int64_t height, int64_t width,
+                                  const dice_params& params, dice_schedule* out) {
+  HIP_TRY(hipSetDevice(device_));
+  JobRequest R;
+  R.main = rgbi;
+  R.height = height;
+  R.width = width;
+  R.params = params;
+  fine_on_ = false;
+  DICE_TRY(prepare(&R, 1, nullptr));
+  const JobDev* d = d_jobs_.as<JobDev>();
+  const JobDev* h = h_jobs_.as<JobDev>();
+  HIP_TRY(launch_zero_jobs(d, h, 1, stream_));
+  HIP_TRY(launch_producer(d, h, 1, stream_));
+  const size_t H = (size_t)height;
+  const size_t sizes[12] = {8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+  for (int i = 0; i < 12; ++i) HIP_TRY(sched_[i].ensure(H * sizes[i]));
+  ScheduleDev S;
+  S.weighted_visible_sum = sched_[0].as<double>();
+  S.weighted_infrared_sum = sched_[1].as<double>();
+  S.weight_sum = sched_[2].as<double>();
+  S.mean_visible = sched_[3].as<float>();
+  S.mean_infrared = sched_[4].as<float>();
+  S.accepted = sched_[5].as<uint32_t>();
+  S.scaled_input = sched_[6].as<float>();
+  S.input_scale = sched_[7].as<float>();
+  S.scale_denominator = sched_[8].as<float>();
+  S.scale_numerator = sched_[9].as<float>();
+  S.scale_add_denominator = sched_[10].as<float>();
+  S.scale_add_numerator = sched_[11].as<float>();
+  HIP_TRY(launch_schedule_rows(h[0], S, stream_));
+  void* dst[12] = {out->weighted_visible_sum, out->weighted_infrared_sum, out->weight_sum,
+                   out->mean_visible,         out->mean_infrared,         out->accepted,
+                   out->scaled_input,         out->input_scale,           out->scale_denominator,
+                   out->scale_numerator,      out->scale_add_denominator, out->scale_add_numerator};
+  for (int i = 0; i < 12; ++i)
+    HIP_TRY(hipMemcpyAsync(dst[i], sched_[i].ptr, H * sizes[i], hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  return DICE_OK;
+}
+
+int32_t Runner::autotune(JobRequest* jobs, int n, RunReport* report) {
+  if (n < 1 || n > kMaxBatch) return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
+  HIP_TRY(hipSetDevice(device_));
+  std::string key = "T";
+  for (int j = 0; j < n; ++j) {
+    char part[64];
+    snprintf(part, sizeof part, "|%lldx%lld", (long long)jobs[j].height, (long long)jobs[j].width);
+    key += part;
+  }
+  auto it = tuned_.find(key);
+  if (it == tuned_.end()) {
+    int shapes[16][2];
+    const int count = std::min(16, tile_shapes(shapes, 16));
+    double best = 1e30;
+    std::pair<int, int> choice(tile_rows_, tile_cols_);
+    // trial copies: results, diagnostics and outputs of the timing runs are
+    // not the caller's
+    std::vector<JobRequest> trial(jobs, jobs + n);
+    std::vector<dice_replay_result> scratch(n);
+    for (int j = 0; j < n; ++j) {
+      trial[j].result = &scratch[j];
+      trial[j].diag = nullptr;
+      trial[j].timings = nullptr;
+      trial[j].cb = Callbacks{};
+    }
+    for (int s = 0; s < count; ++s) {
+      set_tile(shapes[s][0], shapes[s][1]);
+      double shape_ms = 1e30;
+      for (int rep = 0; rep < 2; ++rep) {
+        fine_on_ = false;
+        DICE_TRY(prepare(trial.data(), n, nullptr));
+        HIP_TRY(hipEventRecord(ev_[4], stream_));
+        DICE_TRY(launch_a(n));
+        JobDev* H_jobs = h_jobs_.as<JobDev>();
+        for (int j = 0; j < n; ++j)
+          if (!H_jobs[j].skip)
+            HIP_TRY(hipMemcpyAsync(ws_[j]->h_small.ptr, ws_[j]->counters.ptr, 64,
+                                   hipMemcpyDeviceToHost, stream_));
+        HIP_TRY(hipEventRecord(ev_[5], stream_));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        for (int j = 0; j < n; ++j) {
+          if (H_jobs[j].skip) continue;
+          Workspace& w = *ws_[j];
+          const size_t c = w.h_small.as<uint32_t>()[0];
+          HIP_TRY(w.sites.ensure((c + 1) * sizeof(Site)));
+          HIP_TRY(w.records.ensure((c + 1) * sizeof(WriterRecord)));
+          H_jobs[j].sites = w.sites.as<Site>();
+          H_jobs[j].writer_records = w.records.as<WriterRecord>();
+        }
+        HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n,
+                               hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipEventRecord(ev_[6], stream_));
+        DICE_TRY(launch_b(n));
+        HIP_TRY(hipEventRecord(ev_[7], stream_));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        float a = 0.0f, b = 0.0f;
+        (void)hipEventElapsedTime(&a, ev_[4], ev_[5]);
+        (void)hipEventElapsedTime(&b, ev_[6], ev_[7]);
+        shape_ms = std::min(shape_ms, (double)a + (double)b);
+      }
+      if (shape_ms < best) {
+        best = shape_ms;
+        choice = {shapes[s][0], shapes[s][1]};
+      }
+    }
+    it = tuned_.emplace(key, choice).first;
+  }
+  set_tile(it->second.first, it->second.second);
+  if (report) {
+    report->tile_rows = tile_rows_;
+    report->tile_cols = tile_cols_;
+  }
+  return DICE_OK;
+}
+
+}  // namespace dice

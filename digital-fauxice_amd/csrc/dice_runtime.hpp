@@ -1,0 +1,133 @@
+// dice_runtime.hpp -- the job runtime of libdice_hip.so: one Runner executes
+// a batch of 1..kMaxBatch independent jobs on one HIP stream.
+//
+//   phase A (device, one persistent launch per kernel for the whole batch):
+//     zero flags -> producer + setter -> analysis scan -> segment scan
+//   host: site counts, workspace sizing
+//   phase B (device): sites -> hidden startup -> writer draw schedules
+//   host, per job, two threads:
+//     chain thread   the serial LCG chain over the draw schedules, in row
+//                    bands; after each band its start states go to the
+//                    device and the band's writer values are launched
+//     hash thread    waits for each band, downloads its output rows and
+//                    feeds SHA-256 (so the chain and the hash overlap)
+//
+// Phases A and B have static launch geometry (site counts are read on the
+// device), so a repeated batch replays them from captured hipGraphs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dice_common.hpp"
+#include "dice_device.hpp"
+
+namespace dice {
+
+DevParams device_params(const dice_params& p);
+WriterConsts writer_consts(const dice_params& p);
+
+// One job as the caller described it, plus its outcome.
+struct JobRequest {
+  const uint16_t* main = nullptr;
+  int64_t height = 0, width = 0;
+  uint32_t flags = 0;
+  uint16_t* out = nullptr;
+  dice_params params{};
+  const dice_stage_table* stages = nullptr;  // nullptr: producer on device
+  float factor_b = 1.0f;
+  uint32_t rng_state = 0x3045u;
+  dice_diagnostics* diag = nullptr;
+  Callbacks cb;
+  dice_replay_result* result = nullptr;
+  dice_stage_timings* timings = nullptr;
+  // outcome
+  int32_t code = DICE_OK;
+  std::string error;
+};
+
+struct Workspace;
+
+struct RunOptions {
+  bool graph = false;        // replay phases A/B from captured graphs
+  bool fine_timing = false;  // HIP events between kernel groups (direct launches only)
+  int bands = 8;             // writer/hash pipeline bands per job
+};
+
+struct RunReport {
+  int tile_rows = 0, tile_cols = 0;
+  bool graph_replayed = false;
+  double phase_a_ms = 0.0, phase_b_ms = 0.0;  // device time (HIP events)
+  uint64_t workspace_bytes = 0;
+};
+
+// Device-resident tables shared by every job of a device.
+struct DeviceTables {
+  DeviceBuffer lut, high, low, noop;
+  int32_t upload();
+};
+
+class Runner {
+ public:
+  Runner(int device, const DeviceTables* tables);
+  ~Runner();
+  Runner(const Runner&) = delete;
+  Runner& operator=(const Runner&) = delete;
+  int32_t init();
+  // Runs jobs[0..n) as one batch.  Returns DICE_OK when the batch ran
+  // (each job's own status is in jobs[i].code / .error), or the code of a
+  // failure that prevented the batch from running at all.
+  int32_t run(JobRequest* jobs, int n, const RunOptions& options, RunReport* report);
+  // Times phases A+B of this batch for every supported tile shape and
+  // keeps the fastest (results of the timing runs are discarded).
+  int32_t autotune(JobRequest* jobs, int n, RunReport* report);
+  // The producer record schedule of one frame (arrays only; the caller
+  // packs the payloads).
+  int32_t producer_schedule(const uint16_t* rgbi, int64_t height, int64_t width,
+                            const dice_params& params, dice_schedule* out);
+  void set_tile(int rows, int cols) {
+    tile_rows_ = rows;
+    tile_cols_ = cols;
+  }
+  int tile_rows() const { return tile_rows_; }
+  int tile_cols() const { return tile_cols_; }
+  hipStream_t stream() const { return stream_; }
+  uint64_t workspace_bytes() const;
+
+ private:
+  struct Graph {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    int seen = 0;
+  };
+  int32_t prepare(JobRequest* jobs, int n, bool* any_host_work);
+  int32_t phase_a(int n, bool graph, bool* replayed);
+  int32_t phase_b(int n, bool graph, bool* replayed);
+  int32_t replay_or_capture(Graph& g, const std::string& key, bool graph, bool* replayed,
+                            int32_t (Runner::*launch)(int), int n);
+  int32_t launch_a(int n);
+  int32_t launch_b(int n);
+  int32_t mark(int slot);
+  double fine_ms(int a, int b) const;
+  std::string geometry_key(int n, char phase) const;
+
+  int device_;
+  const DeviceTables* tables_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[8] = {};
+  hipEvent_t fine_[8] = {};
+  bool fine_on_ = false;
+  int tile_rows_ = 16, tile_cols_ = 64;
+  std::vector<std::unique_ptr<Workspace>> ws_;
+  DeviceBuffer d_jobs_;
+  PinnedBuffer h_jobs_;
+  Graph graph_a_, graph_b_;
+  std::map<std::string, std::pair<int, int>> tuned_;
+  DeviceBuffer sched_[12];
+};
+
+}  // namespace dice

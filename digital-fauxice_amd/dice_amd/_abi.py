@@ -23,6 +23,8 @@ DICE_INTERNAL = 7
 DICE_MAIN_ON_DEVICE = 0x1
 DICE_OUT_ON_DEVICE = 0x2
 DICE_SKIP_SHA256 = 0x4
+DICE_BATCH_GRAPH = 0x100
+DICE_BATCH_AUTOTUNE = 0x200
 
 f32p = C.POINTER(C.c_float)
 f64p = C.POINTER(C.c_double)
@@ -173,6 +175,37 @@ class DiceStageTimings(C.Structure):
         return {name: float(getattr(self, name)) for name, _ in self._fields_}
 
 
+class DiceJob(C.Structure):
+    _fields_ = [
+        ("main_rgbi", C.c_void_p),
+        ("height", C.c_int64),
+        ("width", C.c_int64),
+        ("prepass_rgbi", C.c_void_p),
+        ("prepass_height", C.c_int64),
+        ("prepass_width", C.c_int64),
+        ("params", C.POINTER(DiceParams)),
+        ("prepass_params", C.POINTER(DicePrepassParams)),
+        ("flags", C.c_uint32),
+        ("out_rgb16", C.c_void_p),
+        ("result", C.POINTER(DiceReplayResult)),
+        ("timings", C.POINTER(DiceStageTimings)),
+        ("status", C.c_int32),
+        ("error", C.c_char * 256),
+    ]
+
+
+class DiceBatchReport(C.Structure):
+    _fields_ = [
+        ("tile_rows", C.c_int32),
+        ("tile_cols", C.c_int32),
+        ("graph_replayed", C.c_int32),
+        ("phase_a_ms", C.c_double),
+        ("phase_b_ms", C.c_double),
+        ("workspace_bytes", C.c_uint64),
+        ("wall_seconds", C.c_double),
+    ]
+
+
 class DiceDeviceInfo(C.Structure):
     _fields_ = [
         ("name", C.c_char * 128),
@@ -219,6 +252,10 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
          C.POINTER(DicePrepassParams), C.c_uint32, VOIDP,
          C.POINTER(DiceReplayResult), C.POINTER(DiceDiagnostics),
          C.POINTER(DiceCallbacks), C.POINTER(DiceStageTimings)],
+    ),
+    "process_batch": (
+        C.c_int32,
+        [VOIDP, C.POINTER(DiceJob), C.c_int32, C.c_uint32, C.POINTER(DiceBatchReport)],
     ),
     "t_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),
     "t_rgb_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),

@@ -224,6 +224,39 @@ typedef struct dice_ctx dice_ctx;
 #define DICE_OUT_ON_DEVICE 0x2u  /* output RGB16 pointer is device memory */
 #define DICE_SKIP_SHA256 0x4u    /* leave output_sha256 empty */
 
+/* ---- Batches ------------------------------------------------------------
+ * No counterpart in the reference, which processes one job per call
+ * (engine.py:110-267): many independent jobs (crops or frames of uneven
+ * size) run through the same device phases together, one persistent kernel
+ * launch per phase for the whole batch.  Each job is exactly the job
+ * dice_process_job would run (the profile's prepass, producer and replay);
+ * its result is bit-identical to running it alone. */
+typedef struct dice_job {
+  const uint16_t* main_rgbi;          /* H*W*4, host or device (flags) */
+  int64_t height, width;
+  const uint16_t* prepass_rgbi;       /* host */
+  int64_t prepass_height, prepass_width;
+  const dice_params* params;
+  const dice_prepass_params* prepass_params;
+  uint32_t flags;                     /* DICE_MAIN_ON_DEVICE | DICE_OUT_ON_DEVICE | DICE_SKIP_SHA256 */
+  uint16_t* out_rgb16;                /* H*W*3, host or device (flags) */
+  dice_replay_result* result;         /* required */
+  dice_stage_timings* timings;        /* optional */
+  int32_t status;                     /* out: this job's return code */
+  char error[256];                    /* out: this job's message when status != 0 */
+} dice_job;
+
+#define DICE_BATCH_GRAPH 0x100u    /* replay the device phases from captured hipGraphs */
+#define DICE_BATCH_AUTOTUNE 0x200u /* time every tile shape on this batch geometry first */
+
+typedef struct dice_batch_report {
+  int32_t tile_rows, tile_cols;   /* analysis tile shape used */
+  int32_t graph_replayed;         /* 1 when both device phases replayed a captured graph */
+  double phase_a_ms, phase_b_ms;  /* device time of the two batched phases */
+  uint64_t workspace_bytes;       /* HBM held by the context's job workspaces */
+  double wall_seconds;
+} dice_batch_report;
+
 int32_t dice_abi_version(void);
 const char* dice_last_error(void);
 
@@ -249,6 +282,12 @@ int32_t dice_stage_table_from_schedule(
     int64_t count, float* aux_alpha, float* aux_offset, float* aux_replacement,
     uint8_t* aux_has_replacement, float* score_base, float* coarse_reference,
     uint8_t* gate_secondary, int32_t* row_gate);
+
+/* Runs jobs[0..count) (1..64) as one batch.  Returns 0 when every job
+ * succeeded, else the first failing job's code (each job's own outcome is in
+ * jobs[i].status / .error; other jobs are unaffected). */
+int32_t dice_process_batch(dice_ctx* ctx, dice_job* jobs, int32_t count, uint32_t batch_flags,
+                           dice_batch_report* report);
 
 int32_t dice_replay(dice_ctx* ctx, const uint16_t* rgbi, int64_t height,
                     int64_t width, const float* lut, const dice_params* params,

@@ -1,0 +1,195 @@
+"""Batched execution: many uneven jobs in one set of persistent launches.
+
+Every job of a batch must reproduce, byte for byte, both the oracle (the C
+restatement of the reference CPU path) and the same job run alone through
+``process_hip``.  Graph replays are checked replay by replay against the
+eager batch, with unrelated eager work and synchronization between the
+replays; autotuned tile shapes must not change any result; a job that fails
+validation inside a batch fails alone.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib
+from conftest import hip_device_or_skip
+from dice_amd import _abi, synth
+from dice_amd._abi import DiceBatchReport, DiceJob, DiceReplayResult
+from dice_amd.hip_backend import engine, process_hip
+from dice_amd.hip_backend.batch import process_batch_hip
+from dice_amd.hip_backend.process import stable_profile_params
+from dice_amd.profile import DEFAULT_PROFILE
+from test_hip_parity import make_job, replay_fingerprint
+
+pytestmark = pytest.mark.gpu
+
+# Uneven geometries: ragged tiles in both directions, a tiny frame, tall and
+# wide shapes, at the full-frame defect density.
+SHAPES = [(97, 211), (256, 256), (300, 517), (64, 72), (513, 129), (181, 640)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    hip_device_or_skip()
+
+
+@pytest.fixture(scope="module")
+def frames():
+    out = []
+    for i, (h, w) in enumerate(SHAPES):
+        spec = synth.scaled_spec(h, w, seed=700 + i)
+        main = synth.frame(spec)
+        out.append((main, synth.prepass_for(main)))
+    out.append(cases.supported_job_frames())
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracle_fingerprints(frames):
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    prints = []
+    for main, prepass in frames:
+        cpu = oracle_lib.process_job(main, prepass, params, prepass_params)
+        prints.append({
+            "output_sha256": cpu.output_sha256,
+            "attempted_pixels": cpu.attempted_pixels,
+            "written_pixels": cpu.written_pixels,
+            "public_rng_advances": cpu.public_rng_advances,
+            "final_rng_state": cpu.final_rng_state,
+            "changed_pixels": cpu.changed_pixels,
+        })
+    return prints
+
+
+def _core(fingerprint: dict) -> dict:
+    keys = ("output_sha256", "attempted_pixels", "written_pixels", "public_rng_advances",
+            "final_rng_state", "changed_pixels")
+    return {k: fingerprint[k] for k in keys}
+
+
+def _jobs(frames):
+    return [make_job(main, prepass, f"batch-{i}") for i, (main, prepass) in enumerate(frames)]
+
+
+def test_batch_matches_oracle_and_single_runs(frames, oracle_fingerprints):
+    outcome = process_batch_hip(_jobs(frames))
+    outcome.raise_first()
+    for i, (main, prepass) in enumerate(frames):
+        got = replay_fingerprint(outcome.results[i].replay)
+        assert _core(got) == oracle_fingerprints[i], f"job {i} {main.shape}"
+        alone = process_hip(make_job(main, prepass, f"alone-{i}"))
+        assert replay_fingerprint(alone.replay) == got, f"job {i} batch vs alone"
+        assert np.array_equal(alone.output_rgb16, outcome.results[i].output_rgb16)
+
+
+def test_batch_graph_replays_match_eager(frames, oracle_fingerprints):
+    jobs = _jobs(frames)
+    eager = process_batch_hip(jobs)
+    eager.raise_first()
+    expected = [replay_fingerprint(r.replay) for r in eager.results]
+    replayed = []
+    for attempt in range(4):
+        outcome = process_batch_hip(jobs, graph=True)
+        outcome.raise_first()
+        got = [replay_fingerprint(r.replay) for r in outcome.results]
+        assert got == expected, f"graph run {attempt}"
+        for i, r in enumerate(outcome.results):
+            assert np.array_equal(r.output_rgb16, eager.results[i].output_rgb16)
+        replayed.append(outcome.report["graph_replayed"])
+        # unrelated eager work and a synchronization between replays
+        main, prepass = frames[1]
+        alone = process_hip(make_job(main, prepass, "between"))
+        assert _core(replay_fingerprint(alone.replay)) == oracle_fingerprints[1]
+    assert replayed[0] == 0 and all(replayed[2:]), replayed
+
+
+def test_batch_autotune_keeps_results(frames, oracle_fingerprints):
+    outcome = process_batch_hip(_jobs(frames), autotune=True)
+    outcome.raise_first()
+    shape = (outcome.report["tile_rows"], outcome.report["tile_cols"])
+    assert shape in {(16, 64), (8, 128), (32, 32), (16, 32), (32, 64), (8, 64)}
+    for i in range(len(frames)):
+        assert _core(replay_fingerprint(outcome.results[i].replay)) == oracle_fingerprints[i]
+
+
+def test_failing_job_fails_alone(frames, oracle_fingerprints):
+    """A job rejected by validation inside the batch does not disturb the others."""
+
+    lib = engine.library()
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    picks = [frames[0], frames[1]]
+    bad = np.full((4, 16, 4), 40000, np.uint16)  # fewer than five rows
+    keep, outs, results = [], [], []
+    array = (DiceJob * 3)()
+    for i, (main, prepass) in enumerate([picks[0], (bad, bad), picks[1]]):
+        main = np.ascontiguousarray(main)
+        prepass = np.ascontiguousarray(prepass)
+        out = np.zeros(main.shape[:2] + (3,), np.uint16)
+        res = DiceReplayResult()
+        keep += [main, prepass]
+        outs.append(out)
+        results.append(res)
+        e = array[i]
+        e.main_rgbi = main.ctypes.data
+        e.height, e.width = main.shape[:2]
+        e.prepass_rgbi = prepass.ctypes.data
+        e.prepass_height, e.prepass_width = prepass.shape[:2]
+        e.params = C.pointer(params)
+        e.prepass_params = C.pointer(prepass_params)
+        e.out_rgb16 = out.ctypes.data
+        e.result = C.pointer(res)
+    report = DiceBatchReport()
+    code = lib.dice_process_batch(engine.context(), array, 3, 0, C.byref(report))
+    assert code == _abi.DICE_INVALID, lib.dice_last_error()
+    assert array[1].status == _abi.DICE_INVALID
+    assert b"at least 5 rows" in array[1].error
+    for slot, index in ((0, 0), (2, 1)):
+        assert array[slot].status == _abi.DICE_OK
+        assert results[slot].output_sha256.decode() == oracle_fingerprints[index]["output_sha256"]
+        assert results[slot].attempted_pixels == oracle_fingerprints[index]["attempted_pixels"]
+
+
+def test_batch_device_resident_buffers(frames, oracle_fingerprints):
+    """Inputs and outputs already in HBM (the serving layout)."""
+
+    import hashlib
+
+    from dice_amd.hip_backend.device import DeviceArray
+
+    lib = engine.library()
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    picks = frames[:3]
+    keep, results, outs = [], [], []
+    array = (DiceJob * len(picks))()
+    for i, (main, prepass) in enumerate(picks):
+        d_main = DeviceArray.from_host(main)
+        d_out = DeviceArray(main.shape[0] * main.shape[1] * 6)
+        prepass = np.ascontiguousarray(prepass)
+        res = DiceReplayResult()
+        keep += [d_main, prepass]
+        outs.append(d_out)
+        results.append(res)
+        e = array[i]
+        e.main_rgbi = d_main.address
+        e.height, e.width = main.shape[:2]
+        e.prepass_rgbi = prepass.ctypes.data
+        e.prepass_height, e.prepass_width = prepass.shape[:2]
+        e.params = C.pointer(params)
+        e.prepass_params = C.pointer(prepass_params)
+        e.flags = _abi.DICE_MAIN_ON_DEVICE | _abi.DICE_OUT_ON_DEVICE
+        e.out_rgb16 = d_out.address
+        e.result = C.pointer(res)
+    code = lib.dice_process_batch(engine.context(), array, len(picks), 0, None)
+    assert code == 0, lib.dice_last_error()
+    for i, (main, _) in enumerate(picks):
+        assert results[i].output_sha256.decode() == oracle_fingerprints[i]["output_sha256"]
+        host = outs[i].download(np.empty(main.shape[:2] + (3,), np.uint16))
+        assert hashlib.sha256(host.tobytes()).hexdigest() == oracle_fingerprints[i]["output_sha256"]
