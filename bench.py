@@ -10,7 +10,7 @@ the timed region.  One job covers:
 * the SHA-256 of the public RGB16 output, which needs the output bytes on
   the host.
 
-Three legs:
+Four legs:
   plain    one job per call (dice_process_job), W warm-up + K timed jobs:
            p50/p99 latency and Mpix/s of the plain single-frame path;
   batched  the headline `value`: B frames per dice_process_batch call,
@@ -19,6 +19,10 @@ Three legs:
            W warm-up + K timed batches;
   crops    16 crops of uneven size (sides 512-3000 px, aspect 0.5-2) as one
            batch (same machinery): Mpix/s and batch latency.
+  serving  Poisson arrivals (70 % 1024-2048 px crops, 30 % full frames) at
+           80 % of the batched capacity into the serving engine (continuous
+           batching, HBM-budget admission, two drivers): p50/p99 latency and
+           Mpix/s, against the same jobs processed one at a time.
 
 Every timed job's output SHA-256 and counters are compared with the compiled
 CPU oracle (the C restatement of the reference CPU path) run on the same
@@ -56,6 +60,8 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
+    parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    parser.add_argument("--serving-jobs", type=int, default=40)
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -233,6 +239,91 @@ def main() -> int:
         }
         log(f"crops: {crops}")
 
+    # ---- serving leg: Poisson arrivals into the continuous-batching engine
+    serving = None
+    if not args.no_serving:
+        from dice_amd.hip_backend.device import DeviceArray
+        from dice_amd.hip_backend.serving import HipServingEngine
+
+        rng = np.random.default_rng(5)
+        crop_pool = Frames([synth.scaled_spec(int(rng.integers(1024, 2049)),
+                                              int(rng.integers(1024, 2049)), seed=5000 + i)
+                            for i in range(8)])
+        pool = [(full, i) for i in range(min(4, args.batch))] + [(crop_pool, i) for i in range(8)]
+        picks = []
+        for _ in range(args.serving_jobs):
+            if rng.random() < 0.3:
+                picks.append(pool[int(rng.integers(0, min(4, args.batch)))])
+            else:
+                picks.append(pool[min(4, args.batch) + int(rng.integers(0, 8))])
+        job_mpix = [frames.mpix[i] for frames, i in picks]
+        outs = [DeviceArray(frames.specs[i].height * frames.specs[i].width * 6)
+                for frames, i in picks]
+        rate = 0.8 * value / statistics.mean(job_mpix)  # jobs per second
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(picks)))
+        # the same jobs one at a time (plain path)
+        plain_prints, plain_job_lat = [], []
+        a0 = time.perf_counter()
+        for (frames, i), out in zip(picks, outs):
+            spec = frames.specs[i]
+            res = DiceReplayResult()
+            a = time.perf_counter()
+            engine.raise_for(lib.dice_process_job(
+                ctx, C.c_void_p(frames.d_main[i].address), spec.height, spec.width,
+                ptr(frames.prepasses[i], C.c_uint16), frames.prepasses[i].shape[0],
+                frames.prepasses[i].shape[1], C.byref(params), C.byref(prepass_params),
+                device_flags, C.c_void_p(out.address), C.byref(res), None, None, None))
+            plain_job_lat.append((time.perf_counter() - a) * 1e3)
+            plain_prints.append(fingerprint(res))
+        plain_mix_elapsed = time.perf_counter() - a0
+        with HipServingEngine(drivers=2, max_batch=8, graph=False) as served:
+            for (frames, i), out in zip(picks[:4], outs[:4]):  # warm the drivers
+                spec = frames.specs[i]
+                served.wait(served.submit_device(frames.d_main[i].address, spec.height,
+                                                 spec.width, frames.prepasses[i], out.address))
+            first = served.stats()
+            tickets = []
+            t_start = time.perf_counter()
+            for (frames, i), out, at in zip(picks, outs, arrivals):
+                delay = at - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                spec = frames.specs[i]
+                tickets.append(served.submit_device(frames.d_main[i].address, spec.height,
+                                                    spec.width, frames.prepasses[i],
+                                                    out.address))
+            served_prints = []
+            for t in tickets:
+                r = served.wait(t).replay
+                served_prints.append((r.output_sha256, r.attempted_pixels, r.written_pixels,
+                                      r.public_rng_advances, r.final_rng_state,
+                                      r.changed_pixels))
+            serve_elapsed = time.perf_counter() - t_start
+            stats = served.stats()
+        lat = []  # the timed jobs only (the warm-up jobs are the first latencies)
+        serving = {
+            "jobs": len(picks),
+            "arrival_rate_jobs_s": round(rate, 2),
+            "value": round(sum(job_mpix) / serve_elapsed, 2),
+            "unit": "Mpix/s",
+            "latency_p50_ms": round(stats["latency_p50_ms"], 3),
+            "latency_p99_ms": round(stats["latency_p99_ms"], 3),
+            "queue_p50_ms": round(stats["queue_p50_ms"], 3),
+            "mean_batch": round(stats["mean_batch"], 2),
+            "admitted_bytes_peak": int(stats["admitted_bytes_peak"]),
+            "workspace_bytes": int(stats["workspace_bytes"]),
+            "plain_one_at_a_time": {
+                "value": round(sum(job_mpix) / plain_mix_elapsed, 2),
+                "unit": "Mpix/s",
+                "p50_ms": round(float(np.percentile(plain_job_lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(plain_job_lat, 99)), 3),
+            },
+            "identical_to_plain": served_prints == [tuple(p) for p in plain_prints],
+            "latency_note": "engine percentiles include the 4 warm-up jobs",
+        }
+        del lat, first
+        log(f"serving: {serving}")
+
     # ---- CPU baseline (the oracle), which also checks every output
     cpu_baseline = None
     bit_exact = None
@@ -256,6 +347,22 @@ def main() -> int:
         checked = len(expected)
         bit_exact = (all(p == expected[0] for p in plain_results)
                      and all(prints[:checked] == expected for prints in batch_results))
+        if serving is not None:
+            # every served job against the oracle on its pool frame
+            oracle_prints = {}
+            for frames, i in set(picks):
+                if frames is full and i < checked:
+                    oracle_prints[(id(frames), i)] = expected[i]
+                else:
+                    cpu = oracle_lib.process_job(frames.mains[i], frames.prepasses[i], params,
+                                                 prepass_params, threads=threads)
+                    oracle_prints[(id(frames), i)] = (
+                        cpu.output_sha256, cpu.attempted_pixels, cpu.written_pixels,
+                        cpu.public_rng_advances, cpu.final_rng_state, cpu.changed_pixels)
+            serving_exact = all(p == oracle_prints[(id(f), i)]
+                                for p, (f, i) in zip(served_prints, picks))
+            serving["bit_exact"] = serving_exact
+            bit_exact = bit_exact and serving_exact
         timed = runs[1:] if len(runs) > 1 else runs
         median = statistics.median(timed)
         cpu_baseline = {
@@ -305,6 +412,7 @@ def main() -> int:
         },
         "batched_vs_plain": round(value / plain_value, 3),
         "crops": crops,
+        "serving": serving,
         "bit_exact": bit_exact,
         "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
         "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)

@@ -1,0 +1,278 @@
+// dice_engine.cpp -- the serving loop of libdice_hip.so (no counterpart in
+// the reference, which processes one job per call).
+//
+// Jobs are submitted as they arrive and wait in a FIFO queue.  Admission
+// is by HBM budget: each job reserves an upper bound of its device working
+// set (input and output when they come from the host, site lists, site and
+// writer records for every pixel, the hidden-startup slots) and is admitted
+// only while the reservations of the jobs on the device fit the budget.
+// Several drivers -- each a thread with its own Runner, i.e. its own HIP
+// stream and job workspaces -- take batches from the head of the queue
+// whenever they are free, so jobs that arrive while others run join the
+// next batch (continuous batching), and the device phases of one batch
+// overlap the host chain and hash of another.  Every job's result is the
+// one it gets alone: jobs share nothing but the device.
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/dice_hip.h"
+#include "dice_common.hpp"
+#include "dice_runtime.hpp"
+
+using namespace dice;
+
+namespace {
+
+struct Ticket {
+  dice_job* job = nullptr;
+  uint64_t id = 0;
+  uint64_t reserve = 0;
+  double submitted = 0.0, started = 0.0, finished = 0.0;
+  bool done = false;
+};
+
+// Upper bound of one job's device working set (bytes).
+uint64_t reservation(const dice_job& job) {
+  const uint64_t pixels = (uint64_t)std::max<int64_t>(job.height, 0) *
+                          (uint64_t)std::max<int64_t>(job.width, 0);
+  uint64_t bytes = pixels * (2 + sizeof(Site) + sizeof(WriterRecord) + 4);  // lists, records
+  bytes += 6ull * (uint64_t)std::max<int64_t>(job.width, 0) * 128;          // hidden startup
+  bytes += (uint64_t)std::max<int64_t>(job.height, 0) * 160 + (1u << 20);   // per-row tables
+  if (!(job.flags & DICE_MAIN_ON_DEVICE)) bytes += pixels * 8;
+  if (!(job.flags & DICE_OUT_ON_DEVICE)) bytes += pixels * 6;
+  return bytes;
+}
+
+}  // namespace
+
+struct dice_engine {
+  dice_engine_config config{};
+  DeviceTables tables;
+  std::vector<std::unique_ptr<Runner>> runners;
+  std::vector<std::thread> drivers;
+  std::mutex mutex;
+  std::condition_variable work, finished;
+  std::deque<Ticket*> queue;
+  std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
+  uint64_t next_id = 1;
+  bool stopping = false;
+  // accounting
+  uint64_t admitted = 0, admitted_peak = 0;
+  uint64_t submitted = 0, completed = 0, failed = 0, running = 0, batches = 0, batch_jobs = 0;
+  double mpix = 0.0, first_submit = 0.0, last_finish = 0.0;
+  std::vector<double> latency_ms, queue_ms;
+  std::vector<uint64_t> workspace;  // per driver, updated by the driver itself
+
+  void drive(int index);
+};
+
+void dice_engine::drive(int index) {
+  Runner& runner = *runners[index];
+  (void)hipSetDevice(config.device);
+  std::vector<Ticket*> batch;
+  std::vector<dice_job> jobs;
+  for (;;) {
+    batch.clear();
+    {
+      std::unique_lock<std::mutex> lock(mutex);
+      work.wait(lock, [&] {
+        if (stopping) return true;
+        if (queue.empty()) return false;
+        return admitted == 0 || admitted + queue.front()->reserve <= config.hbm_budget;
+      });
+      if (stopping && queue.empty()) return;
+      if (queue.empty()) continue;
+      while (!queue.empty() && (int)batch.size() < config.max_batch) {
+        Ticket* t = queue.front();
+        if (admitted != 0 && admitted + t->reserve > config.hbm_budget) break;
+        queue.pop_front();
+        admitted += t->reserve;
+        batch.push_back(t);
+      }
+      admitted_peak = std::max(admitted_peak, admitted);
+      running += batch.size();
+      const double now = now_seconds();
+      for (Ticket* t : batch) t->started = now;
+    }
+    jobs.resize(batch.size());
+    for (size_t i = 0; i < batch.size(); ++i) jobs[i] = *batch[i]->job;
+    const int32_t rc = process_jobs(runner, jobs.data(), (int32_t)jobs.size(),
+                                    config.batch_flags, nullptr);
+    const std::string batch_error = rc != DICE_OK ? error_buffer() : std::string();
+    const uint64_t held = runner.workspace_bytes();
+    {
+      std::lock_guard<std::mutex> lock(mutex);
+      workspace[index] = held;
+      const double now = now_seconds();
+      batches += 1;
+      batch_jobs += batch.size();
+      for (size_t i = 0; i < batch.size(); ++i) {
+        Ticket* t = batch[i];
+        dice_job& caller = *t->job;
+        caller.status = jobs[i].status;
+        memcpy(caller.error, jobs[i].error, sizeof caller.error);
+        // a batch that could not run at all fails every job in it
+        if (rc != DICE_OK && caller.status == DICE_OK) {
+          caller.status = rc;
+          snprintf(caller.error, sizeof caller.error, "%s", batch_error.c_str());
+        }
+        t->finished = now;
+        t->done = true;
+        admitted -= t->reserve;
+        running -= 1;
+        if (caller.status == DICE_OK) {
+          completed += 1;
+          mpix += (double)caller.height * (double)caller.width / 1e6;
+        } else {
+          failed += 1;
+        }
+        latency_ms.push_back((t->finished - t->submitted) * 1e3);
+        queue_ms.push_back((t->started - t->submitted) * 1e3);
+        last_finish = now;
+      }
+    }
+    finished.notify_all();
+    work.notify_all();
+  }
+}
+
+namespace {
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const double pos = q * (double)(v.size() - 1);
+  const size_t lo = (size_t)pos;
+  const size_t hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+}
+}  // namespace
+
+extern "C" {
+
+int32_t dice_engine_create(const dice_engine_config* config, dice_engine** out) {
+  *out = nullptr;
+  DICE_TRY(require_tables());
+  dice_engine_config c = *config;
+  if (c.drivers < 1) c.drivers = 2;
+  if (c.drivers > 8) return set_error(DICE_INVALID, "at most 8 drivers");
+  if (c.max_batch < 1) c.max_batch = 8;
+  if (c.max_batch > kMaxBatch) return set_error(DICE_INVALID, "max_batch must be 1..%d", kMaxBatch);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || c.device < 0 || c.device >= count)
+    return set_error(DICE_UNAVAILABLE, "HIP device %d is not available", c.device);
+  HIP_TRY(hipSetDevice(c.device));
+  if (c.hbm_budget == 0) {
+    size_t free_bytes = 0, total_bytes = 0;
+    HIP_TRY(hipMemGetInfo(&free_bytes, &total_bytes));
+    c.hbm_budget = (uint64_t)(0.8 * (double)free_bytes);
+  }
+  std::unique_ptr<dice_engine> engine(new dice_engine());
+  engine->config = c;
+  engine->workspace.assign(c.drivers, 0);
+  if (int32_t rc = engine->tables.upload()) return rc;
+  for (int i = 0; i < c.drivers; ++i) {
+    engine->runners.emplace_back(new Runner(c.device, &engine->tables));
+    DICE_TRY(engine->runners.back()->init());
+  }
+  for (int i = 0; i < c.drivers; ++i)
+    engine->drivers.emplace_back([e = engine.get(), i] { e->drive(i); });
+  *out = engine.release();
+  return DICE_OK;
+}
+
+void dice_engine_destroy(dice_engine* engine) {
+  if (!engine) return;
+  {
+    std::lock_guard<std::mutex> lock(engine->mutex);
+    engine->stopping = true;
+  }
+  engine->work.notify_all();
+  for (std::thread& t : engine->drivers) t.join();
+  (void)hipSetDevice(engine->config.device);
+  engine->runners.clear();
+  delete engine;
+}
+
+int32_t dice_engine_submit(dice_engine* engine, dice_job* job, uint64_t* ticket) {
+  if (!engine || !job || !ticket) return set_error(DICE_INVALID, "null engine, job or ticket");
+  const uint64_t reserve = reservation(*job);
+  if (reserve > engine->config.hbm_budget)
+    return set_error(DICE_OVER_BUDGET,
+                     "job needs up to %llu bytes of HBM; the engine budget is %llu",
+                     (unsigned long long)reserve,
+                     (unsigned long long)engine->config.hbm_budget);
+  {
+    std::lock_guard<std::mutex> lock(engine->mutex);
+    if (engine->stopping) return set_error(DICE_UNAVAILABLE, "engine is shutting down");
+    std::unique_ptr<Ticket> t(new Ticket());
+    t->job = job;
+    t->id = engine->next_id++;
+    t->reserve = reserve;
+    t->submitted = now_seconds();
+    if (engine->submitted == 0) engine->first_submit = t->submitted;
+    engine->submitted += 1;
+    job->status = DICE_PENDING;
+    job->error[0] = 0;
+    *ticket = t->id;
+    engine->queue.push_back(t.get());
+    engine->tickets.emplace(t->id, std::move(t));
+  }
+  engine->work.notify_all();
+  return DICE_OK;
+}
+
+int32_t dice_engine_wait(dice_engine* engine, uint64_t ticket, double timeout_seconds) {
+  if (!engine) return set_error(DICE_INVALID, "null engine");
+  std::unique_lock<std::mutex> lock(engine->mutex);
+  auto it = engine->tickets.find(ticket);
+  if (it == engine->tickets.end()) return set_error(DICE_INVALID, "unknown ticket %llu",
+                                                    (unsigned long long)ticket);
+  Ticket* t = it->second.get();
+  auto ready = [&] { return t->done; };
+  if (timeout_seconds < 0) {
+    engine->finished.wait(lock, ready);
+  } else if (!engine->finished.wait_for(lock, std::chrono::duration<double>(timeout_seconds),
+                                        ready)) {
+    return set_error(DICE_PENDING, "job %llu has not finished", (unsigned long long)ticket);
+  }
+  const int32_t status = t->job->status;
+  if (status != DICE_OK) set_error(status, "%s", t->job->error);
+  engine->tickets.erase(it);
+  return status;
+}
+
+int32_t dice_engine_stats(dice_engine* engine, dice_engine_report* stats) {
+  if (!engine || !stats) return set_error(DICE_INVALID, "null engine or stats");
+  memset(stats, 0, sizeof *stats);
+  std::lock_guard<std::mutex> lock(engine->mutex);
+  stats->submitted = engine->submitted;
+  stats->completed = engine->completed;
+  stats->failed = engine->failed;
+  stats->queued = engine->queue.size();
+  stats->running = engine->running;
+  stats->batches = engine->batches;
+  stats->mean_batch = engine->batches ? (double)engine->batch_jobs / (double)engine->batches : 0.0;
+  stats->mpix_completed = engine->mpix;
+  stats->latency_p50_ms = percentile(engine->latency_ms, 0.50);
+  stats->latency_p99_ms = percentile(engine->latency_ms, 0.99);
+  stats->latency_max_ms = percentile(engine->latency_ms, 1.0);
+  stats->queue_p50_ms = percentile(engine->queue_ms, 0.50);
+  stats->admitted_bytes = engine->admitted;
+  stats->admitted_bytes_peak = engine->admitted_peak;
+  stats->hbm_budget = engine->config.hbm_budget;
+  uint64_t held = 0;
+  for (uint64_t w : engine->workspace) held += w;
+  stats->workspace_bytes = held;
+  stats->elapsed_seconds =
+      engine->submitted ? std::max(0.0, engine->last_finish - engine->first_submit) : 0.0;
+  return DICE_OK;
+}
+
+}  // extern "C"

@@ -305,6 +305,80 @@ int32_t run_single(dice_ctx* ctx, JobRequest& R, bool timings) {
   return DICE_OK;
 }
 
+}  // namespace
+
+namespace dice {
+// Admission (validation + prepass) of every job, then one batch run; each
+// job's outcome lands in its dice_job.  Shared by dice_process_batch and
+// the serving engine.
+int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t batch_flags,
+                     dice_batch_report* report) {
+  if (report) memset(report, 0, sizeof *report);
+  if (count < 1 || count > kMaxBatch)
+    return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
+  const double start = now_seconds();
+  std::vector<JobRequest> requests(count);
+  for (int32_t j = 0; j < count; ++j) {
+    dice_job& job = jobs[j];
+    JobRequest& R = requests[j];
+    job.status = DICE_OK;
+    job.error[0] = 0;
+    if (job.result) memset(job.result, 0, sizeof *job.result);
+    if (job.timings) memset(job.timings, 0, sizeof *job.timings);
+    R.main = job.main_rgbi;
+    R.height = job.height;
+    R.width = job.width;
+    R.flags = job.flags;
+    R.out = job.out_rgb16;
+    R.result = job.result;
+    R.timings = job.timings;
+    if (!job.params || !job.prepass_params || !job.result || !job.main_rgbi || !job.out_rgb16) {
+      R.code = set_error(DICE_INVALID, "batch job %d is missing a required pointer", j);
+      R.error = error_buffer();
+      continue;
+    }
+    R.params = *job.params;
+    double prepass_seconds = 0.0;
+    if (int32_t rc = admit_job(job.prepass_rgbi, job.prepass_height, job.prepass_width,
+                               job.height, job.width, *job.params, *job.prepass_params,
+                               &prepass_seconds)) {
+      R.code = rc;
+      R.error = error_buffer();
+    }
+    if (job.timings) job.timings->prepass = prepass_seconds;
+  }
+  RunReport rr;
+  if (batch_flags & DICE_BATCH_AUTOTUNE) DICE_TRY(runner.autotune(requests.data(), count, &rr));
+  RunOptions options;
+  options.graph = (batch_flags & DICE_BATCH_GRAPH) != 0;
+  DICE_TRY(runner.run(requests.data(), count, options, &rr));
+  int32_t first_error = DICE_OK;
+  for (int32_t j = 0; j < count; ++j) {
+    dice_job& job = jobs[j];
+    job.status = requests[j].code;
+    snprintf(job.error, sizeof job.error, "%s", requests[j].error.c_str());
+    if (job.timings) job.timings->total = now_seconds() - start;
+    if (job.status != DICE_OK && first_error == DICE_OK) {
+      first_error = job.status;
+      set_error(job.status, "batch job %d: %s", j, job.error);
+    }
+  }
+  if (report) {
+    report->tile_rows = rr.tile_rows;
+    report->tile_cols = rr.tile_cols;
+    report->graph_replayed = rr.graph_replayed ? 1 : 0;
+    report->phase_a_ms = rr.phase_a_ms;
+    report->phase_b_ms = rr.phase_b_ms;
+    report->workspace_bytes = rr.workspace_bytes;
+    report->wall_seconds = now_seconds() - start;
+  }
+  return first_error;
+}
+
+}  // namespace dice
+
+namespace {
+
 template <class F>
 int32_t with_device_buffers(dice_ctx* ctx, F&& body) {
   DICE_TRY(activate(ctx));
@@ -586,66 +660,7 @@ int32_t dice_process_batch(dice_ctx* ctx, dice_job* jobs, int32_t count, uint32_
                            dice_batch_report* report) {
   DICE_TRY(activate(ctx));
   std::lock_guard<std::mutex> lock(ctx->mutex);
-  if (report) memset(report, 0, sizeof *report);
-  if (count < 1 || count > kMaxBatch)
-    return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
-  const double start = now_seconds();
-  std::vector<JobRequest> requests(count);
-  for (int32_t j = 0; j < count; ++j) {
-    dice_job& job = jobs[j];
-    JobRequest& R = requests[j];
-    job.status = DICE_OK;
-    job.error[0] = 0;
-    if (job.result) memset(job.result, 0, sizeof *job.result);
-    if (job.timings) memset(job.timings, 0, sizeof *job.timings);
-    R.main = job.main_rgbi;
-    R.height = job.height;
-    R.width = job.width;
-    R.flags = job.flags;
-    R.out = job.out_rgb16;
-    R.result = job.result;
-    R.timings = job.timings;
-    if (!job.params || !job.prepass_params || !job.result || !job.main_rgbi || !job.out_rgb16) {
-      R.code = set_error(DICE_INVALID, "batch job %d is missing a required pointer", j);
-      R.error = error_buffer();
-      continue;
-    }
-    R.params = *job.params;
-    double prepass_seconds = 0.0;
-    if (int32_t rc = admit_job(job.prepass_rgbi, job.prepass_height, job.prepass_width,
-                               job.height, job.width, *job.params, *job.prepass_params,
-                               &prepass_seconds)) {
-      R.code = rc;
-      R.error = error_buffer();
-    }
-    if (job.timings) job.timings->prepass = prepass_seconds;
-  }
-  RunReport rr;
-  if (batch_flags & DICE_BATCH_AUTOTUNE) DICE_TRY(ctx->runner->autotune(requests.data(), count, &rr));
-  RunOptions options;
-  options.graph = (batch_flags & DICE_BATCH_GRAPH) != 0;
-  DICE_TRY(ctx->runner->run(requests.data(), count, options, &rr));
-  int32_t first_error = DICE_OK;
-  for (int32_t j = 0; j < count; ++j) {
-    dice_job& job = jobs[j];
-    job.status = requests[j].code;
-    snprintf(job.error, sizeof job.error, "%s", requests[j].error.c_str());
-    if (job.timings) job.timings->total = now_seconds() - start;
-    if (job.status != DICE_OK && first_error == DICE_OK) {
-      first_error = job.status;
-      set_error(job.status, "batch job %d: %s", j, job.error);
-    }
-  }
-  if (report) {
-    report->tile_rows = rr.tile_rows;
-    report->tile_cols = rr.tile_cols;
-    report->graph_replayed = rr.graph_replayed ? 1 : 0;
-    report->phase_a_ms = rr.phase_a_ms;
-    report->phase_b_ms = rr.phase_b_ms;
-    report->workspace_bytes = rr.workspace_bytes;
-    report->wall_seconds = now_seconds() - start;
-  }
-  return first_error;
+  return process_jobs(*ctx->runner, jobs, count, batch_flags, report);
 }
 
 int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches, int64_t count,

@@ -130,4 +130,9 @@ class Runner {
   DeviceBuffer sched_[12];
 };
 
+// Validation, prepass and one batch run of caller-described jobs (the body
+// of dice_process_batch, also used by the serving engine).
+int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t batch_flags,
+                     dice_batch_report* report);
+
 }  // namespace dice

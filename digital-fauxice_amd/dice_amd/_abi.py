@@ -19,6 +19,7 @@ DICE_NONFINITE_WORK = 4
 DICE_CANCELLED = 5
 DICE_OVER_BUDGET = 6
 DICE_INTERNAL = 7
+DICE_PENDING = 8
 
 DICE_MAIN_ON_DEVICE = 0x1
 DICE_OUT_ON_DEVICE = 0x2
@@ -206,6 +207,41 @@ class DiceBatchReport(C.Structure):
     ]
 
 
+class DiceEngineConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("drivers", C.c_int32),
+        ("max_batch", C.c_int32),
+        ("batch_flags", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+    ]
+
+
+class DiceEngineStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("queued", C.c_uint64),
+        ("running", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("mean_batch", C.c_double),
+        ("mpix_completed", C.c_double),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("latency_max_ms", C.c_double),
+        ("queue_p50_ms", C.c_double),
+        ("admitted_bytes", C.c_uint64),
+        ("admitted_bytes_peak", C.c_uint64),
+        ("hbm_budget", C.c_uint64),
+        ("workspace_bytes", C.c_uint64),
+        ("elapsed_seconds", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class DiceDeviceInfo(C.Structure):
     _fields_ = [
         ("name", C.c_char * 128),
@@ -257,6 +293,11 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
         C.c_int32,
         [VOIDP, C.POINTER(DiceJob), C.c_int32, C.c_uint32, C.POINTER(DiceBatchReport)],
     ),
+    "engine_create": (C.c_int32, [C.POINTER(DiceEngineConfig), C.POINTER(VOIDP)]),
+    "engine_destroy": (None, [VOIDP]),
+    "engine_submit": (C.c_int32, [VOIDP, C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
+    "engine_wait": (C.c_int32, [VOIDP, C.c_uint64, C.c_double]),
+    "engine_stats": (C.c_int32, [VOIDP, C.POINTER(DiceEngineStats)]),
     "t_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),
     "t_rgb_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),
     "t_dither_chain": (

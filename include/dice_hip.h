@@ -44,6 +44,7 @@
  *   5 DICE_CANCELLED         -> ProcessingCancelled
  *   6 DICE_OVER_BUDGET       -> HipBackendUnavailable (HBM budget)
  *   7 DICE_INTERNAL          -> HipBackendUnavailable (HIP runtime error)
+ *   8 DICE_PENDING           -> TimeoutError (dice_engine_wait: not finished yet)
  */
 #ifndef DICE_HIP_H
 #define DICE_HIP_H
@@ -65,7 +66,8 @@ enum dice_status {
   DICE_NONFINITE_WORK = 4,
   DICE_CANCELLED = 5,
   DICE_OVER_BUDGET = 6,
-  DICE_INTERNAL = 7
+  DICE_INTERNAL = 7,
+  DICE_PENDING = 8
 };
 
 /* Stable (non-stage) replay parameters.  Every float is the binary32 value
@@ -282,6 +284,47 @@ int32_t dice_stage_table_from_schedule(
     int64_t count, float* aux_alpha, float* aux_offset, float* aux_replacement,
     uint8_t* aux_has_replacement, float* score_base, float* coarse_reference,
     uint8_t* gate_secondary, int32_t* row_gate);
+
+/* ---- Serving engine -----------------------------------------------------
+ * No counterpart in the reference (one job per call).  Jobs are submitted as
+ * they arrive; drivers (threads, each with its own HIP stream and job
+ * workspaces) take batches from the head of the FIFO queue as soon as they
+ * are free, admitting a job only while the HBM reservations of the jobs on
+ * the device fit hbm_budget (continuous batching).  A submitted dice_job and
+ * every buffer it points to stay owned by the caller and must stay valid
+ * until dice_engine_wait has returned for its ticket; the engine writes only
+ * its result, timings, status and error. */
+typedef struct dice_engine dice_engine;
+
+typedef struct dice_engine_config {
+  int32_t device;
+  int32_t drivers;          /* concurrent batches, 1..8 (0: 2) */
+  int32_t max_batch;        /* jobs per batch, 1..64 (0: 8) */
+  uint32_t batch_flags;     /* DICE_BATCH_GRAPH | DICE_BATCH_AUTOTUNE */
+  uint64_t hbm_budget;      /* bytes of job reservations admitted at once (0: 80% of free HBM) */
+} dice_engine_config;
+
+typedef struct dice_engine_report {
+  uint64_t submitted, completed, failed, queued, running;
+  uint64_t batches;
+  double mean_batch;              /* jobs per batch */
+  double mpix_completed;
+  double latency_p50_ms, latency_p99_ms, latency_max_ms;  /* submit -> finished */
+  double queue_p50_ms;            /* submit -> started */
+  uint64_t admitted_bytes, admitted_bytes_peak, hbm_budget;
+  uint64_t workspace_bytes;       /* HBM the drivers' workspaces hold */
+  double elapsed_seconds;         /* first submit -> last finish */
+} dice_engine_report;
+
+int32_t dice_engine_create(const dice_engine_config* config, dice_engine** out);
+void dice_engine_destroy(dice_engine* engine);
+/* DICE_OVER_BUDGET when the job alone cannot fit the budget. */
+int32_t dice_engine_submit(dice_engine* engine, dice_job* job, uint64_t* ticket);
+/* Returns the job's status (DICE_PENDING when timeout_seconds >= 0 runs out;
+ * a negative timeout waits until the job is done).  A finished ticket is
+ * forgotten once waited for. */
+int32_t dice_engine_wait(dice_engine* engine, uint64_t ticket, double timeout_seconds);
+int32_t dice_engine_stats(dice_engine* engine, dice_engine_report* stats);
 
 /* Runs jobs[0..count) (1..64) as one batch.  Returns 0 when every job
  * succeeded, else the first failing job's code (each job's own outcome is in

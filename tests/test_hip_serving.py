@@ -1,0 +1,124 @@
+"""Serving: continuous batching with HBM-budget admission.
+
+Jobs of mixed sizes are submitted from several threads while others are on
+the device; each result must equal the oracle's and the same job processed
+alone (``process_hip``), byte for byte.  A budget that admits one job's
+reservation at a time must never be exceeded, and a job that cannot fit the
+budget at all is refused at submission.
+"""
+
+from __future__ import annotations
+
+import random
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from conftest import hip_device_or_skip
+from dice_amd import _abi, synth
+from dice_amd.hip_backend import process_hip
+from dice_amd.hip_backend.process import stable_profile_params
+from dice_amd.hip_backend.serving import HipServingEngine
+from dice_amd.profile import DEFAULT_PROFILE
+from test_hip_parity import make_job, replay_fingerprint
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(300, 517), (97, 211), (513, 129), (256, 256), (181, 640), (400, 400), (64, 72),
+          (222, 333)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    hip_device_or_skip()
+
+
+@pytest.fixture(scope="module")
+def frames():
+    out = []
+    for i, (h, w) in enumerate(SHAPES):
+        main = synth.frame(synth.scaled_spec(h, w, seed=900 + i))
+        out.append((main, synth.prepass_for(main)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracle_shas(frames):
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    return [oracle_lib.process_job(m, p, params, prepass_params).output_sha256 for m, p in frames]
+
+
+def test_results_alone_and_in_the_mix_are_identical(frames, oracle_shas):
+    alone = [replay_fingerprint(process_hip(make_job(m, p, f"alone-{i}")).replay)
+             for i, (m, p) in enumerate(frames)]
+    for i, fp in enumerate(alone):
+        assert fp["output_sha256"] == oracle_shas[i]
+    rng = random.Random(11)
+    order = [i for i in range(len(frames)) for _ in range(3)]
+    rng.shuffle(order)
+    tickets: dict[int, list] = {i: [] for i in range(len(frames))}
+    lock = threading.Lock()
+
+    with HipServingEngine(drivers=2, max_batch=4) as engine:
+        def client(indices):
+            for i in indices:
+                time.sleep(rng.random() * 0.01)
+                main, prepass = frames[i]
+                t = engine.submit(make_job(main, prepass, f"mix-{i}"))
+                with lock:
+                    tickets[i].append(t)
+
+        workers = [threading.Thread(target=client, args=(order[k::3],)) for k in range(3)]
+        for w in workers:
+            w.start()
+        for w in workers:
+            w.join()
+        for i, ts in tickets.items():
+            for t in ts:
+                result = engine.wait(t, timeout=120)
+                assert replay_fingerprint(result.replay) == alone[i], f"frame {i}"
+                digest = result.replay.output_sha256
+                assert digest == oracle_shas[i]
+        stats = engine.stats()
+    assert stats["completed"] == len(order) and stats["failed"] == 0
+    assert stats["batches"] >= 1 and stats["mean_batch"] >= 1.0
+    assert stats["latency_p99_ms"] >= stats["latency_p50_ms"] > 0
+
+
+def test_budget_admits_one_reservation_at_a_time(frames, oracle_shas):
+    largest = max(m.shape[0] * m.shape[1] for m, _ in frames)
+    # about one largest job's reservation: admission must queue the rest
+    budget = largest * 120 + (2 << 20)
+    with HipServingEngine(drivers=2, max_batch=8, hbm_budget=budget) as engine:
+        tickets = [(i, engine.submit(make_job(m, p, f"budget-{i}")))
+                   for i, (m, p) in enumerate(frames)]
+        for i, t in tickets:
+            assert engine.wait(t, timeout=120).replay.output_sha256 == oracle_shas[i]
+        stats = engine.stats()
+    assert 0 < stats["admitted_bytes_peak"] <= budget
+
+
+def test_job_larger_than_the_budget_is_refused(frames):
+    main, prepass = frames[0]
+    with HipServingEngine(drivers=1, hbm_budget=1 << 20) as engine:
+        with pytest.raises(Exception) as error:
+            engine.submit(make_job(main, prepass, "too-big"))
+        assert "budget" in str(error.value)
+
+
+def test_wait_reports_pending_then_result(frames, oracle_shas):
+    main, prepass = frames[5]
+    with HipServingEngine(drivers=1) as engine:
+        t = engine.submit(make_job(main, prepass, "pending"))
+        try:
+            engine.wait(t, timeout=0.0)
+        except TimeoutError:
+            pass
+        else:  # finished already: the ticket is gone, nothing more to check
+            return
+        assert engine.wait(t, timeout=120).replay.output_sha256 == oracle_shas[5]
+    assert _abi.DICE_PENDING == 8
