@@ -56,13 +56,23 @@ struct DeviceBuffer {
   ~DeviceBuffer() {
     if (ptr) (void)hipFree(ptr);
   }
-  hipError_t ensure(size_t need) {
+  // Grows to at least `need` bytes (contents are not kept).  With a stream,
+  // the old block is released and the new one taken in stream order from the
+  // device's memory pool, so growing one job's workspace does not wait for
+  // the work of other streams.
+  hipError_t ensure(size_t need, hipStream_t stream = nullptr) {
     if (need <= bytes) return hipSuccess;
-    if (ptr) (void)hipFree(ptr);
+    const size_t grown = bytes ? need + need / 4 : need;
+    const size_t rounded = (grown + 0xFFFFF) & ~(size_t)0xFFFFF;
+    if (ptr) {
+      if (stream)
+        (void)hipFreeAsync(ptr, stream);
+      else
+        (void)hipFree(ptr);
+    }
     ptr = nullptr;
     bytes = 0;
-    size_t rounded = (need + 0xFFFFF) & ~(size_t)0xFFFFF;
-    hipError_t e = hipMalloc(&ptr, rounded);
+    hipError_t e = stream ? hipMallocAsync(&ptr, rounded, stream) : hipMalloc(&ptr, rounded);
     if (e == hipSuccess) bytes = rounded;
     return e;
   }
@@ -83,10 +93,11 @@ struct PinnedBuffer {
   }
   hipError_t ensure(size_t need) {
     if (need <= bytes) return hipSuccess;
+    const size_t grown = bytes ? need + need / 4 : need;
+    const size_t rounded = (grown + 0xFFFF) & ~(size_t)0xFFFF;
     if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     bytes = 0;
-    size_t rounded = (need + 0xFFFF) & ~(size_t)0xFFFF;
     hipError_t e = hipHostMalloc(&ptr, rounded, hipHostMallocDefault);
     if (e == hipSuccess) bytes = rounded;
     return e;

@@ -19,26 +19,32 @@ struct alignas(8) Site {
 };
 static_assert(sizeof(Site) == 40, "Site record must stay 40 bytes");
 
-// The writer's draw schedule of one site in integer form.  For every
-// channel that draws (low < f32(cand) < high), u is the LCG state after the
-// first draw (or its mirror 2^24-1-state when the dither span is negative,
-// so the dither delta is non-decreasing in u); the draw is accepted for
-// u in [accept_lo, accept_hi) and, when accepted, the original-floor test
-// cand + delta <= orig holds for u < floor_end.  Flags: bit 0 valid
-// candidate, bit 1 floor gate; channel c: bit 2+4c draws, bit 3+4c floor
-// test result when the draw is rejected, bit 4+4c mirrored state; bit 31
-// the site is attempted (always for main-frame sites).
+// The writer's draw schedule of one site in integer form, one 64-bit word
+// per channel.  A channel that draws takes the LCG step s -> t; with
+// u = t (or its mirror 2^24-1-t when the dither span is negative, so that
+// the dither delta is non-decreasing in u) it keeps t ("copy": the value is
+// decided by this draw) when ((u - a) mod 2^32 < len) != inv, and otherwise
+// redraws once more.  A channel without the floor gate always keeps t
+// (a = 0, len = 2^24); one that can never satisfy the floor test always
+// redraws (len = 0).  Bits: 0-23 a, 24-48 len, 49 inv, 50 mirror, 51 draws;
+// bit 63 of channel 0: the site is attempted (always for main-frame sites).
 struct WriterRecord {
-  uint32_t flags;
-  uint32_t accept_lo[3];
-  uint32_t accept_hi[3];
-  uint32_t floor_end[3];
+  uint64_t ch[3];
 };
-static_assert(sizeof(WriterRecord) == 40, "WriterRecord must stay 40 bytes");
+static_assert(sizeof(WriterRecord) == 24, "WriterRecord must stay 24 bytes");
+
+constexpr uint64_t kChanInv = 1ull << 49;
+constexpr uint64_t kChanMirror = 1ull << 50;
+constexpr uint64_t kChanDraws = 1ull << 51;
+constexpr uint64_t kRecordAttemptedBit = 1ull << 63;
+
+DICE_HD uint64_t channel_word(uint32_t a, uint32_t len, bool inv, bool mirror) {
+  return (uint64_t)(a & 0xFFFFFFu) | ((uint64_t)len << 24) | (inv ? kChanInv : 0ull) |
+         (mirror ? kChanMirror : 0ull) | kChanDraws;
+}
 
 constexpr uint32_t kSitePixelMask = 0x7FFFFFFFu;
 constexpr uint32_t kSiteFloorBit = 0x80000000u;
-constexpr uint32_t kRecordAttempted = 0x80000000u;
 
 // Written/changed counters are spread over slots 128 B apart so that the
 // per-workgroup atomics do not all hit one address.
@@ -73,20 +79,18 @@ DICE_HD uint32_t prefix_count(const F& holds) {
 // checks the whole chain.
 DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const WriterConsts& K) {
   WriterRecord r;
-  r.flags = 0;
-  for (int c = 0; c < 3; ++c) r.accept_lo[c] = r.accept_hi[c] = r.floor_end[c] = 0;
+  r.ch[0] = r.ch[1] = r.ch[2] = 0ull;
   if (!attempted) return r;
   const bool valid = finite64(site.cand[0]) && finite64(site.cand[1]) &&
                      finite64(site.cand[2]) && site.cand[0] > 0.0 && site.cand[1] > 0.0 &&
                      site.cand[2] > 0.0;
   if (valid) {
     const bool floor = (site.pixel & kSiteFloorBit) != 0;
-    r.flags = 1u | (floor ? 2u : 0u);
     for (int c = 0; c < 3; ++c) {
       const double cd = (double)(float)site.cand[c];
       if (!(K.low_lt_high && K.low < cd && cd < K.high)) continue;
       if (!floor) {  // one draw, whatever its value
-        r.flags |= 1u << (2 + 4 * c);
+        r.ch[c] = channel_word(0u, 1u << 24, false, false);
         continue;
       }
       const double width = K.high - K.low;
@@ -102,44 +106,56 @@ DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const Write
       };
       const double cand = site.cand[c];
       const double source = (double)site.orig[c];
-      r.accept_lo[c] = prefix_count([&](uint32_t u) { return cd + delta(u) <= K.low; });
-      r.accept_hi[c] = prefix_count([&](uint32_t u) { return cd + delta(u) < K.high; });
-      r.floor_end[c] = prefix_count([&](uint32_t u) { return cand + delta(u) <= source; });
-      const bool floor_when_rejected = cand + 0.0 <= source;
-      r.flags |= (1u | (floor_when_rejected ? 2u : 0u) | (mirrored ? 4u : 0u)) << (2 + 4 * c);
+      // acceptance low < cd + delta < high holds for u in [lo, hi); when
+      // accepted, the floor test cand + delta <= orig holds for u < fe; a
+      // rejected draw (delta = 0) keeps the value when cand <= orig
+      const uint32_t lo = prefix_count([&](uint32_t u) { return cd + delta(u) <= K.low; });
+      const uint32_t hi = prefix_count([&](uint32_t u) { return cd + delta(u) < K.high; });
+      const uint32_t fe = prefix_count([&](uint32_t u) { return cand + delta(u) <= source; });
+      const bool keep_when_rejected = cand + 0.0 <= source;
+      if (!keep_when_rejected) {
+        // keep <=> lo <= u < min(hi, fe)
+        const uint32_t end = hi < fe ? hi : fe;
+        r.ch[c] = channel_word(lo, end > lo ? end - lo : 0u, false, mirrored);
+      } else {
+        // redraw <=> max(lo, fe) <= u < hi
+        const uint32_t begin = lo > fe ? lo : fe;
+        r.ch[c] = channel_word(begin, hi > begin ? hi - begin : 0u, true, mirrored);
+      }
     }
   }
-  r.flags |= kRecordAttempted;
+  r.ch[0] |= kRecordAttemptedBit;
   return r;
 }
 
 DICE_HD uint32_t lcg_step(uint32_t s) { return (125u * s + 1u) & kLcgMask; }
 
-// The serial part of the writer on one record: one LCG step per drawing
-// channel, and a redraw step when the floor gate is on and the floor test
-// fails.  The LCG has no fixed point (124 s = -1 mod 2^24 has no
-// solution), so every draw is one counted advance.
+// The serial part of the writer on one record, branch-free: per drawing
+// channel one LCG step, and a second one unless the draw is kept.  The LCG
+// has no fixed point (124 s = -1 mod 2^24 has no solution), so every draw
+// is one counted advance.
 DICE_HD uint32_t chain_site(const WriterRecord& r, uint32_t s, uint64_t& draws) {
-  const uint32_t f = r.flags;
-  if (!(f & 1u)) return s;
-  const bool floor = (f & 2u) != 0;
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const uint32_t cf = f >> (2 + 4 * c);
-    if (!(cf & 1u)) continue;
+    const uint64_t w = r.ch[c];
+    const uint32_t a = (uint32_t)(w & 0xFFFFFFu);
+    const uint32_t len = (uint32_t)((w >> 24) & 0x1FFFFFFu);
+    const uint32_t mirror = (w & kChanMirror) ? kLcgMask : 0u;
+    const bool inv = (w & kChanInv) != 0;
+    const bool drawn = (w & kChanDraws) != 0;
     const uint32_t t = lcg_step(s);
-    draws += 1;
-    s = t;
-    if (floor) {
-      const uint32_t u = (cf & 4u) ? kLcgMask - t : t;
-      const bool accepted = u >= r.accept_lo[c] && u < r.accept_hi[c];
-      const bool copy = accepted ? u < r.floor_end[c] : (cf & 2u) != 0;
-      if (!copy) {
-        s = lcg_step(t);
-        draws += 1;
-      }
-    }
+    const uint32_t t2 = lcg_step(t);
+    const uint32_t u = t ^ mirror;
+    const bool keep = ((u - a) < len) != inv;
+    const uint32_t next = keep ? t : t2;
+    draws += drawn ? (keep ? 1u : 2u) : 0u;
+    s = drawn ? next : s;
   }
   return s;
+}
+
+DICE_HD bool record_attempted(const WriterRecord& r) {
+  return (r.ch[0] & kRecordAttemptedBit) != 0;
 }
 
 // Device error flags (atomicOr): the host maps them to the reference's

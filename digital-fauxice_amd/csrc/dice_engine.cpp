@@ -216,7 +216,7 @@ int32_t dice_engine_submit(dice_engine* engine, dice_job* job, uint64_t* ticket)
     t->id = engine->next_id++;
     t->reserve = reserve;
     t->submitted = now_seconds();
-    if (engine->submitted == 0) engine->first_submit = t->submitted;
+    if (engine->first_submit == 0.0) engine->first_submit = t->submitted;
     engine->submitted += 1;
     job->status = DICE_PENDING;
     job->error[0] = 0;
@@ -246,6 +246,20 @@ int32_t dice_engine_wait(dice_engine* engine, uint64_t ticket, double timeout_se
   if (status != DICE_OK) set_error(status, "%s", t->job->error);
   engine->tickets.erase(it);
   return status;
+}
+
+int32_t dice_engine_reset_stats(dice_engine* engine) {
+  if (!engine) return set_error(DICE_INVALID, "null engine");
+  std::lock_guard<std::mutex> lock(engine->mutex);
+  engine->completed = engine->failed = engine->batches = engine->batch_jobs = 0;
+  engine->submitted = engine->queue.size() + engine->running;
+  engine->mpix = 0.0;
+  engine->latency_ms.clear();
+  engine->queue_ms.clear();
+  engine->admitted_peak = engine->admitted;
+  engine->first_submit = now_seconds();
+  engine->last_finish = engine->first_submit;
+  return DICE_OK;
 }
 
 int32_t dice_engine_stats(dice_engine* engine, dice_engine_report* stats) {

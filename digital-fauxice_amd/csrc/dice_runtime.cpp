@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -24,6 +25,20 @@ namespace dice {
 
 namespace {
 constexpr int kMaxBands = 64;
+
+// DICE_TRACE=1 prints one timeline line per batch to stderr.
+bool trace_enabled() {
+  static const bool on = [] {
+    const char* v = getenv("DICE_TRACE");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+
+double trace_origin() {
+  static const double origin = now_seconds();
+  return origin;
+}
 }  // namespace
 
 DevParams device_params(const dice_params& p) {
@@ -117,7 +132,14 @@ struct Workspace {
   PinnedBuffer h_row_start;  // H+1
   PinnedBuffer h_states;     // 6W startup states, then the main states
   PinnedBuffer h_out;        // output bytes to hash when the output stays on the device
-  hipEvent_t band[kMaxBands] = {};
+  hipEvent_t band[kMaxBands] = {};     // writer values of band b done (runner stream)
+  hipEvent_t fetched[kMaxBands] = {};  // draw schedules of band b on the host (copy stream)
+  hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (copy stream)
+  hipEvent_t startup_fetched = nullptr;
+  // this job's output rows to the host: its own in-order stream, so a copy
+  // waiting for this job's next band never holds up another job's copies
+  hipStream_t copy_stream = nullptr;
+  int bands = 1;
   bool events = false;
   // chain -> hash hand-off
   std::mutex mutex;
@@ -126,12 +148,24 @@ struct Workspace {
   bool aborted = false;
 
   ~Workspace() {
-    for (hipEvent_t& e : band)
-      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t* set : {band, fetched, copied})
+      for (int b = 0; b < kMaxBands; ++b)
+        if (set[b]) (void)hipEventDestroy(set[b]);
+    if (startup_fetched) (void)hipEventDestroy(startup_fetched);
+    if (copy_stream) {
+      (void)hipStreamSynchronize(copy_stream);
+      (void)hipStreamDestroy(copy_stream);
+    }
   }
   int32_t init_events() {
     if (events) return DICE_OK;
-    for (hipEvent_t& e : band) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // worker threads wait on these: block instead of spinning, so waiting
+    // threads leave the cores to the chains and hashes
+    const unsigned flags = hipEventDisableTiming | hipEventBlockingSync;
+    for (hipEvent_t* set : {band, fetched, copied})
+      for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
+    HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
+    HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     events = true;
     return DICE_OK;
   }
@@ -165,15 +199,20 @@ Runner::~Runner() {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : fine_)
     if (e) (void)hipEventDestroy(e);
+  if (sync_event_) (void)hipEventDestroy(sync_event_);
+  if (fetch_stream_) (void)hipStreamSynchronize(fetch_stream_);
   ws_.clear();
+  if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int32_t Runner::init() {
   HIP_TRY(hipSetDevice(device_));
   HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking));
   for (hipEvent_t& e : ev_) HIP_TRY(hipEventCreate(&e));
   for (hipEvent_t& e : fine_) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventCreateWithFlags(&sync_event_, hipEventDisableTiming | hipEventBlockingSync));
   HIP_TRY(d_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
   HIP_TRY(h_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
   return DICE_OK;
@@ -218,45 +257,45 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
     const size_t stages = (size_t)H + 6;
     const size_t cells = (size_t)W / 8;
-    HIP_TRY(w.site_lists.ensure(tiles * TY * TX * sizeof(uint16_t)));
-    HIP_TRY(w.tile_count.ensure(tiles * 4));
-    HIP_TRY(w.seg_count.ensure(segments * 4));
-    HIP_TRY(w.seg_offset.ensure(segments * 4));
-    HIP_TRY(w.seg_block_sums.ensure((segments / 1024 + 2) * 4));
-    HIP_TRY(w.row_start.ensure((H + 1) * 4));
-    HIP_TRY(w.counters.ensure(64));
-    HIP_TRY(w.flags.ensure(64));
-    HIP_TRY(w.writer_counts.ensure(kCountSlots * kCountStride * 8));
-    HIP_TRY(w.su_aux.ensure(5 * W * 4));
-    HIP_TRY(w.su_score.ensure(5 * W * 4));
-    HIP_TRY(w.su_work.ensure(5 * W * 12));
-    HIP_TRY(w.su_sites.ensure(6 * W * sizeof(Site)));
-    HIP_TRY(w.su_attempted.ensure(6 * W));
-    HIP_TRY(w.failpos.ensure(((H + 7) / 8) * cells + 8));
-    HIP_TRY(w.row_sums.ensure(H * 3 * 8));
-    HIP_TRY(w.accepted.ensure(H * 4));
-    HIP_TRY(w.epoch_adds.ensure((H / 8) * 16 + 16));
-    HIP_TRY(w.cumulative.ensure(H * 24));
-    HIP_TRY(w.epoch_wide.ensure((H / 8) * 16 + 16));
-    HIP_TRY(w.epoch_store.ensure((H / 8) * 8 + 8));
-    HIP_TRY(w.st_alpha.ensure(stages * 4));
-    HIP_TRY(w.st_offset.ensure(stages * 4));
-    HIP_TRY(w.st_replacement.ensure(stages * 4));
-    HIP_TRY(w.st_has.ensure(stages));
-    HIP_TRY(w.st_score_base.ensure(stages * 4));
-    HIP_TRY(w.st_coarse.ensure(stages * 4));
-    HIP_TRY(w.st_gate.ensure(stages));
-    HIP_TRY(w.st_row_gate.ensure(stages * 4));
-    HIP_TRY(w.su_records.ensure(6 * W * sizeof(WriterRecord)));
+    HIP_TRY(w.site_lists.ensure(tiles * TY * TX * sizeof(uint16_t), stream_));
+    HIP_TRY(w.tile_count.ensure(tiles * 4, stream_));
+    HIP_TRY(w.seg_count.ensure(segments * 4, stream_));
+    HIP_TRY(w.seg_offset.ensure(segments * 4, stream_));
+    HIP_TRY(w.seg_block_sums.ensure((segments / 1024 + 2) * 4, stream_));
+    HIP_TRY(w.row_start.ensure((H + 1) * 4, stream_));
+    HIP_TRY(w.counters.ensure(64, stream_));
+    HIP_TRY(w.flags.ensure(64, stream_));
+    HIP_TRY(w.writer_counts.ensure(kCountSlots * kCountStride * 8, stream_));
+    HIP_TRY(w.su_aux.ensure(5 * W * 4, stream_));
+    HIP_TRY(w.su_score.ensure(5 * W * 4, stream_));
+    HIP_TRY(w.su_work.ensure(5 * W * 12, stream_));
+    HIP_TRY(w.su_sites.ensure(6 * W * sizeof(Site), stream_));
+    HIP_TRY(w.su_attempted.ensure(6 * W, stream_));
+    HIP_TRY(w.failpos.ensure(((H + 7) / 8) * cells + 8, stream_));
+    HIP_TRY(w.row_sums.ensure(H * 3 * 8, stream_));
+    HIP_TRY(w.accepted.ensure(H * 4, stream_));
+    HIP_TRY(w.epoch_adds.ensure((H / 8) * 16 + 16, stream_));
+    HIP_TRY(w.cumulative.ensure(H * 24, stream_));
+    HIP_TRY(w.epoch_wide.ensure((H / 8) * 16 + 16, stream_));
+    HIP_TRY(w.epoch_store.ensure((H / 8) * 8 + 8, stream_));
+    HIP_TRY(w.st_alpha.ensure(stages * 4, stream_));
+    HIP_TRY(w.st_offset.ensure(stages * 4, stream_));
+    HIP_TRY(w.st_replacement.ensure(stages * 4, stream_));
+    HIP_TRY(w.st_has.ensure(stages, stream_));
+    HIP_TRY(w.st_score_base.ensure(stages * 4, stream_));
+    HIP_TRY(w.st_coarse.ensure(stages * 4, stream_));
+    HIP_TRY(w.st_gate.ensure(stages, stream_));
+    HIP_TRY(w.st_row_gate.ensure(stages * 4, stream_));
+    HIP_TRY(w.su_records.ensure(6 * W * sizeof(WriterRecord), stream_));
     HIP_TRY(w.h_small.ensure(128 + kCountSlots * kCountStride * 8));
     HIP_TRY(w.h_row_start.ensure((H + 1) * 4));
     const bool want_scores = R.diag && (R.diag->score_plane || R.diag->at_floor_mask);
     if (want_scores) {
-      HIP_TRY(w.diag_score.ensure(pixels * 4));
-      HIP_TRY(w.diag_floor.ensure(pixels));
+      HIP_TRY(w.diag_score.ensure(pixels * 4, stream_));
+      HIP_TRY(w.diag_floor.ensure(pixels, stream_));
     }
     const bool want_changed = R.diag && R.diag->changed_mask;
-    if (want_changed) HIP_TRY(w.diag_changed.ensure(pixels));
+    if (want_changed) HIP_TRY(w.diag_changed.ensure(pixels, stream_));
     J.lut = tables_->lut.as<float>();
     J.noop_emit = tables_->noop.as<uint16_t>();
     J.factor_high = tables_->high.as<uint32_t>();
@@ -298,14 +337,14 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     if (R.flags & DICE_MAIN_ON_DEVICE) {
       J.rgbi = R.main;
     } else {
-      HIP_TRY(w.rgbi.ensure(pixels * 8));
+      HIP_TRY(w.rgbi.ensure(pixels * 8, stream_));
       HIP_TRY(hipMemcpyAsync(w.rgbi.ptr, R.main, pixels * 8, hipMemcpyHostToDevice, stream_));
       J.rgbi = w.rgbi.as<uint16_t>();
     }
     if (R.flags & DICE_OUT_ON_DEVICE) {
       J.out = R.out;
     } else {
-      HIP_TRY(w.out.ensure(pixels * 6));
+      HIP_TRY(w.out.ensure(pixels * 6, stream_));
       J.out = w.out.as<uint16_t>();
     }
     if (R.stages) {
@@ -329,6 +368,13 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
   }
   HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
                          stream_));
+  return DICE_OK;
+}
+
+// Waits for everything queued on `stream` without spinning a core.
+int32_t Runner::wait_stream(hipStream_t stream) {
+  HIP_TRY(hipEventRecord(sync_event_, stream));
+  HIP_TRY(hipEventSynchronize(sync_event_));
   return DICE_OK;
 }
 
@@ -468,7 +514,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, w.flags.ptr, 4, hipMemcpyDeviceToHost,
                            stream_));
   }
-  HIP_TRY(hipStreamSynchronize(stream_));
+  DICE_TRY(wait_stream(stream_));
   const double t_a = now_seconds();
   float ms_a = 0.0f;
   (void)hipEventElapsedTime(&ms_a, ev_[0], ev_[1]);
@@ -499,9 +545,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     R.cb.progress(3, 0, R.height, 0, 0);
     const size_t count = w.h_small.as<uint32_t>()[0];
     const size_t su = 6 * (size_t)R.width;
-    HIP_TRY(w.sites.ensure((count + 1) * sizeof(Site)));
-    HIP_TRY(w.records.ensure((count + 1) * sizeof(WriterRecord)));
-    HIP_TRY(w.states.ensure((count + su + 1) * 4));
+    HIP_TRY(w.sites.ensure((count + 1) * sizeof(Site), stream_));
+    HIP_TRY(w.records.ensure((count + 1) * sizeof(WriterRecord), stream_));
+    HIP_TRY(w.states.ensure((count + su + 1) * 4, stream_));
     HIP_TRY(w.h_records.ensure((count + su + 1) * sizeof(WriterRecord)));
     HIP_TRY(w.h_states.ensure((count + su + 1) * 4));
     J.sites = w.sites.as<Site>();
@@ -520,35 +566,56 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   HIP_TRY(hipEventRecord(ev_[2], stream_));
   DICE_TRY(phase_b(n, options.graph, &replayed_b));
   HIP_TRY(hipEventRecord(ev_[3], stream_));
+  // row tables and flags first (small), then the draw schedules band by
+  // band, interleaved across jobs on the fetch stream, so every chain can
+  // start as soon as its first band is on the host (output rows go back on
+  // each job's own copy stream, which does not queue behind these)
   for (int j = 0; j < n; ++j) {
     const JobDev& J = H_jobs[j];
     if (J.skip) continue;
     Workspace& w = *ws_[j];
-    const size_t count = w.h_small.as<uint32_t>()[0];
-    const size_t su = 6 * (size_t)J.width;
-    WriterRecord* hr = w.h_records.as<WriterRecord>();
-    HIP_TRY(hipMemcpyAsync(hr, J.su_writer_records, su * sizeof(WriterRecord),
-                           hipMemcpyDeviceToHost, stream_));
-    if (count)
-      HIP_TRY(hipMemcpyAsync(hr + su, J.writer_records, count * sizeof(WriterRecord),
-                             hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipMemcpyAsync(w.h_row_start.ptr, J.row_start, ((size_t)J.height + 1) * 4,
                            hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, w.flags.ptr, 4, hipMemcpyDeviceToHost,
                            stream_));
   }
-  HIP_TRY(hipStreamSynchronize(stream_));
+  DICE_TRY(wait_stream(stream_));
   const double t_b = now_seconds();
   float ms_b = 0.0f;
   (void)hipEventElapsedTime(&ms_b, ev_[2], ev_[3]);
+  int max_bands = 1;
   for (int j = 0; j < n; ++j) {
     JobDev& J = H_jobs[j];
     if (J.skip) continue;
-    const uint32_t flags = *reinterpret_cast<uint32_t*>(ws_[j]->h_small.as<uint8_t>() + 64);
+    Workspace& w = *ws_[j];
+    const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
     if (int32_t rc = flag_error(flags)) {
       jobs[j].code = rc;
       jobs[j].error = error_buffer();
       J.skip = 1;
+      continue;
+    }
+    w.bands = std::max(1, std::min<int>(std::min(options.bands, kMaxBands), J.height));
+    max_bands = std::max(max_bands, w.bands);
+    const size_t su = 6 * (size_t)J.width;
+    HIP_TRY(hipMemcpyAsync(w.h_records.ptr, J.su_writer_records, su * sizeof(WriterRecord),
+                           hipMemcpyDeviceToHost, fetch_stream_));
+    HIP_TRY(hipEventRecord(w.startup_fetched, fetch_stream_));
+  }
+  for (int b = 0; b < max_bands; ++b) {
+    for (int j = 0; j < n; ++j) {
+      const JobDev& J = H_jobs[j];
+      Workspace& w = *ws_[j];
+      if (J.skip || b >= w.bands) continue;
+      const uint32_t* row_start = w.h_row_start.as<uint32_t>();
+      const uint32_t s0 = row_start[band_row(J.height, w.bands, b)];
+      const uint32_t s1 = row_start[band_row(J.height, w.bands, b + 1)];
+      const size_t su = 6 * (size_t)J.width;
+      if (s1 > s0)
+        HIP_TRY(hipMemcpyAsync(w.h_records.as<WriterRecord>() + su + s0, J.writer_records + s0,
+                               (size_t)(s1 - s0) * sizeof(WriterRecord), hipMemcpyDeviceToHost,
+                               fetch_stream_));
+      HIP_TRY(hipEventRecord(w.fetched[b], fetch_stream_));
     }
   }
 
@@ -556,6 +623,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   std::vector<JobThreads> outcome(n);
   std::vector<std::thread> threads;
   std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
+  std::vector<double> chain_waits(n, 0.0), band_waits(n, 0.0);
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
     Workspace& w = *ws_[j];
@@ -564,15 +632,16 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       w.published = 0;
       w.aborted = false;
     }
-    const int bands = std::max(1, std::min<int>(std::min(options.bands, kMaxBands),
-                                                (int)jobs[j].height));
+    const int bands = w.bands;
     // chain thread
-    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds]() {
+    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
+                          &chain_waits]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const JobDev& J = H_jobs[j];
       JobThreads& out = outcome[j];
       const double t0 = now_seconds();
+      double fetch_wait = 0.0;
       auto body = [&]() -> int32_t {
         HIP_TRY(hipSetDevice(device_));
         const int64_t H = R.height, W = R.width;
@@ -582,13 +651,16 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         uint32_t* states = w.h_states.as<uint32_t>();
         dice_replay_result* res = R.result;
         uint32_t state = R.rng_state;
+        const double f0 = now_seconds();
+        HIP_TRY(hipEventSynchronize(w.startup_fetched));
+        fetch_wait += now_seconds() - f0;
         for (int stage = 0; stage < 6; ++stage) {
           uint64_t draws = 0;
           int64_t attempted = 0;
           for (int64_t x = 0; x < W; ++x) {
             const size_t i = (size_t)stage * W + x;
             states[i] = state;
-            attempted += (records[i].flags & kRecordAttempted) ? 1 : 0;
+            attempted += record_attempted(records[i]) ? 1 : 0;
             state = chain_site(records[i], state, draws);
           }
           res->startup_attempted[stage] = attempted;
@@ -603,6 +675,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         for (int b = 0; b < bands; ++b) {
           const uint32_t s0 = row_start[band_row(H, bands, b)];
           const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
+          const double f0 = now_seconds();
+          HIP_TRY(hipEventSynchronize(w.fetched[b]));
+          fetch_wait += now_seconds() - f0;
           for (uint32_t i = s0; i < s1; ++i) {
             main_states[i] = state;
             state = chain_site(main[i], state, advances);
@@ -635,10 +710,11 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       }
       w.cv.notify_all();
       chain_seconds[j] = now_seconds() - t0;
+      chain_waits[j] = fetch_wait;
     });
     // hash / download thread
     threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &hash_seconds,
-                          &copy_seconds]() {
+                          &copy_seconds, &band_waits]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const JobDev& J = H_jobs[j];
@@ -658,18 +734,34 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         }
         const uint8_t* source = reinterpret_cast<const uint8_t*>(J.out);
         Sha256 sha;
+        // Each published band's rows are queued on the copy stream behind
+        // that band's writer values; hashing band b overlaps the copies of
+        // the bands after it (a pageable caller buffer copies synchronously).
+        int queued = 0;
         for (int b = 0; b < bands; ++b) {
+          int published;
+          const double p0 = now_seconds();
           {
             std::unique_lock<std::mutex> lock(w.mutex);
             w.cv.wait(lock, [&] { return w.published > b || w.aborted; });
             if (w.published <= b) return DICE_OK;  // the chain failed; it reports
+            published = w.published;
           }
-          HIP_TRY(hipEventSynchronize(w.band[b]));
+          const double c0 = now_seconds();
+          band_waits[j] += c0 - p0;
+          for (; queued < published; ++queued) {
+            const size_t begin = (size_t)band_row(H, bands, queued) * W * 6;
+            const size_t end = (size_t)band_row(H, bands, queued + 1) * W * 6;
+            HIP_TRY(hipStreamWaitEvent(w.copy_stream, w.band[queued], 0));
+            if (end > begin)
+              HIP_TRY(hipMemcpyAsync(target + begin, source + begin, end - begin,
+                                     hipMemcpyDeviceToHost, w.copy_stream));
+            HIP_TRY(hipEventRecord(w.copied[queued], w.copy_stream));
+          }
+          HIP_TRY(hipEventSynchronize(w.copied[b]));
+          const double c1 = now_seconds();
           const size_t begin = (size_t)band_row(H, bands, b) * W * 6;
           const size_t end = (size_t)band_row(H, bands, b + 1) * W * 6;
-          const double c0 = now_seconds();
-          HIP_TRY(hipMemcpy(target + begin, source + begin, end - begin, hipMemcpyDeviceToHost));
-          const double c1 = now_seconds();
           if (want_sha) sha.update(target + begin, end - begin);
           copying += c1 - c0;
           hashing += now_seconds() - c1;
@@ -684,7 +776,10 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     });
   }
   for (std::thread& t : threads) t.join();
-  HIP_TRY(hipStreamSynchronize(stream_));
+  DICE_TRY(wait_stream(stream_));
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip) DICE_TRY(wait_stream(ws_[j]->copy_stream));
+  DICE_TRY(wait_stream(fetch_stream_));
   const double t_host = now_seconds();
 
   // ---- counts, checks, diagnostics, results
@@ -728,6 +823,8 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       T.producer_device += t_a - t_start;
       T.analysis_device += t_b - t_a;
       T.writer_host += chain_seconds[j];
+      T.download_sites += chain_waits[j];  // chain waiting for its draw schedules
+      T.upload += band_waits[j];           // hash waiting for the chain
       T.download_output += copy_seconds[j];
       T.sha256_host += hash_seconds[j];
       T.finish_device += now_seconds() - t_host;
@@ -743,6 +840,14 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         T.sites_kernel_ms += ms_b;
       }
     }
+  }
+  if (trace_enabled()) {
+    const double t_end = now_seconds();
+    fprintf(stderr,
+            "dice-trace runner=%p jobs=%d start=%.4f phaseA=%.2fms phaseB+fetch-setup=%.2fms "
+            "host=%.2fms finish=%.2fms\n",
+            (void*)this, n, t_start - trace_origin(), (t_a - t_start) * 1e3, (t_b - t_a) * 1e3,
+            (t_host - t_b) * 1e3, (t_end - t_host) * 1e3);
   }
   if (report) {
     report->tile_rows = tile_rows_;
@@ -792,7 +897,7 @@ int32_t Runner::producer_schedule(const uint16_t* rgbi, int64_t height, int64_t 
                    out->scale_numerator,      out->scale_add_denominator, out->scale_add_numerator};
   for (int i = 0; i < 12; ++i)
     HIP_TRY(hipMemcpyAsync(dst[i], sched_[i].ptr, H * sizes[i], hipMemcpyDeviceToHost, stream_));
-  HIP_TRY(hipStreamSynchronize(stream_));
+  DICE_TRY(wait_stream(stream_));
   return DICE_OK;
 }
 
@@ -835,13 +940,13 @@ int32_t Runner::autotune(JobRequest* jobs, int n, RunReport* report) {
             HIP_TRY(hipMemcpyAsync(ws_[j]->h_small.ptr, ws_[j]->counters.ptr, 64,
                                    hipMemcpyDeviceToHost, stream_));
         HIP_TRY(hipEventRecord(ev_[5], stream_));
-        HIP_TRY(hipStreamSynchronize(stream_));
+        DICE_TRY(wait_stream(stream_));
         for (int j = 0; j < n; ++j) {
           if (H_jobs[j].skip) continue;
           Workspace& w = *ws_[j];
           const size_t c = w.h_small.as<uint32_t>()[0];
-          HIP_TRY(w.sites.ensure((c + 1) * sizeof(Site)));
-          HIP_TRY(w.records.ensure((c + 1) * sizeof(WriterRecord)));
+          HIP_TRY(w.sites.ensure((c + 1) * sizeof(Site), stream_));
+          HIP_TRY(w.records.ensure((c + 1) * sizeof(WriterRecord), stream_));
           H_jobs[j].sites = w.sites.as<Site>();
           H_jobs[j].writer_records = w.records.as<WriterRecord>();
         }
@@ -850,7 +955,7 @@ int32_t Runner::autotune(JobRequest* jobs, int n, RunReport* report) {
         HIP_TRY(hipEventRecord(ev_[6], stream_));
         DICE_TRY(launch_b(n));
         HIP_TRY(hipEventRecord(ev_[7], stream_));
-        HIP_TRY(hipStreamSynchronize(stream_));
+        DICE_TRY(wait_stream(stream_));
         float a = 0.0f, b = 0.0f;
         (void)hipEventElapsedTime(&a, ev_[4], ev_[5]);
         (void)hipEventElapsedTime(&b, ev_[6], ev_[7]);

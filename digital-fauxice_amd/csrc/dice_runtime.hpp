@@ -112,15 +112,18 @@ class Runner {
   int32_t launch_a(int n);
   int32_t launch_b(int n);
   int32_t mark(int slot);
+  int32_t wait_stream(hipStream_t stream);
   double fine_ms(int a, int b) const;
   std::string geometry_key(int n, char phase) const;
 
   int device_;
   const DeviceTables* tables_;
   hipStream_t stream_ = nullptr;
+  hipStream_t fetch_stream_ = nullptr;  // draw schedules to the host (chain)
   hipEvent_t ev_[8] = {};
   hipEvent_t fine_[8] = {};
   bool fine_on_ = false;
+  hipEvent_t sync_event_ = nullptr;
   int tile_rows_ = 16, tile_cols_ = 64;
   std::vector<std::unique_ptr<Workspace>> ws_;
   DeviceBuffer d_jobs_;

@@ -298,6 +298,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "engine_submit": (C.c_int32, [VOIDP, C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
     "engine_wait": (C.c_int32, [VOIDP, C.c_uint64, C.c_double]),
     "engine_stats": (C.c_int32, [VOIDP, C.POINTER(DiceEngineStats)]),
+    "engine_reset_stats": (C.c_int32, [VOIDP]),
     "t_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),
     "t_rgb_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),
     "t_dither_chain": (

@@ -129,6 +129,11 @@ class HipServingEngine:
             raise HipBackendUnavailable(self._lib.dice_last_error().decode(errors="replace"))
         return report.as_dict()
 
+    def reset_stats(self) -> None:
+        """Start a new measurement window for ``stats``."""
+
+        self._lib.dice_engine_reset_stats(self._handle)
+
     def close(self) -> None:
         if self._handle is not None:
             self._lib.dice_engine_destroy(self._handle)

@@ -325,6 +325,9 @@ int32_t dice_engine_submit(dice_engine* engine, dice_job* job, uint64_t* ticket)
  * forgotten once waited for. */
 int32_t dice_engine_wait(dice_engine* engine, uint64_t ticket, double timeout_seconds);
 int32_t dice_engine_stats(dice_engine* engine, dice_engine_report* stats);
+/* Starts a new measurement window: counters, latencies and the admission
+ * peak restart from now (jobs in flight stay in flight). */
+int32_t dice_engine_reset_stats(dice_engine* engine);
 
 /* Runs jobs[0..count) (1..64) as one batch.  Returns 0 when every job
  * succeeded, else the first failing job's code (each job's own outcome is in
