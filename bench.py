@@ -59,7 +59,7 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--steps", type=int, default=6, help="timed batches (and plain jobs)")
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
-    parser.add_argument("--drivers", type=int, default=3,
+    parser.add_argument("--drivers", type=int, default=2,
                         help="concurrent batches in the pipelined and serving legs")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")

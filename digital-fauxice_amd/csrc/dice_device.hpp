@@ -20,27 +20,45 @@ struct alignas(8) Site {
 static_assert(sizeof(Site) == 40, "Site record must stay 40 bytes");
 
 // The writer's draw schedule of one site in integer form, one 64-bit word
-// per channel.  A channel that draws takes the LCG step s -> t; with
-// u = t (or its mirror 2^24-1-t when the dither span is negative, so that
-// the dither delta is non-decreasing in u) it keeps t ("copy": the value is
-// decided by this draw) when ((u - a) mod 2^32 < len) != inv, and otherwise
-// redraws once more.  A channel without the floor gate always keeps t
-// (a = 0, len = 2^24); one that can never satisfy the floor test always
-// redraws (len = 0).  Bits: 0-23 a, 24-48 len, 49 inv, 50 mirror, 51 draws;
-// bit 63 of channel 0: the site is attempted (always for main-frame sites).
+// (two 32-bit halves) per channel.  A channel that draws takes the LCG step
+// s -> t; with u = t (or its mirror 2^24-1-t when the dither span is
+// negative, so that the dither delta is non-decreasing in u) it keeps t
+// ("keep": the value is decided by this draw) when
+// ((u - a) mod 2^32 < len) != inv, and otherwise redraws once more.  A
+// channel without the floor gate always keeps t (a = 0, len = 2^24); one
+// that can never satisfy the floor test always redraws (len = 0).
+// Low half: bits 0-23 a, bit 31 (channel 0 only) the site is attempted
+// (always for main-frame sites).  High half: bits 0-24 len, 25 inv,
+// 26 mirror, 27 the channel draws.
 struct WriterRecord {
   uint64_t ch[3];
 };
 static_assert(sizeof(WriterRecord) == 24, "WriterRecord must stay 24 bytes");
 
-constexpr uint64_t kChanInv = 1ull << 49;
-constexpr uint64_t kChanMirror = 1ull << 50;
-constexpr uint64_t kChanDraws = 1ull << 51;
-constexpr uint64_t kRecordAttemptedBit = 1ull << 63;
+constexpr uint32_t kChanLen = 0x1FFFFFFu;
+constexpr uint32_t kChanInv = 1u << 25;
+constexpr uint32_t kChanMirror = 1u << 26;
+constexpr uint32_t kChanDraws = 1u << 27;
+constexpr uint64_t kRecordAttemptedBit = 1ull << 31;
 
 DICE_HD uint64_t channel_word(uint32_t a, uint32_t len, bool inv, bool mirror) {
-  return (uint64_t)(a & 0xFFFFFFu) | ((uint64_t)len << 24) | (inv ? kChanInv : 0ull) |
-         (mirror ? kChanMirror : 0ull) | kChanDraws;
+  const uint32_t high = len | (inv ? kChanInv : 0u) | (mirror ? kChanMirror : 0u) | kChanDraws;
+  return (uint64_t)(a & 0xFFFFFFu) | ((uint64_t)high << 32);
+}
+
+// One channel of the chain on the two halves of its word: the next state,
+// and the draws it took (0, 1 or 2).
+DICE_HD uint32_t chain_channel(uint32_t low, uint32_t high, uint32_t s, uint32_t& draws) {
+  const uint32_t a = low & 0xFFFFFFu;
+  const uint32_t len = high & kChanLen;
+  const uint32_t mirror = (high & kChanMirror) ? kLcgMask : 0u;
+  const bool inv = (high & kChanInv) != 0;
+  const bool drawn = (high & kChanDraws) != 0;
+  const uint32_t t = (125u * s + 1u) & kLcgMask;
+  const uint32_t t2 = (125u * t + 1u) & kLcgMask;
+  const bool keep = (((t ^ mirror) - a) < len) != inv;
+  draws = drawn ? (keep ? 1u : 2u) : 0u;
+  return drawn ? (keep ? t : t2) : s;
 }
 
 constexpr uint32_t kSitePixelMask = 0x7FFFFFFFu;
@@ -137,19 +155,9 @@ DICE_HD uint32_t lcg_step(uint32_t s) { return (125u * s + 1u) & kLcgMask; }
 DICE_HD uint32_t chain_site(const WriterRecord& r, uint32_t s, uint64_t& draws) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const uint64_t w = r.ch[c];
-    const uint32_t a = (uint32_t)(w & 0xFFFFFFu);
-    const uint32_t len = (uint32_t)((w >> 24) & 0x1FFFFFFu);
-    const uint32_t mirror = (w & kChanMirror) ? kLcgMask : 0u;
-    const bool inv = (w & kChanInv) != 0;
-    const bool drawn = (w & kChanDraws) != 0;
-    const uint32_t t = lcg_step(s);
-    const uint32_t t2 = lcg_step(t);
-    const uint32_t u = t ^ mirror;
-    const bool keep = ((u - a) < len) != inv;
-    const uint32_t next = keep ? t : t2;
-    draws += drawn ? (keep ? 1u : 2u) : 0u;
-    s = drawn ? next : s;
+    uint32_t d;
+    s = chain_channel((uint32_t)r.ch[c], (uint32_t)(r.ch[c] >> 32), s, d);
+    draws += d;
   }
   return s;
 }
@@ -190,6 +198,17 @@ struct StageDev {
   int32_t* row_gate;
 };
 
+// The writer chain's carried state and receipt when it runs on the device
+// (one wavefront): the LCG state between launches, the public advances of
+// the main frame and the hidden-startup receipt (startup.py:37-43).
+struct ChainState {
+  uint32_t state;
+  uint32_t startup_final;
+  unsigned long long advances;
+  long long startup_attempted[6];
+  long long startup_advances[6];
+};
+
 struct JobDev {
   DevParams P;                  // stable replay parameters of this job
   WriterConsts K;               // dither bounds and scales of this job
@@ -217,6 +236,8 @@ struct JobDev {
   WriterRecord* su_writer_records; // 6*W
   uint32_t* site_state;         // LCG state at the start of each site
   uint32_t* su_site_state;      // 6*W, the same for the hidden startup rows
+  ChainState* chain;            // the device writer chain's state and receipt
+  uint32_t* band_end;           // LCG state after each writer band
   unsigned long long* writer_counts; // kCountSlots slots of {written, changed}
   uint8_t* diag_changed;        // optional H*W
   uint32_t* counters;           // [0] total sites

@@ -13,6 +13,7 @@
 // overlap the host chain and hash of another.  Every job's result is the
 // one it gets alone: jobs share nothing but the device.
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -89,6 +90,15 @@ void dice_engine::drive(int index) {
       });
       if (stopping && queue.empty()) return;
       if (queue.empty()) continue;
+      // While another batch keeps the device busy, give a partial batch a
+      // short linger to fill up (more jobs per persistent launch); an idle
+      // device starts at once.
+      if ((int)queue.size() < config.max_batch && running > 0 && config.linger_ms > 0.0) {
+        work.wait_for(lock, std::chrono::duration<double, std::milli>(config.linger_ms), [&] {
+          return stopping || (int)queue.size() >= config.max_batch;
+        });
+        if (queue.empty()) continue;
+      }
       while (!queue.empty() && (int)batch.size() < config.max_batch) {
         Ticket* t = queue.front();
         if (admitted != 0 && admitted + t->reserve > config.hbm_budget) break;

@@ -1124,7 +1124,9 @@ __global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
 // one (`end_state` after the last site of the range).
 __global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, uint32_t begin,
                                                             uint32_t end,
-                                                            uint32_t end_state) {
+                                                            uint32_t end_state_value,
+                                                            const uint32_t* end_state_at) {
+  const uint32_t end_state = end_state_at ? *end_state_at : end_state_value;
   const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   bool written = false, changed = false;
   if (i < end) {
@@ -1179,7 +1181,9 @@ __global__ __launch_bounds__(kThreads) void k_writer_values(JobDev J, uint32_t b
 // each attempted site, run from its recorded start state, must end in the
 // state recorded for the next index (a skipped index passes the state on).
 __global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J,
-                                                                   uint32_t final_state) {
+                                                                   uint32_t final_value,
+                                                                   const uint32_t* final_at) {
+  const uint32_t final_state = final_at ? *final_at : final_value;
   const uint32_t count = 6u * (uint32_t)J.width;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -1191,6 +1195,109 @@ __global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J,
   }
   const uint32_t expected = i + 1 < count ? J.su_site_state[i + 1] : final_state;
   if (state != expected) atomicOr(J.error_flags, kErrWriterChain);
+}
+
+// The writer's serial LCG chain on one wavefront (host_writer.py:73-93,
+// reconstruction.py:916-982 through chain_channel): records [begin, end)
+// of one job, in order, from the state carried in J.chain.  Every value the
+// chain touches is wave-uniform (read through readfirstlane), so the chain
+// runs on the scalar unit.  The 64 lanes stage the records through LDS (the
+// next chunk loads while the current one is chained), the next site's words
+// are read one site ahead, and the start state of every site is collected
+// one per lane and stored 64 at a time.  kStartup: the six hidden rows
+// (6*W slots, with the per-stage receipt); otherwise main sites, and the
+// state after the range goes to band_end[band].
+constexpr int kWriterChunk = 512;
+
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+template <bool kStartup>
+__global__ __launch_bounds__(64) void k_writer_chain(JobDev J, uint32_t begin, uint32_t end,
+                                                     int band) {
+  __shared__ uint32_t words[2][kWriterChunk * 6];
+  const int lane = threadIdx.x;
+  const WriterRecord* records = kStartup ? J.su_writer_records : J.writer_records;
+  uint32_t* states = kStartup ? J.su_site_state : J.site_state;
+  const uint32_t W = (uint32_t)J.width;
+  uint32_t state = uniform32(J.chain->state);
+  unsigned long long advances = 0;
+  long long stage_attempted = 0, stage_draws = 0;
+  uint32_t stage = 0, stage_end = W;
+  constexpr int kPer = kWriterChunk * 6 / 64;
+  uint32_t next[kPer];
+  auto fetch = [&](uint32_t first) {
+    const uint32_t* source = reinterpret_cast<const uint32_t*>(records + first);
+    const uint32_t n = min((uint32_t)kWriterChunk, end - first) * 6;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t e = (uint32_t)(lane + j * 64);
+      next[j] = e < n ? source[e] : 0u;
+    }
+  };
+  if (begin < end) fetch(begin);
+  int buffer = 0;
+  for (uint32_t first = begin; first < end; first += kWriterChunk) {
+    const uint32_t n = min((uint32_t)kWriterChunk, end - first);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) words[buffer][lane + j * 64] = next[j];
+    __syncthreads();
+    if (first + kWriterChunk < end) fetch(first + kWriterChunk);
+    const uint32_t* w = words[buffer];
+    uint32_t collected = 0;
+    uint32_t v0 = w[0], v1 = w[1], v2 = w[2], v3 = w[3], v4 = w[4], v5 = w[5];
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t l0 = uniform32(v0), h0 = uniform32(v1);
+      const uint32_t l1 = uniform32(v2), h1 = uniform32(v3);
+      const uint32_t l2 = uniform32(v4), h2 = uniform32(v5);
+      const uint32_t ahead = (k + 1 < n ? k + 1 : k) * 6;  // next site's words, in flight
+      v0 = w[ahead];
+      v1 = w[ahead + 1];
+      v2 = w[ahead + 2];
+      v3 = w[ahead + 3];
+      v4 = w[ahead + 4];
+      v5 = w[ahead + 5];
+      if (kStartup && first + k == stage_end) {  // next hidden stage
+        if (lane == 0) {
+          J.chain->startup_attempted[stage] = stage_attempted;
+          J.chain->startup_advances[stage] = stage_draws;
+        }
+        stage_attempted = 0;
+        stage_draws = 0;
+        ++stage;
+        stage_end += W;
+      }
+      collected = lane == (int)(k & 63u) ? state : collected;
+      uint32_t d0, d1, d2;
+      state = chain_channel(l0, h0, state, d0);
+      state = chain_channel(l1, h1, state, d1);
+      state = chain_channel(l2, h2, state, d2);
+      if (kStartup) {
+        stage_draws += (long long)(d0 + d1 + d2);
+        stage_attempted += (l0 >> 31) & 1u;
+      } else {
+        advances += d0 + d1 + d2;
+      }
+      if ((k & 63u) == 63u || k + 1 == n) {
+        const uint32_t base = first + (k & ~63u);
+        if (lane <= (int)(k & 63u)) states[base + lane] = collected;
+      }
+    }
+    __syncthreads();
+    buffer ^= 1;
+  }
+  if (lane == 0) {
+    if (kStartup) {
+      J.chain->startup_attempted[stage] = stage_attempted;
+      J.chain->startup_advances[stage] = stage_draws;
+      J.chain->startup_final = state;
+    } else {
+      J.chain->advances += advances;
+      J.band_end[band] = state;
+    }
+    J.chain->state = state;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
@@ -1534,17 +1641,27 @@ hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream) {
 }
 
 hipError_t launch_writer_values(const JobDev& J, uint32_t begin, uint32_t end,
-                                uint32_t end_state, hipStream_t stream) {
+                                uint32_t end_state, const uint32_t* end_state_at,
+                                hipStream_t stream) {
   if (end <= begin) return hipSuccess;
   hipLaunchKernelGGL(k_writer_values, dim3(blocks_for(end - begin, kThreads)), dim3(kThreads),
-                     0, stream, J, begin, end, end_state);
+                     0, stream, J, begin, end, end_state, end_state_at);
   return hipGetLastError();
 }
 
 hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
-                                       hipStream_t stream) {
+                                       const uint32_t* final_at, hipStream_t stream) {
   hipLaunchKernelGGL(k_writer_startup_check, dim3(blocks_for(6L * J.width, kThreads)),
-                     dim3(kThreads), 0, stream, J, final_state);
+                     dim3(kThreads), 0, stream, J, final_state, final_at);
+  return hipGetLastError();
+}
+
+hipError_t launch_writer_chain(const JobDev& J, uint32_t begin, uint32_t end, bool startup,
+                               int band, hipStream_t stream) {
+  if (startup)
+    hipLaunchKernelGGL(k_writer_chain<true>, dim3(1), dim3(64), 0, stream, J, begin, end, band);
+  else
+    hipLaunchKernelGGL(k_writer_chain<false>, dim3(1), dim3(64), 0, stream, J, begin, end, band);
   return hipGetLastError();
 }
 

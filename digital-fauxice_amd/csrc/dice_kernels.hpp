@@ -43,11 +43,18 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipSt
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                           hipStream_t stream);
 hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream);
-// Per job: the writer over sites [begin, end) and the startup check.
+// Per job: the writer over sites [begin, end) and the startup check; the
+// state after the range is end_state, or *end_state_at when that is given
+// (device memory written by the device chain).
 hipError_t launch_writer_values(const JobDev& J, uint32_t begin, uint32_t end,
-                                uint32_t end_state, hipStream_t stream);
+                                uint32_t end_state, const uint32_t* end_state_at,
+                                hipStream_t stream);
 hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
-                                       hipStream_t stream);
+                                       const uint32_t* final_at, hipStream_t stream);
+// The serial writer chain on one wavefront over records [begin, end) of one
+// job (startup: the 6*W hidden slots), carrying its state in J.chain.
+hipError_t launch_writer_chain(const JobDev& J, uint32_t begin, uint32_t end, bool startup,
+                               int band, hipStream_t stream);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t stream);
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -668,7 +668,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         }
         res->startup_final_rng_state = state;
         HIP_TRY(hipMemcpyAsync(J.su_site_state, states, su * 4, hipMemcpyHostToDevice, stream_));
-        HIP_TRY(launch_writer_startup_check(J, state, stream_));
+        HIP_TRY(launch_writer_startup_check(J, state, nullptr, stream_));
         uint64_t advances = 0;
         const WriterRecord* main = records + su;
         uint32_t* main_states = states + su;
@@ -685,7 +685,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
           if (s1 > s0) {
             HIP_TRY(hipMemcpyAsync(J.site_state + s0, main_states + s0, (size_t)(s1 - s0) * 4,
                                    hipMemcpyHostToDevice, stream_));
-            HIP_TRY(launch_writer_values(J, s0, s1, state, stream_));
+            HIP_TRY(launch_writer_values(J, s0, s1, state, nullptr, stream_));
           }
           HIP_TRY(hipEventRecord(w.band[b], stream_));
           {

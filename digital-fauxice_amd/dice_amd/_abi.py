@@ -214,6 +214,7 @@ class DiceEngineConfig(C.Structure):
         ("max_batch", C.c_int32),
         ("batch_flags", C.c_uint32),
         ("hbm_budget", C.c_uint64),
+        ("linger_ms", C.c_double),
     ]
 
 

@@ -40,12 +40,13 @@ class HipServingEngine:
     """Submit / wait front end of the serving engine."""
 
     def __init__(self, *, device: int = 0, drivers: int = 2, max_batch: int = 8,
-                 hbm_budget: int = 0, graph: bool = True, autotune: bool = False) -> None:
+                 hbm_budget: int = 0, graph: bool = True, autotune: bool = False,
+                 linger_ms: float = 2.0) -> None:
         self._lib = library()
         config = DiceEngineConfig(device, drivers, max_batch,
                                   (_abi.DICE_BATCH_GRAPH if graph else 0)
                                   | (_abi.DICE_BATCH_AUTOTUNE if autotune else 0),
-                                  hbm_budget)
+                                  hbm_budget, linger_ms)
         handle = C.c_void_p()
         code = self._lib.dice_engine_create(C.byref(config), C.byref(handle))
         if code != _abi.DICE_OK:

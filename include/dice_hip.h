@@ -302,6 +302,8 @@ typedef struct dice_engine_config {
   int32_t max_batch;        /* jobs per batch, 1..64 (0: 8) */
   uint32_t batch_flags;     /* DICE_BATCH_GRAPH | DICE_BATCH_AUTOTUNE */
   uint64_t hbm_budget;      /* bytes of job reservations admitted at once (0: 80% of free HBM) */
+  double linger_ms;         /* a partial batch waits this long for more jobs while another
+                               batch is running (0: never waits) */
 } dice_engine_config;
 
 typedef struct dice_engine_report {
