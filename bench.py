@@ -59,6 +59,8 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--steps", type=int, default=6, help="timed batches (and plain jobs)")
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
+    parser.add_argument("--device-chain", action="store_true",
+                        help="batched legs run each job's writer chain on one device wavefront")
     parser.add_argument("--drivers", type=int, default=2,
                         help="concurrent batches in the pipelined and serving legs")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
@@ -203,6 +205,8 @@ def main() -> int:
         return [fingerprint(r) for r in results], report
 
     batch_flags = _abi.DICE_BATCH_GRAPH | _abi.DICE_BATCH_AUTOTUNE
+    if args.device_chain:
+        batch_flags |= _abi.DICE_BATCH_DEVICE_CHAIN
     for _ in range(max(args.warmup, 2)):  # the second run captures the graphs
         run_batch(full, batch_flags)
     synchronize()
@@ -230,6 +234,7 @@ def main() -> int:
     jobs_total = args.steps * args.batch
     pipe_outs = [DeviceArray(height * width * 6) for _ in range(jobs_total)]
     with HipServingEngine(drivers=args.drivers, max_batch=args.batch, graph=True,
+                          device_chain=args.device_chain,
                           autotune=True) as served:
         warm = [served.submit_device(full.d_main[k % args.batch].address, height, width,
                                      full.prepasses[k % args.batch],
@@ -317,7 +322,8 @@ def main() -> int:
             plain_job_lat.append((time.perf_counter() - a) * 1e3)
             plain_prints.append(fingerprint(res))
         plain_mix_elapsed = time.perf_counter() - a0
-        with HipServingEngine(drivers=args.drivers, max_batch=8, graph=False) as served:
+        with HipServingEngine(drivers=args.drivers, max_batch=8, graph=False,
+                              device_chain=args.device_chain) as served:
             def closed_loop():
                 a = time.perf_counter()
                 ts = []
@@ -465,6 +471,7 @@ def main() -> int:
                          "launches over each batch, graph replay, autotuned tiles; SHA-256 of "
                          "every output"),
             "drivers": args.drivers,
+            "writer_chain": "device wavefront" if args.device_chain else "host core",
             "global_batch": args.batch,
             "height": height,
             "width": width,

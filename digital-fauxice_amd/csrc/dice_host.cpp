@@ -351,6 +351,7 @@ int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t bat
   if (batch_flags & DICE_BATCH_AUTOTUNE) DICE_TRY(runner.autotune(requests.data(), count, &rr));
   RunOptions options;
   options.graph = (batch_flags & DICE_BATCH_GRAPH) != 0;
+  options.device_chain = (batch_flags & DICE_BATCH_DEVICE_CHAIN) != 0;
   DICE_TRY(runner.run(requests.data(), count, options, &rr));
   int32_t first_error = DICE_OK;
   for (int32_t j = 0; j < count; ++j) {

@@ -127,6 +127,8 @@ struct Workspace {
   DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse, st_gate,
       st_row_gate;
   DeviceBuffer sites, records, su_records, states, writer_counts;
+  DeviceBuffer chain;        // ChainState, then the LCG state after each band
+  PinnedBuffer h_chain;      // the same on the host
   PinnedBuffer h_small;      // [0,64) counters, [64,128) flags, [128, ...) writer slots
   PinnedBuffer h_records;    // 6W startup records, then the main records
   PinnedBuffer h_row_start;  // H+1
@@ -139,6 +141,8 @@ struct Workspace {
   // this job's output rows to the host: its own in-order stream, so a copy
   // waiting for this job's next band never holds up another job's copies
   hipStream_t copy_stream = nullptr;
+  // the device writer chain of this job and its writer values, band by band
+  hipStream_t chain_stream = nullptr;
   int bands = 1;
   bool events = false;
   // chain -> hash hand-off
@@ -152,9 +156,10 @@ struct Workspace {
       for (int b = 0; b < kMaxBands; ++b)
         if (set[b]) (void)hipEventDestroy(set[b]);
     if (startup_fetched) (void)hipEventDestroy(startup_fetched);
-    if (copy_stream) {
-      (void)hipStreamSynchronize(copy_stream);
-      (void)hipStreamDestroy(copy_stream);
+    for (hipStream_t stream : {copy_stream, chain_stream}) {
+      if (!stream) continue;
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
     }
   }
   int32_t init_events() {
@@ -166,6 +171,7 @@ struct Workspace {
       for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
     HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
     HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&chain_stream, hipStreamNonBlocking));
     events = true;
     return DICE_OK;
   }
@@ -177,7 +183,7 @@ struct Workspace {
                                  &epoch_adds, &cumulative, &epoch_wide, &epoch_store,
                                  &st_alpha, &st_offset, &st_replacement, &st_has,
                                  &st_score_base, &st_coarse, &st_gate, &st_row_gate, &sites,
-                                 &records, &su_records, &states, &writer_counts};
+                                 &records, &su_records, &states, &writer_counts, &chain};
     uint64_t total = 0;
     for (const DeviceBuffer* b : all) total += b->bytes;
     return total;
@@ -287,6 +293,8 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     HIP_TRY(w.st_gate.ensure(stages, stream_));
     HIP_TRY(w.st_row_gate.ensure(stages * 4, stream_));
     HIP_TRY(w.su_records.ensure(6 * W * sizeof(WriterRecord), stream_));
+    HIP_TRY(w.chain.ensure(sizeof(ChainState) + kMaxBands * 4, stream_));
+    HIP_TRY(w.h_chain.ensure(sizeof(ChainState) + kMaxBands * 4));
     HIP_TRY(w.h_small.ensure(128 + kCountSlots * kCountStride * 8));
     HIP_TRY(w.h_row_start.ensure((H + 1) * 4));
     const bool want_scores = R.diag && (R.diag->score_plane || R.diag->at_floor_mask);
@@ -326,6 +334,8 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     J.su_sites = w.su_sites.as<Site>();
     J.su_attempted = w.su_attempted.as<uint8_t>();
     J.su_writer_records = w.su_records.as<WriterRecord>();
+    J.chain = w.chain.as<ChainState>();
+    J.band_end = reinterpret_cast<uint32_t*>(w.chain.as<uint8_t>() + sizeof(ChainState));
     J.failpos = w.failpos.as<uint8_t>();
     J.row_sums = w.row_sums.as<double>();
     J.accepted = w.accepted.as<uint32_t>();
@@ -597,12 +607,13 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     }
     w.bands = std::max(1, std::min<int>(std::min(options.bands, kMaxBands), J.height));
     max_bands = std::max(max_bands, w.bands);
+    if (options.device_chain) continue;  // the schedules stay on the device
     const size_t su = 6 * (size_t)J.width;
     HIP_TRY(hipMemcpyAsync(w.h_records.ptr, J.su_writer_records, su * sizeof(WriterRecord),
                            hipMemcpyDeviceToHost, fetch_stream_));
     HIP_TRY(hipEventRecord(w.startup_fetched, fetch_stream_));
   }
-  for (int b = 0; b < max_bands; ++b) {
+  for (int b = 0; b < max_bands && !options.device_chain; ++b) {
     for (int j = 0; j < n; ++j) {
       const JobDev& J = H_jobs[j];
       Workspace& w = *ws_[j];
@@ -633,6 +644,48 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       w.aborted = false;
     }
     const int bands = w.bands;
+    if (options.device_chain) {
+      // the serial chain on one wavefront of the device, band by band on
+      // this job's chain stream after phase B; every band is published at
+      // once (its event orders the hash thread's copy behind it)
+      const double t0 = now_seconds();
+      const JobDev& J = H_jobs[j];
+      const JobRequest& R = jobs[j];
+      const uint32_t* row_start = w.h_row_start.as<uint32_t>();
+      ChainState* init = w.h_chain.as<ChainState>();
+      memset(init, 0, sizeof(ChainState));
+      init->state = R.rng_state;
+      hipStream_t cs = w.chain_stream;
+      auto enqueue = [&]() -> int32_t {
+      HIP_TRY(hipStreamWaitEvent(cs, ev_[3], 0));
+      HIP_TRY(hipMemcpyAsync(J.chain, init, sizeof(ChainState), hipMemcpyHostToDevice, cs));
+      HIP_TRY(launch_writer_chain(J, 0, 6u * (uint32_t)J.width, true, 0, cs));
+      HIP_TRY(launch_writer_startup_check(J, 0, &J.chain->startup_final, cs));
+      for (int b = 0; b < bands; ++b) {
+        const uint32_t s0 = row_start[band_row(J.height, bands, b)];
+        const uint32_t s1 = row_start[band_row(J.height, bands, b + 1)];
+        HIP_TRY(launch_writer_chain(J, s0, s1, false, b, cs));
+        HIP_TRY(launch_writer_values(J, s0, s1, 0, J.band_end + b, cs));
+        HIP_TRY(hipEventRecord(w.band[b], cs));
+      }
+      HIP_TRY(hipMemcpyAsync(w.h_chain.ptr, J.chain, sizeof(ChainState), hipMemcpyDeviceToHost,
+                             cs));
+      HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 128, J.writer_counts,
+                             kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, cs));
+      HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
+                             hipMemcpyDeviceToHost, cs));
+      return DICE_OK;
+      };
+      if (int32_t rc = enqueue()) {  // the hash thread must not wait for it
+        outcome[j].fail(rc);
+        std::lock_guard<std::mutex> lock(w.mutex);
+        w.aborted = true;
+      } else {
+        std::lock_guard<std::mutex> lock(w.mutex);
+        w.published = bands;
+      }
+      chain_seconds[j] = now_seconds() - t0;
+    } else {
     // chain thread
     threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
                           &chain_waits]() {
@@ -712,6 +765,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       chain_seconds[j] = now_seconds() - t0;
       chain_waits[j] = fetch_wait;
     });
+    }
     // hash / download thread
     threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &hash_seconds,
                           &copy_seconds, &band_waits]() {
@@ -777,8 +831,11 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   }
   for (std::thread& t : threads) t.join();
   DICE_TRY(wait_stream(stream_));
-  for (int j = 0; j < n; ++j)
-    if (!H_jobs[j].skip) DICE_TRY(wait_stream(ws_[j]->copy_stream));
+  for (int j = 0; j < n; ++j) {
+    if (H_jobs[j].skip) continue;
+    DICE_TRY(wait_stream(ws_[j]->copy_stream));
+    if (options.device_chain) DICE_TRY(wait_stream(ws_[j]->chain_stream));
+  }
   DICE_TRY(wait_stream(fetch_stream_));
   const double t_host = now_seconds();
 
@@ -808,6 +865,17 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     }
     R.result->written_pixels = written;
     R.result->changed_pixels = changed;
+    if (options.device_chain) {
+      const ChainState& chain = *w.h_chain.as<ChainState>();
+      for (int stage = 0; stage < 6; ++stage) {
+        R.result->startup_attempted[stage] = chain.startup_attempted[stage];
+        R.result->startup_advances[stage] = chain.startup_advances[stage];
+      }
+      R.result->startup_final_rng_state = chain.startup_final;
+      R.result->attempted_pixels = w.h_row_start.as<uint32_t>()[R.height];
+      R.result->public_rng_advances = chain.advances;
+      R.result->final_rng_state = chain.state;
+    }
     const size_t pixels = (size_t)R.height * R.width;
     if (R.diag) {
       if (R.diag->score_plane)

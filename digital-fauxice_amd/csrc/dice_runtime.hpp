@@ -55,6 +55,7 @@ struct Workspace;
 struct RunOptions {
   bool graph = false;        // replay phases A/B from captured graphs
   bool fine_timing = false;  // HIP events between kernel groups (direct launches only)
+  bool device_chain = false; // the serial writer chain on one device wavefront per job
   int bands = 8;             // writer/hash pipeline bands per job
 };
 

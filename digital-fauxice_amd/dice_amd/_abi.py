@@ -26,6 +26,7 @@ DICE_OUT_ON_DEVICE = 0x2
 DICE_SKIP_SHA256 = 0x4
 DICE_BATCH_GRAPH = 0x100
 DICE_BATCH_AUTOTUNE = 0x200
+DICE_BATCH_DEVICE_CHAIN = 0x400
 
 f32p = C.POINTER(C.c_float)
 f64p = C.POINTER(C.c_double)

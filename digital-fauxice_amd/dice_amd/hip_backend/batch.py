@@ -48,8 +48,13 @@ def _exception_for(code: int, message: str) -> BaseException:
 
 
 def process_batch_hip(jobs: list[ProcessingJob], *, graph: bool = False,
-                      autotune: bool = False) -> BatchOutcome:
-    """Run ``jobs`` (1..64) as one batch on the HIP device."""
+                      autotune: bool = False, device_chain: bool = False) -> BatchOutcome:
+    """Run ``jobs`` (1..64) as one batch on the HIP device.
+
+    ``device_chain`` runs each job's serial writer chain on one device
+    wavefront instead of a host core (same results; frees the host and the
+    host link for throughput).
+    """
 
     if not 1 <= len(jobs) <= 64:
         raise ValueError("a batch holds 1..64 jobs")
@@ -84,7 +89,8 @@ def process_batch_hip(jobs: list[ProcessingJob], *, graph: bool = False,
         entry.out_rgb16 = out.ctypes.data
         entry.result = C.pointer(res)
         entry.timings = C.pointer(tim)
-    flags = (_abi.DICE_BATCH_GRAPH if graph else 0) | (_abi.DICE_BATCH_AUTOTUNE if autotune else 0)
+    flags = ((_abi.DICE_BATCH_GRAPH if graph else 0) | (_abi.DICE_BATCH_AUTOTUNE if autotune else 0)
+             | (_abi.DICE_BATCH_DEVICE_CHAIN if device_chain else 0))
     report = DiceBatchReport()
     code = library().dice_process_batch(context(), array, len(jobs), flags, C.byref(report))
     outcome = BatchOutcome(report={name: getattr(report, name) for name, _ in report._fields_})

@@ -41,11 +41,12 @@ class HipServingEngine:
 
     def __init__(self, *, device: int = 0, drivers: int = 2, max_batch: int = 8,
                  hbm_budget: int = 0, graph: bool = True, autotune: bool = False,
-                 linger_ms: float = 2.0) -> None:
+                 linger_ms: float = 2.0, device_chain: bool = False) -> None:
         self._lib = library()
         config = DiceEngineConfig(device, drivers, max_batch,
                                   (_abi.DICE_BATCH_GRAPH if graph else 0)
-                                  | (_abi.DICE_BATCH_AUTOTUNE if autotune else 0),
+                                  | (_abi.DICE_BATCH_AUTOTUNE if autotune else 0)
+                                  | (_abi.DICE_BATCH_DEVICE_CHAIN if device_chain else 0),
                                   hbm_budget, linger_ms)
         handle = C.c_void_p()
         code = self._lib.dice_engine_create(C.byref(config), C.byref(handle))

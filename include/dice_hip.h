@@ -250,6 +250,8 @@ typedef struct dice_job {
 
 #define DICE_BATCH_GRAPH 0x100u    /* replay the device phases from captured hipGraphs */
 #define DICE_BATCH_AUTOTUNE 0x200u /* time every tile shape on this batch geometry first */
+#define DICE_BATCH_DEVICE_CHAIN 0x400u /* run each job's serial writer chain on one device
+                                          wavefront instead of a host core */
 
 typedef struct dice_batch_report {
   int32_t tile_rows, tile_cols;   /* analysis tile shape used */

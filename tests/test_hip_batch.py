@@ -109,6 +109,23 @@ def test_batch_graph_replays_match_eager(frames, oracle_fingerprints):
     assert replayed[0] == 0 and all(replayed[2:]), replayed
 
 
+def test_device_chain_matches_host_chain(frames, oracle_fingerprints):
+    """The single-wavefront device chain reproduces every counter, the
+    startup receipt and the output of the host chain."""
+
+    jobs = _jobs(frames)
+    host = process_batch_hip(jobs)
+    host.raise_first()
+    for attempt in range(2):
+        device = process_batch_hip(jobs, device_chain=True, graph=attempt == 1)
+        device.raise_first()
+        for i in range(len(frames)):
+            got = replay_fingerprint(device.results[i].replay)
+            assert got == replay_fingerprint(host.results[i].replay), f"job {i} run {attempt}"
+            assert _core(got) == oracle_fingerprints[i]
+            assert np.array_equal(device.results[i].output_rgb16, host.results[i].output_rgb16)
+
+
 def test_batch_autotune_keeps_results(frames, oracle_fingerprints):
     outcome = process_batch_hip(_jobs(frames), autotune=True)
     outcome.raise_first()

@@ -89,6 +89,14 @@ def test_results_alone_and_in_the_mix_are_identical(frames, oracle_shas):
     assert stats["latency_p99_ms"] >= stats["latency_p50_ms"] > 0
 
 
+def test_engine_with_device_chain(frames, oracle_shas):
+    with HipServingEngine(drivers=2, max_batch=4, device_chain=True) as engine:
+        tickets = [(i, engine.submit(make_job(m, p, f"device-chain-{i}")))
+                   for i, (m, p) in enumerate(frames)]
+        for i, t in tickets:
+            assert engine.wait(t, timeout=120).replay.output_sha256 == oracle_shas[i]
+
+
 def test_budget_admits_one_reservation_at_a_time(frames, oracle_shas):
     largest = max(m.shape[0] * m.shape[1] for m, _ in frames)
     # about one largest job's reservation: admission must queue the rest
