@@ -316,76 +316,96 @@ __global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobSet set
   });
 }
 
-// Per-row binary64 sums in column order (producer_parameters.py:303-360):
-// a workgroup owns kRowSumRows rows.  Four staging waves load 128-column
-// chunks (coalesced), apply the cell latch and gather the two LUT values;
-// one summing wave runs the row chains (one lane per row) over the
-// previous chunk, so chunk loads overlap the dependent adds.  A rejected
-// pixel is staged as weight 0, whose three products are +0.0: the exact
-// identity of the non-negative sums.  Three LDS buffers keep the staging
-// of chunk k+1 clear of the chunk still being summed.
+// Per-row binary64 sums in column order (producer_parameters.py:303-360).
+// A workgroup owns kRowSumRows rows.  Lane r*3+q of the summing wave
+// carries chain q (visible, infrared, weight) of row r, so a column step is
+// one LDS read and one dependent add per lane.  Eight staging waves turn
+// each 128-column chunk into the three binary64 terms of every pixel --
+// weight * lut(visible), weight * lut(infrared) and weight, each a single
+// rounded product whichever thread forms it -- and store them column-major,
+// the summing wave's read order.  A rejected pixel has weight 0 and three
+// +0.0 terms, the exact identity of the non-negative sums (so are the
+// padding columns past the active width).  Every staging load is issued
+// unconditionally at a clamped address; the next chunk's pixels are
+// requested after the current chunk's gathers, so they stay in flight while
+// the current chunk is converted.  Two LDS buffers alternate, ordered by the
+// one barrier per chunk.
 constexpr int kRowSumRows = 16;
+constexpr int kRowSumLanes = kRowSumRows * 3;
+constexpr int kRowSumPitch = kRowSumLanes + 1;  // odd pitch: conflict-free column stores
 constexpr int kRowSumCols = 128;
-constexpr int kRowSumStagers = 256;
+constexpr int kRowSumStagers = 512;
+constexpr int kRowSumPer = kRowSumRows * kRowSumCols / kRowSumStagers;
+static_assert(kRowSumRows * kRowSumCols % kRowSumStagers == 0, "whole pixels per stager");
 
 __global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobSet set) {
-  __shared__ float s_visible[3][kRowSumRows][kRowSumCols + 1];
-  __shared__ float s_infrared[3][kRowSumRows][kRowSumCols + 1];
-  __shared__ uint32_t s_weight[3][kRowSumRows][kRowSumCols + 1];
+  __shared__ double s_terms[2][kRowSumCols][kRowSumPitch];
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const int H = J.height, W = J.width;
     const int cells = W / 8, active = cells * 8;
     const int row0 = unit * kRowSumRows;
     const int tid = threadIdx.x;
     const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
-    double sum_visible = 0.0, sum_infrared = 0.0, sum_weight = 0.0;
+    const int stager = tid - 64;
+    uint2 raw[kRowSumPer];
+    auto fetch = [&](int k) {
+#pragma unroll
+      for (int i = 0; i < kRowSumPer; ++i) {
+        const int e = stager + i * kRowSumStagers;
+        const int y = min(row0 + e / kRowSumCols, H - 1);
+        const int x = min(k * kRowSumCols + e % kRowSumCols, active - 1);
+        raw[i] = *reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + x) * 4);
+      }
+    };
+    if (stager >= 0 && chunks > 0) fetch(0);
+    double sum = 0.0;
     uint32_t accepted = 0;
     for (int k = 0; k <= chunks; ++k) {
-      if (tid >= 64 && k < chunks) {
-        const int buffer = k % 3, c0 = k * kRowSumCols;
-        constexpr int kPer = kRowSumRows * kRowSumCols / kRowSumStagers;
+      if (stager >= 0 && k < chunks) {
+        uint32_t fail[kRowSumPer], infrared[kRowSumPer];
+        float lut_visible[kRowSumPer], lut_infrared[kRowSumPer];
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-          const int e = (tid - 64) + i * kRowSumStagers;
+        for (int i = 0; i < kRowSumPer; ++i) {
+          const int e = stager + i * kRowSumStagers;
+          const int y = min(row0 + e / kRowSumCols, H - 1);
+          const int x = min(k * kRowSumCols + e % kRowSumCols, active - 1);
+          infrared[i] = raw[i].y >> 16;
+          fail[i] = J.failpos[(size_t)(y >> 3) * cells + (x >> 3)];
+          lut_visible[i] = J.lut[raw[i].x & 0xFFFFu];
+          lut_infrared[i] = J.lut[infrared[i]];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(min(k + 1, chunks - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        double(*terms)[kRowSumPitch] = s_terms[k & 1];
+#pragma unroll
+        for (int i = 0; i < kRowSumPer; ++i) {
+          const int e = stager + i * kRowSumStagers;
           const int r = e / kRowSumCols, c = e % kRowSumCols;
-          const int y = row0 + r, x = c0 + c;
-          float lut_visible = 0.0f, lut_infrared = 0.0f;
-          uint32_t weight = 0;
-          if (y < H && x < active) {
-            uint32_t lanes[4];
-            load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-            const int position = (y & 7) * 8 + (x & 7);
-            if (position < (int)J.failpos[(size_t)(y >> 3) * cells + (x >> 3)]) {
-              lut_visible = J.lut[lanes[0]];
-              lut_infrared = J.lut[lanes[3]];
-              weight = lanes[3] * lanes[3];
-            }
-          }
-          s_visible[buffer][r][c] = lut_visible;
-          s_infrared[buffer][r][c] = lut_infrared;
-          s_weight[buffer][r][c] = weight;
+          const int y = row0 + r, x = k * kRowSumCols + c;
+          const int position = (y & 7) * 8 + (x & 7);
+          const bool keep = y < H && x < active && position < (int)fail[i];
+          const double weight = keep ? (double)(infrared[i] * infrared[i]) : 0.0;
+          terms[c][r * 3 + 0] = weight * (double)lut_visible[i];
+          terms[c][r * 3 + 1] = weight * (double)lut_infrared[i];
+          terms[c][r * 3 + 2] = weight;
+        }
+      }
+      if (tid < kRowSumLanes && k >= 1) {
+        double(*terms)[kRowSumPitch] = s_terms[(k - 1) & 1];
+#pragma unroll 16
+        for (int c = 0; c < kRowSumCols; ++c) {
+          const double term = terms[c][tid];
+          sum += term;
+          accepted += term != 0.0;
         }
       }
       __syncthreads();
-      if (tid < kRowSumRows && k >= 1) {
-        const int buffer = (k - 1) % 3;
-#pragma unroll 8
-        for (int c = 0; c < kRowSumCols; ++c) {
-          const uint32_t w = s_weight[buffer][tid][c];
-          const double weight = (double)w;
-          sum_visible += weight * (double)s_visible[buffer][tid][c];
-          sum_infrared += weight * (double)s_infrared[buffer][tid][c];
-          sum_weight += weight;
-          accepted += w != 0u;
-        }
-      }
     }
-    const int y = row0 + tid;
-    if (tid < kRowSumRows && y < H) {
-      J.row_sums[(size_t)y * 3 + 0] = sum_visible;
-      J.row_sums[(size_t)y * 3 + 1] = sum_infrared;
-      J.row_sums[(size_t)y * 3 + 2] = sum_weight;
-      J.accepted[y] = accepted;
+    const int y = row0 + tid / 3;
+    if (tid < kRowSumLanes && y < H) {
+      J.row_sums[(size_t)row0 * 3 + tid] = sum;
+      if (tid % 3 == 2) J.accepted[y] = accepted;
     }
   });
 }
@@ -425,6 +445,7 @@ __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
         __syncthreads();
         if (base + kChainChunk < H) load(base + kChainChunk);
         if (lane < 3) {
+#pragma unroll 16
           for (int r = 0; r < n; ++r) {
             total += in_buffer[r * 3 + lane];
             out_buffer[r * 3 + lane] = total;
@@ -443,6 +464,7 @@ __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
         for (int i = lane; i < n * 2; i += 64) in_buffer[i] = source[(size_t)base * 2 + i];
         __syncthreads();
         if (lane == 0) {
+#pragma unroll 8
           for (int e = 0; e < n; ++e) {
             const double denominator_wide = (double)denominator + in_buffer[e * 2 + 0];
             const double numerator_wide = (double)numerator + in_buffer[e * 2 + 1];
