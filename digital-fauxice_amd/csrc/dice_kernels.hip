@@ -79,6 +79,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value,
   return offset + inclusive - value;
 }
 
+// The job table is written before a launch and only read by kernels.
+// Reading it through the constant address space lets the compiler load its
+// fields with scalar loads and treat the buffer pointers read from it as
+// global memory: global rather than flat loads and stores, whose shared
+// completion counter would make every LDS wait also wait for memory.
+__device__ __forceinline__ const JobDev& job_at(const JobSet& set, int j) {
+  using ConstantJob = const __attribute__((address_space(4))) JobDev;
+  return *(const JobDev*)((ConstantJob*)set.jobs + j);
+}
+
 // Job of work unit u: the last j with base[j] <= u (empty ranges skipped).
 __device__ __forceinline__ int job_of(const JobSet& set, uint32_t u) {
   int lo = 0, hi = set.count - 1;
@@ -105,7 +115,7 @@ __device__ __forceinline__ void for_each_unit(const JobSet& set, Body&& body) {
     const uint32_t u = first + me;
     if (u < total) {
       const int j = job_of(set, u);
-      const JobDev& J = set.jobs[j];
+      const JobDev& J = job_at(set, j);
       if (!J.skip) body(J, (int)(u - set.base[j]));
     }
     __syncthreads();
@@ -317,28 +327,42 @@ __global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobSet set
 }
 
 // Per-row binary64 sums in column order (producer_parameters.py:303-360).
-// A workgroup owns kRowSumRows rows.  Lane r*3+q of the summing wave
-// carries chain q (visible, infrared, weight) of row r, so a column step is
-// one LDS read and one dependent add per lane.  Eight staging waves turn
-// each 128-column chunk into the three binary64 terms of every pixel --
-// weight * lut(visible), weight * lut(infrared) and weight, each a single
-// rounded product whichever thread forms it -- and store them column-major,
-// the summing wave's read order.  A rejected pixel has weight 0 and three
-// +0.0 terms, the exact identity of the non-negative sums (so are the
-// padding columns past the active width).  Every staging load is issued
-// unconditionally at a clamped address; the next chunk's pixels are
-// requested after the current chunk's gathers, so they stay in flight while
-// the current chunk is converted.  Two LDS buffers alternate, ordered by the
-// one barrier per chunk.
+// A workgroup owns kRowSumRows rows and pipelines 128-column chunks through
+// three kinds of waves, one barrier per step:
+//   loader waves     take turns: wave w holds chunk k = w (mod 4) in
+//                    registers, moves it into a two-slot LDS ring at step k
+//                    and requests chunk k + 4, so four chunks are in flight
+//                    and each wave only ever waits for its own loads;
+//   converter groups alternate chunks: at step c+1 a group reads chunk c
+//                    from the ring and requests the latch byte and the two
+//                    LUT values of every pixel; at step c+2 it turns them
+//                    into the three binary64 terms of every pixel --
+//                    weight * lut(visible), weight * lut(infrared) and
+//                    weight, each one rounded product whichever thread
+//                    forms it -- stored column-major;
+//   the summing wave adds chunk c at step c+3: lane r*3+q carries chain q
+//                    (visible, infrared, weight) of row r, with the LDS
+//                    reads of the next 16 columns in flight while the
+//                    current 16 are added.
+// A rejected pixel has weight 0 and three +0.0 terms, the exact identity of
+// the non-negative sums; so do the padding columns and rows past the frame,
+// whose loads are clamped into the frame.
 constexpr int kRowSumRows = 16;
 constexpr int kRowSumLanes = kRowSumRows * 3;
 constexpr int kRowSumPitch = kRowSumLanes + 1;  // odd pitch: conflict-free column stores
 constexpr int kRowSumCols = 128;
-constexpr int kRowSumStagers = 512;
-constexpr int kRowSumPer = kRowSumRows * kRowSumCols / kRowSumStagers;
-static_assert(kRowSumRows * kRowSumCols % kRowSumStagers == 0, "whole pixels per stager");
+constexpr int kRowSumChunk = kRowSumRows * kRowSumCols;
+constexpr int kRowSumLoaderWaves = 4;
+constexpr int kRowSumGroup = 256;  // threads of one converter group
+constexpr int kRowSumThreads = 64 * (1 + kRowSumLoaderWaves) + 2 * kRowSumGroup;
+constexpr int kRowSumLoad = kRowSumChunk / 64;
+constexpr int kRowSumConvert = kRowSumChunk / kRowSumGroup;
+constexpr int kRowSumBatch = 16;
+static_assert(kRowSumChunk % kRowSumGroup == 0, "whole pixels per converter");
+static_assert(kRowSumCols % kRowSumBatch == 0, "whole summing batches");
 
-__global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobSet set) {
+__global__ __launch_bounds__(kRowSumThreads) void k_row_sums(JobSet set) {
+  __shared__ uint2 s_raw[2][kRowSumChunk];
   __shared__ double s_terms[2][kRowSumCols][kRowSumPitch];
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const int H = J.height, W = J.width;
@@ -346,66 +370,102 @@ __global__ __launch_bounds__(64 + kRowSumStagers) void k_row_sums(JobSet set) {
     const int row0 = unit * kRowSumRows;
     const int tid = threadIdx.x;
     const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
-    const int stager = tid - 64;
-    uint2 raw[kRowSumPer];
-    auto fetch = [&](int k) {
+    const int steps = chunks == 0 ? 0 : chunks + 3;
+    if (tid < 64) {
+      double sum = 0.0;
+      uint32_t accepted = 0;
+      for (int k = 0; k < steps; ++k) {
+        if (k >= 3 && tid < kRowSumLanes) {
+          const double(*terms)[kRowSumPitch] = s_terms[(k - 3) & 1];
+          double now[kRowSumBatch];
 #pragma unroll
-      for (int i = 0; i < kRowSumPer; ++i) {
-        const int e = stager + i * kRowSumStagers;
-        const int y = min(row0 + e / kRowSumCols, H - 1);
-        const int x = min(k * kRowSumCols + e % kRowSumCols, active - 1);
-        raw[i] = *reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + x) * 4);
+          for (int i = 0; i < kRowSumBatch; ++i) now[i] = terms[i][tid];
+#pragma unroll
+          for (int c0 = kRowSumBatch; c0 <= kRowSumCols; c0 += kRowSumBatch) {
+            double next[kRowSumBatch];
+            if (c0 < kRowSumCols) {
+#pragma unroll
+              for (int i = 0; i < kRowSumBatch; ++i) next[i] = terms[c0 + i][tid];
+            }
+#pragma unroll
+            for (int i = 0; i < kRowSumBatch; ++i) {
+              sum += now[i];
+              accepted += now[i] != 0.0;
+            }
+            if (c0 < kRowSumCols) {
+#pragma unroll
+              for (int i = 0; i < kRowSumBatch; ++i) now[i] = next[i];
+            }
+          }
+        }
+        __syncthreads();
       }
-    };
-    if (stager >= 0 && chunks > 0) fetch(0);
-    double sum = 0.0;
-    uint32_t accepted = 0;
-    for (int k = 0; k <= chunks; ++k) {
-      if (stager >= 0 && k < chunks) {
-        uint32_t fail[kRowSumPer], infrared[kRowSumPer];
-        float lut_visible[kRowSumPer], lut_infrared[kRowSumPer];
+      const int y = row0 + tid / 3;
+      if (tid < kRowSumLanes && y < H) {
+        J.row_sums[(size_t)row0 * 3 + tid] = sum;
+        if (tid % 3 == 2) J.accepted[y] = accepted;
+      }
+    } else if (tid < 64 * (1 + kRowSumLoaderWaves)) {
+      const int wave = (tid >> 6) - 1, lane = tid & 63;
+      uint2 raw[kRowSumLoad];
+      auto fetch = [&](int chunk) {
 #pragma unroll
-        for (int i = 0; i < kRowSumPer; ++i) {
-          const int e = stager + i * kRowSumStagers;
+        for (int i = 0; i < kRowSumLoad; ++i) {
+          const int e = lane + i * 64;
           const int y = min(row0 + e / kRowSumCols, H - 1);
-          const int x = min(k * kRowSumCols + e % kRowSumCols, active - 1);
-          infrared[i] = raw[i].y >> 16;
-          fail[i] = J.failpos[(size_t)(y >> 3) * cells + (x >> 3)];
-          lut_visible[i] = J.lut[raw[i].x & 0xFFFFu];
-          lut_infrared[i] = J.lut[infrared[i]];
+          const int x = min(chunk * kRowSumCols + e % kRowSumCols, active - 1);
+          raw[i] = *reinterpret_cast<const uint2*>(J.rgbi + ((size_t)y * W + x) * 4);
         }
-        __builtin_amdgcn_sched_barrier(0);
-        fetch(min(k + 1, chunks - 1));
-        __builtin_amdgcn_sched_barrier(0);
-        double(*terms)[kRowSumPitch] = s_terms[k & 1];
+      };
+      if (wave < chunks) fetch(wave);
+      for (int k = 0; k < steps; ++k) {
+        if (k < chunks && k % kRowSumLoaderWaves == wave) {
+          uint2* ring = s_raw[k & 1];
 #pragma unroll
-        for (int i = 0; i < kRowSumPer; ++i) {
-          const int e = stager + i * kRowSumStagers;
-          const int r = e / kRowSumCols, c = e % kRowSumCols;
-          const int y = row0 + r, x = k * kRowSumCols + c;
-          const int position = (y & 7) * 8 + (x & 7);
-          const bool keep = y < H && x < active && position < (int)fail[i];
-          const double weight = keep ? (double)(infrared[i] * infrared[i]) : 0.0;
-          terms[c][r * 3 + 0] = weight * (double)lut_visible[i];
-          terms[c][r * 3 + 1] = weight * (double)lut_infrared[i];
-          terms[c][r * 3 + 2] = weight;
+          for (int i = 0; i < kRowSumLoad; ++i) ring[lane + i * 64] = raw[i];
+          if (k + kRowSumLoaderWaves < chunks) fetch(k + kRowSumLoaderWaves);
         }
+        __syncthreads();
       }
-      if (tid < kRowSumLanes && k >= 1) {
-        double(*terms)[kRowSumPitch] = s_terms[(k - 1) & 1];
-#pragma unroll 16
-        for (int c = 0; c < kRowSumCols; ++c) {
-          const double term = terms[c][tid];
-          sum += term;
-          accepted += term != 0.0;
+    } else {
+      const int converter = tid - 64 * (1 + kRowSumLoaderWaves);
+      const int group = converter / kRowSumGroup, t = converter % kRowSumGroup;
+      uint2 pixel[kRowSumConvert];
+      uint32_t fail[kRowSumConvert];
+      float lut_visible[kRowSumConvert], lut_infrared[kRowSumConvert];
+      for (int k = 0; k < steps; ++k) {
+        if (k >= 1 && k - 1 < chunks && ((k - 1) & 1) == group) {
+          const int chunk = k - 1;
+          const uint2* ring = s_raw[chunk & 1];
+#pragma unroll
+          for (int i = 0; i < kRowSumConvert; ++i) {
+            const int e = t + i * kRowSumGroup;
+            const int y = min(row0 + e / kRowSumCols, H - 1);
+            const int x = min(chunk * kRowSumCols + e % kRowSumCols, active - 1);
+            pixel[i] = ring[e];
+            fail[i] = J.failpos[(size_t)(y >> 3) * cells + (x >> 3)];
+            lut_visible[i] = J.lut[pixel[i].x & 0xFFFFu];
+            lut_infrared[i] = J.lut[pixel[i].y >> 16];
+          }
+        } else if (k >= 2 && k - 2 < chunks && ((k - 2) & 1) == group) {
+          const int chunk = k - 2;
+          double(*terms)[kRowSumPitch] = s_terms[chunk & 1];
+#pragma unroll
+          for (int i = 0; i < kRowSumConvert; ++i) {
+            const int e = t + i * kRowSumGroup;
+            const int r = e / kRowSumCols, c = e % kRowSumCols;
+            const int y = row0 + r, x = chunk * kRowSumCols + c;
+            const int position = (y & 7) * 8 + (x & 7);
+            const bool keep = y < H && x < active && position < (int)fail[i];
+            const uint32_t infrared = pixel[i].y >> 16;
+            const double weight = keep ? (double)(infrared * infrared) : 0.0;
+            terms[c][r * 3 + 0] = weight * (double)lut_visible[i];
+            terms[c][r * 3 + 1] = weight * (double)lut_infrared[i];
+            terms[c][r * 3 + 2] = weight;
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
-    }
-    const int y = row0 + tid / 3;
-    if (tid < kRowSumLanes && y < H) {
-      J.row_sums[(size_t)row0 * 3 + tid] = sum;
-      if (tid % 3 == 2) J.accepted[y] = accepted;
     }
   });
 }
@@ -1124,7 +1184,7 @@ __global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
   for (int j = 0; j < set.count; ++j) {
-    const JobDev J = set.jobs[j];
+    const JobDev& J = job_at(set, j);
     if (J.skip) continue;
     const uint32_t count = J.counters[0];
     const uint32_t total = count + 6u * (uint32_t)J.width;
@@ -1528,7 +1588,7 @@ hipError_t launch_producer(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                     }), kProducerThreads, stream);
   launch_persistent(k_row_sums, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
                       return when(J, blocks_for(J.height, kRowSumRows));
-                    }), 64 + kRowSumStagers, stream);
+                    }), kRowSumThreads, stream);
   launch_persistent(k_producer_chain, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
                       return when(J, 2);
                     }), 64, stream);
