@@ -63,6 +63,8 @@ def parse_args() -> argparse.Namespace:
                         help="batched legs run each job's writer chain on one device wavefront")
     parser.add_argument("--drivers", type=int, default=2,
                         help="concurrent batches in the pipelined and serving legs")
+    parser.add_argument("--pipeline-rounds", type=int, default=4,
+                        help="the pipelined leg runs steps x batch x this many frames")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     parser.add_argument("--serving-jobs", type=int, default=40)
@@ -231,14 +233,18 @@ def main() -> int:
     from dice_amd.hip_backend.device import DeviceArray
     from dice_amd.hip_backend.serving import HipServingEngine
 
-    jobs_total = args.steps * args.batch
-    pipe_outs = [DeviceArray(height * width * 6) for _ in range(jobs_total)]
+    jobs_total = args.steps * args.batch * max(1, args.pipeline_rounds)
+    # output buffers are recycled: a batch takes jobs in submission order and at
+    # most `drivers` batches run at once, so a buffer is free again long before
+    # its next job starts
+    pipe_outs = [DeviceArray(height * width * 6)
+                 for _ in range(min(jobs_total, max(64, 4 * args.drivers * args.batch)))]
     with HipServingEngine(drivers=args.drivers, max_batch=args.batch, graph=True,
                           device_chain=args.device_chain,
                           autotune=True) as served:
         warm = [served.submit_device(full.d_main[k % args.batch].address, height, width,
                                      full.prepasses[k % args.batch],
-                                     pipe_outs[k].address)
+                                     pipe_outs[k % len(pipe_outs)].address)
                 for k in range(min(jobs_total, 2 * args.drivers * args.batch))]
         for t in warm:
             served.wait(t)
@@ -246,7 +252,8 @@ def main() -> int:
         served.reset_stats()
         start = time.perf_counter()
         tickets = [served.submit_device(full.d_main[k % args.batch].address, height, width,
-                                        full.prepasses[k % args.batch], pipe_outs[k].address)
+                                        full.prepasses[k % args.batch],
+                                        pipe_outs[k % len(pipe_outs)].address)
                    for k in range(jobs_total)]
         pipe_prints = []
         for t in tickets:

@@ -21,42 +21,48 @@ static_assert(sizeof(Site) == 40, "Site record must stay 40 bytes");
 
 // The writer's draw schedule of one site in integer form, one 64-bit word
 // (two 32-bit halves) per channel.  A channel that draws takes the LCG step
-// s -> t; with u = t (or its mirror 2^24-1-t when the dither span is
-// negative, so that the dither delta is non-decreasing in u) it keeps t
-// ("keep": the value is decided by this draw) when
-// ((u - a) mod 2^32 < len) != inv, and otherwise redraws once more.  A
-// channel without the floor gate always keeps t (a = 0, len = 2^24); one
-// that can never satisfy the floor test always redraws (len = 0).
+// s -> t and keeps t (the value is decided by this draw) when
+// ((t - a) mod 2^24) < len, and otherwise redraws once more.  The keep set
+// is an interval of the 2^24 states, possibly wrapping: the writer's own
+// predicate is an interval of the dither variable u (t, or its mirror
+// 2^24-1-t when the dither span is negative, so that the delta is
+// non-decreasing in u), or its complement, and channel_word maps it onto t.
+// A channel without the floor gate always keeps t (len = 2^24); one that
+// can never satisfy the floor test always redraws (len = 0).
 // Low half: bits 0-23 a, bit 31 (channel 0 only) the site is attempted
-// (always for main-frame sites).  High half: bits 0-24 len, 25 inv,
-// 26 mirror, 27 the channel draws.
+// (always for main-frame sites).  High half: bits 0-24 len, 27 the channel
+// draws.
 struct WriterRecord {
   uint64_t ch[3];
 };
 static_assert(sizeof(WriterRecord) == 24, "WriterRecord must stay 24 bytes");
 
 constexpr uint32_t kChanLen = 0x1FFFFFFu;
-constexpr uint32_t kChanInv = 1u << 25;
-constexpr uint32_t kChanMirror = 1u << 26;
 constexpr uint32_t kChanDraws = 1u << 27;
 constexpr uint64_t kRecordAttemptedBit = 1ull << 31;
+constexpr uint32_t kStates = 1u << 24;
 
+// keep <=> (u in [a, a + len)) != inv, with u = t or, mirrored, 2^24-1-t.
 DICE_HD uint64_t channel_word(uint32_t a, uint32_t len, bool inv, bool mirror) {
-  const uint32_t high = len | (inv ? kChanInv : 0u) | (mirror ? kChanMirror : 0u) | kChanDraws;
-  return (uint64_t)(a & 0xFFFFFFu) | ((uint64_t)high << 32);
+  if (inv) {  // the complement [a + len, a + 2^24) mod 2^24
+    a = (a + len) & kLcgMask;
+    len = kStates - len;
+  }
+  if (mirror) a = (kStates - a - len) & kLcgMask;  // u in [a, a+len) <=> t in [2^24-a-len, ...)
+  return (uint64_t)(a & kLcgMask) | ((uint64_t)(len | kChanDraws) << 32);
 }
+
+DICE_HD bool keeps(uint32_t t, uint32_t a, uint32_t len) { return ((t - a) & kLcgMask) < len; }
 
 // One channel of the chain on the two halves of its word: the next state,
 // and the draws it took (0, 1 or 2).
 DICE_HD uint32_t chain_channel(uint32_t low, uint32_t high, uint32_t s, uint32_t& draws) {
-  const uint32_t a = low & 0xFFFFFFu;
+  const uint32_t a = low & kLcgMask;
   const uint32_t len = high & kChanLen;
-  const uint32_t mirror = (high & kChanMirror) ? kLcgMask : 0u;
-  const bool inv = (high & kChanInv) != 0;
   const bool drawn = (high & kChanDraws) != 0;
   const uint32_t t = (125u * s + 1u) & kLcgMask;
   const uint32_t t2 = (125u * t + 1u) & kLcgMask;
-  const bool keep = (((t ^ mirror) - a) < len) != inv;
+  const bool keep = keeps(t, a, len);
   draws = drawn ? (keep ? 1u : 2u) : 0u;
   return drawn ? (keep ? t : t2) : s;
 }
@@ -158,6 +164,57 @@ DICE_HD uint32_t chain_site(const WriterRecord& r, uint32_t s, uint64_t& draws) 
     uint32_t d;
     s = chain_channel((uint32_t)r.ch[c], (uint32_t)(r.ch[c] >> 32), s, d);
     draws += d;
+  }
+  return s;
+}
+
+// k LCG steps as one affine map: lcg^k(s) = (lcg_pow(k) * s + lcg_add(k)) mod 2^24
+// (evaluated at compile time).
+constexpr uint32_t lcg_pow(int k) { return k == 0 ? 1u : (lcg_pow(k - 1) * 125u) & kLcgMask; }
+constexpr uint32_t lcg_add(int k) { return k == 0 ? 0u : (lcg_add(k - 1) * 125u + 1u) & kLcgMask; }
+
+DICE_HD bool draws_all_channels(const WriterRecord& r) {
+  return ((r.ch[0] & r.ch[1] & r.ch[2]) >> 32) & kChanDraws;
+}
+
+// chain_site for a record whose three channels all draw (nearly every
+// site), with a short dependency chain: the six states the site can reach,
+// lcg^1..6(s), come straight from s; channel 1 tests t1, channel 2 tests t2
+// or t3, channel 3 one of t3..t5, and the site ends on t3..t6 -- every test
+// is evaluated at once and the outcome selected.
+DICE_HD uint32_t chain_site_drawing(const WriterRecord& r, uint32_t s, uint64_t& draws) {
+  constexpr uint32_t p1 = lcg_pow(1), p2 = lcg_pow(2), p3 = lcg_pow(3);
+  constexpr uint32_t p4 = lcg_pow(4), p5 = lcg_pow(5), p6 = lcg_pow(6);
+  constexpr uint32_t c1_ = lcg_add(1), c2_ = lcg_add(2), c3_ = lcg_add(3);
+  constexpr uint32_t c4_ = lcg_add(4), c5_ = lcg_add(5), c6_ = lcg_add(6);
+  const uint32_t t[7] = {s,
+                         (p1 * s + c1_) & kLcgMask,
+                         (p2 * s + c2_) & kLcgMask,
+                         (p3 * s + c3_) & kLcgMask,
+                         (p4 * s + c4_) & kLcgMask,
+                         (p5 * s + c5_) & kLcgMask,
+                         (p6 * s + c6_) & kLcgMask};
+  const uint32_t a0 = (uint32_t)r.ch[0] & kLcgMask, l0 = (uint32_t)(r.ch[0] >> 32) & kChanLen;
+  const uint32_t a1 = (uint32_t)r.ch[1] & kLcgMask, l1 = (uint32_t)(r.ch[1] >> 32) & kChanLen;
+  const uint32_t a2 = (uint32_t)r.ch[2] & kLcgMask, l2 = (uint32_t)(r.ch[2] >> 32) & kChanLen;
+  const bool c1 = keeps(t[1], a0, l0);
+  const bool m22 = keeps(t[2], a1, l1), m23 = keeps(t[3], a1, l1);
+  const bool m33 = keeps(t[3], a2, l2), m34 = keeps(t[4], a2, l2), m35 = keeps(t[5], a2, l2);
+  const bool c2 = c1 ? m22 : m23;  // channel 2 draws t2 after a kept t1, else t3
+  const bool c3 = c1 ? (c2 ? m33 : m34) : (c2 ? m34 : m35);
+  const uint32_t kept = c1 ? (c2 ? t[3] : t[4]) : (c2 ? t[4] : t[5]);
+  const uint32_t redrawn = c1 ? (c2 ? t[4] : t[5]) : (c2 ? t[5] : t[6]);
+  draws += 6u - c1 - c2 - c3;
+  return c3 ? kept : redrawn;
+}
+
+// The chain over n consecutive records: each record's start state into
+// states[i]; returns the state after the last.
+inline uint32_t chain_records(const WriterRecord* r, size_t n, uint32_t s, uint32_t* states,
+                              uint64_t& draws) {
+  for (size_t i = 0; i < n; ++i) {
+    states[i] = s;
+    s = draws_all_channels(r[i]) ? chain_site_drawing(r[i], s, draws) : chain_site(r[i], s, draws);
   }
   return s;
 }

@@ -719,6 +719,72 @@ int32_t dice_t_dither_chain(dice_ctx* ctx, const double* values, const float* sc
   });
 }
 
+int32_t dice_t_writer_chain_host(const double* cand, const float* orig, const uint8_t* floor,
+                                 int64_t count, float low, float high, const float* scales,
+                                 uint32_t state_in, int32_t generic, uint32_t* states,
+                                 uint32_t* state_out, uint64_t* advances_out) {
+  if (count < 0 || (count > 0 && (!cand || !orig || !floor || !states)) || !scales ||
+      !state_out || !advances_out)
+    return set_error(DICE_INVALID, "dice_t_writer_chain_host: bad arguments");
+  WriterConsts K;
+  K.low = (double)low;
+  K.high = (double)high;
+  K.low_lt_high = K.low < K.high;
+  for (int c = 0; c < 3; ++c) K.scales[c] = scales[c];
+  std::vector<WriterRecord> records((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    Site site;
+    for (int c = 0; c < 3; ++c) {
+      site.cand[c] = cand[i * 3 + c];
+      site.orig[c] = orig[i * 3 + c];
+    }
+    site.pixel = floor[i] ? kSiteFloorBit : 0u;
+    records[(size_t)i] = writer_record(site, true, K);
+  }
+  uint64_t draws = 0;
+  uint32_t state = state_in & kLcgMask;
+  if (generic) {
+    for (int64_t i = 0; i < count; ++i) {
+      states[i] = state;
+      state = chain_site(records[(size_t)i], state, draws);
+    }
+  } else {
+    state = chain_records(records.data(), (size_t)count, state, states, draws);
+  }
+  *state_out = state;
+  *advances_out = draws;
+  return DICE_OK;
+}
+
+int32_t dice_t_writer_site_host(const double* cand, const float* orig, uint8_t floor, float low,
+                                float high, const float* scales, const uint32_t* states_in,
+                                int64_t count, int32_t generic, uint32_t* states_out,
+                                uint8_t* advances) {
+  if (!cand || !orig || !scales || count < 0 ||
+      (count > 0 && (!states_in || !states_out || !advances)))
+    return set_error(DICE_INVALID, "dice_t_writer_site_host: bad arguments");
+  WriterConsts K;
+  K.low = (double)low;
+  K.high = (double)high;
+  K.low_lt_high = K.low < K.high;
+  for (int c = 0; c < 3; ++c) K.scales[c] = scales[c];
+  Site site;
+  for (int c = 0; c < 3; ++c) {
+    site.cand[c] = cand[c];
+    site.orig[c] = orig[c];
+  }
+  site.pixel = floor ? kSiteFloorBit : 0u;
+  const WriterRecord record = writer_record(site, true, K);
+  const bool fast = !generic && draws_all_channels(record);
+  for (int64_t i = 0; i < count; ++i) {
+    uint64_t draws = 0;
+    const uint32_t s = states_in[i] & kLcgMask;
+    states_out[i] = fast ? chain_site_drawing(record, s, draws) : chain_site(record, s, draws);
+    advances[i] = (uint8_t)draws;
+  }
+  return DICE_OK;
+}
+
 int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count, uint16_t* out) {
   return with_device_buffers(ctx, [&]() -> int32_t {
     Scratch S;

@@ -133,19 +133,17 @@ struct Workspace {
   PinnedBuffer h_records;    // 6W startup records, then the main records
   PinnedBuffer h_row_start;  // H+1
   PinnedBuffer h_states;     // 6W startup states, then the main states
-  PinnedBuffer h_out;        // output bytes to hash when the output stays on the device
+  PinnedBuffer h_out;        // staging of the output rows on their way to the hash and caller
   hipEvent_t band[kMaxBands] = {};     // writer values of band b done (runner stream)
   hipEvent_t fetched[kMaxBands] = {};  // draw schedules of band b on the host (copy stream)
-  hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (copy stream)
+  hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (output stream)
   hipEvent_t startup_fetched = nullptr;
-  // this job's output rows to the host: its own in-order stream, so a copy
-  // waiting for this job's next band never holds up another job's copies
-  hipStream_t copy_stream = nullptr;
   // the device writer chain of this job and its writer values, band by band
   hipStream_t chain_stream = nullptr;
   int bands = 1;
   bool events = false;
-  // chain -> hash hand-off
+  // chain -> copy scheduler -> hash hand-off: `published` counts the bands
+  // whose output copy is queued, `aborted` that the chain failed
   std::mutex mutex;
   std::condition_variable cv;
   int published = 0;
@@ -156,10 +154,9 @@ struct Workspace {
       for (int b = 0; b < kMaxBands; ++b)
         if (set[b]) (void)hipEventDestroy(set[b]);
     if (startup_fetched) (void)hipEventDestroy(startup_fetched);
-    for (hipStream_t stream : {copy_stream, chain_stream}) {
-      if (!stream) continue;
-      (void)hipStreamSynchronize(stream);
-      (void)hipStreamDestroy(stream);
+    if (chain_stream) {
+      (void)hipStreamSynchronize(chain_stream);
+      (void)hipStreamDestroy(chain_stream);
     }
   }
   int32_t init_events() {
@@ -170,7 +167,6 @@ struct Workspace {
     for (hipEvent_t* set : {band, fetched, copied})
       for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
     HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
-    HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&chain_stream, hipStreamNonBlocking));
     events = true;
     return DICE_OK;
@@ -207,8 +203,10 @@ Runner::~Runner() {
     if (e) (void)hipEventDestroy(e);
   if (sync_event_) (void)hipEventDestroy(sync_event_);
   if (fetch_stream_) (void)hipStreamSynchronize(fetch_stream_);
+  if (out_stream_) (void)hipStreamSynchronize(out_stream_);
   ws_.clear();
   if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
+  if (out_stream_) (void)hipStreamDestroy(out_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -216,6 +214,7 @@ int32_t Runner::init() {
   HIP_TRY(hipSetDevice(device_));
   HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&out_stream_, hipStreamNonBlocking));
   for (hipEvent_t& e : ev_) HIP_TRY(hipEventCreate(&e));
   for (hipEvent_t& e : fine_) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventCreateWithFlags(&sync_event_, hipEventDisableTiming | hipEventBlockingSync));
@@ -486,10 +485,14 @@ int32_t Runner::phase_b(int n, bool graph, bool* replayed) {
 
 namespace {
 
+// The first failure of a job's worker threads (chain, copy scheduler, hash)
+// with the failing thread's message.
 struct JobThreads {
+  std::mutex mutex;
   int32_t code = DICE_OK;
   std::string error;
   void fail(int32_t rc) {
+    std::lock_guard<std::mutex> lock(mutex);
     if (code == DICE_OK) {
       code = rc;
       error = error_buffer();
@@ -613,24 +616,73 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
                            hipMemcpyDeviceToHost, fetch_stream_));
     HIP_TRY(hipEventRecord(w.startup_fetched, fetch_stream_));
   }
-  for (int b = 0; b < max_bands && !options.device_chain; ++b) {
+  // The first kFetchAhead bands of every job now; each chain then asks for
+  // band b + kFetchAhead when it starts band b, so the schedules come down
+  // just ahead of need and never queue the output copies behind them.
+  auto fetch_band = [this](const JobDev& J, Workspace& w, int b) -> int32_t {
+    const uint32_t* row_start = w.h_row_start.as<uint32_t>();
+    const uint32_t s0 = row_start[band_row(J.height, w.bands, b)];
+    const uint32_t s1 = row_start[band_row(J.height, w.bands, b + 1)];
+    const size_t su = 6 * (size_t)J.width;
+    if (s1 > s0)
+      HIP_TRY(hipMemcpyAsync(w.h_records.as<WriterRecord>() + su + s0, J.writer_records + s0,
+                             (size_t)(s1 - s0) * sizeof(WriterRecord), hipMemcpyDeviceToHost,
+                             fetch_stream_));
+    HIP_TRY(hipEventRecord(w.fetched[b], fetch_stream_));
+    return DICE_OK;
+  };
+  constexpr int kFetchAhead = 2;
+  for (int b = 0; b < std::min(max_bands, kFetchAhead) && !options.device_chain; ++b) {
     for (int j = 0; j < n; ++j) {
       const JobDev& J = H_jobs[j];
       Workspace& w = *ws_[j];
       if (J.skip || b >= w.bands) continue;
-      const uint32_t* row_start = w.h_row_start.as<uint32_t>();
-      const uint32_t s0 = row_start[band_row(J.height, w.bands, b)];
-      const uint32_t s1 = row_start[band_row(J.height, w.bands, b + 1)];
-      const size_t su = 6 * (size_t)J.width;
-      if (s1 > s0)
-        HIP_TRY(hipMemcpyAsync(w.h_records.as<WriterRecord>() + su + s0, J.writer_records + s0,
-                               (size_t)(s1 - s0) * sizeof(WriterRecord), hipMemcpyDeviceToHost,
-                               fetch_stream_));
-      HIP_TRY(hipEventRecord(w.fetched[b], fetch_stream_));
+      DICE_TRY(fetch_band(J, w, b));
     }
   }
 
-  // ---- host: per job, the serial chain and the hash, overlapped by band
+  // ---- host: per job, the serial chain and the hash, overlapped by band.
+  // Output rows always come down into pinned staging (a DMA straight into
+  // a pageable caller buffer is staged by the runtime and far slower).  The
+  // chains hand each band whose writer values are queued to one copy
+  // scheduler, which queues the output copies on one stream, lowest band
+  // first across the jobs: every job's first band arrives early, so every
+  // hash starts early, and a job whose chain runs ahead does not take the
+  // host link from the others.
+  struct CopyQueue {
+    std::mutex mutex;
+    std::condition_variable cv;
+    std::vector<int> ready, queued, bands;
+    std::vector<bool> closed;  // chain failed or nothing to copy
+  } cq;
+  cq.ready.assign(n, 0);
+  cq.queued.assign(n, 0);
+  cq.bands.assign(n, 0);
+  cq.closed.assign(n, true);
+  for (int j = 0; j < n; ++j) {
+    if (H_jobs[j].skip) continue;
+    const JobRequest& R = jobs[j];
+    const bool want_sha = (R.flags & DICE_SKIP_SHA256) == 0;
+    const bool host_out = (R.flags & DICE_OUT_ON_DEVICE) == 0;
+    if (!want_sha && !host_out) continue;
+    HIP_TRY(ws_[j]->h_out.ensure((size_t)R.height * R.width * 6));
+    cq.bands[j] = ws_[j]->bands;
+    cq.closed[j] = false;
+  }
+  auto values_queued = [&cq](int j, int bands_ready) {
+    {
+      std::lock_guard<std::mutex> lock(cq.mutex);
+      cq.ready[j] = bands_ready;
+    }
+    cq.cv.notify_one();
+  };
+  auto chain_failed = [&cq](int j) {
+    {
+      std::lock_guard<std::mutex> lock(cq.mutex);
+      cq.closed[j] = true;
+    }
+    cq.cv.notify_one();
+  };
   std::vector<JobThreads> outcome(n);
   std::vector<std::thread> threads;
   std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
@@ -678,23 +730,25 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       };
       if (int32_t rc = enqueue()) {  // the hash thread must not wait for it
         outcome[j].fail(rc);
-        std::lock_guard<std::mutex> lock(w.mutex);
-        w.aborted = true;
+        {
+          std::lock_guard<std::mutex> lock(w.mutex);
+          w.aborted = true;
+        }
+        chain_failed(j);
       } else {
-        std::lock_guard<std::mutex> lock(w.mutex);
-        w.published = bands;
+        values_queued(j, bands);
       }
       chain_seconds[j] = now_seconds() - t0;
     } else {
     // chain thread
     threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
-                          &chain_waits]() {
+                          &chain_waits, &values_queued, &chain_failed, &fetch_band]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const JobDev& J = H_jobs[j];
       JobThreads& out = outcome[j];
       const double t0 = now_seconds();
-      double fetch_wait = 0.0;
+      double fetch_wait = 0.0, walking = 0.0;
       auto body = [&]() -> int32_t {
         HIP_TRY(hipSetDevice(device_));
         const int64_t H = R.height, W = R.width;
@@ -728,24 +782,20 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         for (int b = 0; b < bands; ++b) {
           const uint32_t s0 = row_start[band_row(H, bands, b)];
           const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
+          if (b + kFetchAhead < bands) DICE_TRY(fetch_band(J, w, b + kFetchAhead));
           const double f0 = now_seconds();
           HIP_TRY(hipEventSynchronize(w.fetched[b]));
           fetch_wait += now_seconds() - f0;
-          for (uint32_t i = s0; i < s1; ++i) {
-            main_states[i] = state;
-            state = chain_site(main[i], state, advances);
-          }
+          const double k0 = now_seconds();
+          state = chain_records(main + s0, s1 - s0, state, main_states + s0, advances);
+          walking += now_seconds() - k0;
           if (s1 > s0) {
             HIP_TRY(hipMemcpyAsync(J.site_state + s0, main_states + s0, (size_t)(s1 - s0) * 4,
                                    hipMemcpyHostToDevice, stream_));
             HIP_TRY(launch_writer_values(J, s0, s1, state, nullptr, stream_));
           }
           HIP_TRY(hipEventRecord(w.band[b], stream_));
-          {
-            std::lock_guard<std::mutex> lock(w.mutex);
-            w.published = b + 1;
-          }
-          w.cv.notify_all();
+          values_queued(j, b + 1);
         }
         res->attempted_pixels = row_start[H];
         res->public_rng_advances = advances;
@@ -758,20 +808,25 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       };
       if (int32_t rc = body()) {
         out.fail(rc);
-        std::lock_guard<std::mutex> lock(w.mutex);
-        w.aborted = true;
+        {
+          std::lock_guard<std::mutex> lock(w.mutex);
+          w.aborted = true;
+        }
+        chain_failed(j);
       }
       w.cv.notify_all();
       chain_seconds[j] = now_seconds() - t0;
       chain_waits[j] = fetch_wait;
+      if (trace_enabled())
+        fprintf(stderr, "dice-trace chain job=%d total=%.2fms fetch_wait=%.2fms walk=%.2fms\n", j,
+                chain_seconds[j] * 1e3, fetch_wait * 1e3, walking * 1e3);
     });
     }
     // hash / download thread
-    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &hash_seconds,
-                          &copy_seconds, &band_waits]() {
+    threads.emplace_back([this, j, bands, jobs, &outcome, &hash_seconds, &copy_seconds,
+                          &band_waits]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
-      const JobDev& J = H_jobs[j];
       const bool want_sha = (R.flags & DICE_SKIP_SHA256) == 0;
       const bool host_out = (R.flags & DICE_OUT_ON_DEVICE) == 0;
       if (!want_sha && !host_out) return;
@@ -779,44 +834,31 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       auto body = [&]() -> int32_t {
         HIP_TRY(hipSetDevice(device_));
         const int64_t H = R.height, W = R.width;
-        uint8_t* target;
-        if (host_out) {
-          target = reinterpret_cast<uint8_t*>(R.out);
-        } else {
-          HIP_TRY(w.h_out.ensure((size_t)H * W * 6));
-          target = w.h_out.as<uint8_t>();
-        }
-        const uint8_t* source = reinterpret_cast<const uint8_t*>(J.out);
+        uint8_t* staging = w.h_out.as<uint8_t>();
+        uint8_t* caller = host_out ? reinterpret_cast<uint8_t*>(R.out) : nullptr;
         Sha256 sha;
-        // Each published band's rows are queued on the copy stream behind
-        // that band's writer values; hashing band b overlaps the copies of
-        // the bands after it (a pageable caller buffer copies synchronously).
-        int queued = 0;
         for (int b = 0; b < bands; ++b) {
-          int published;
           const double p0 = now_seconds();
           {
             std::unique_lock<std::mutex> lock(w.mutex);
             w.cv.wait(lock, [&] { return w.published > b || w.aborted; });
             if (w.published <= b) return DICE_OK;  // the chain failed; it reports
-            published = w.published;
           }
           const double c0 = now_seconds();
           band_waits[j] += c0 - p0;
-          for (; queued < published; ++queued) {
-            const size_t begin = (size_t)band_row(H, bands, queued) * W * 6;
-            const size_t end = (size_t)band_row(H, bands, queued + 1) * W * 6;
-            HIP_TRY(hipStreamWaitEvent(w.copy_stream, w.band[queued], 0));
-            if (end > begin)
-              HIP_TRY(hipMemcpyAsync(target + begin, source + begin, end - begin,
-                                     hipMemcpyDeviceToHost, w.copy_stream));
-            HIP_TRY(hipEventRecord(w.copied[queued], w.copy_stream));
-          }
           HIP_TRY(hipEventSynchronize(w.copied[b]));
           const double c1 = now_seconds();
+          if (trace_enabled())
+            fprintf(stderr, "dice-trace band job=%d b=%d published=%.2f copied=%.2f\n", j, b,
+                    (c0 - trace_origin()) * 1e3, (c1 - trace_origin()) * 1e3);
           const size_t begin = (size_t)band_row(H, bands, b) * W * 6;
           const size_t end = (size_t)band_row(H, bands, b + 1) * W * 6;
-          if (want_sha) sha.update(target + begin, end - begin);
+          constexpr size_t kPiece = 256 << 10;
+          for (size_t at = begin; at < end; at += kPiece) {
+            const size_t len = std::min(kPiece, end - at);
+            if (want_sha) sha.update(staging + at, len);
+            if (caller) memcpy(caller + at, staging + at, len);
+          }
           copying += c1 - c0;
           hashing += now_seconds() - c1;
         }
@@ -827,13 +869,66 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       if (int32_t rc = body()) outcome[j].fail(rc);
       hash_seconds[j] = hashing;
       copy_seconds[j] = copying;
+      if (trace_enabled())
+        fprintf(stderr, "dice-trace hash job=%d band_wait=%.2fms copy_wait=%.2fms hash=%.2fms\n",
+                j, band_waits[j] * 1e3, copying * 1e3, hashing * 1e3);
     });
   }
+  // the copy scheduler: the lowest band ready across all jobs goes next
+  threads.emplace_back([this, n, jobs, H_jobs, &cq, &outcome]() {
+    (void)hipSetDevice(device_);
+    for (;;) {
+      int j = -1, b = 0;
+      {
+        std::unique_lock<std::mutex> lock(cq.mutex);
+        cq.cv.wait(lock, [&] {
+          j = -1;
+          bool finished = true;
+          for (int k = 0; k < n; ++k) {
+            if (cq.closed[k] || cq.queued[k] == cq.bands[k]) continue;
+            finished = false;
+            if (cq.queued[k] < cq.ready[k] && (j < 0 || cq.queued[k] < cq.queued[j])) j = k;
+          }
+          return j >= 0 || finished;
+        });
+        if (j < 0) break;
+        b = cq.queued[j];
+      }
+      Workspace& w = *ws_[j];
+      const JobRequest& R = jobs[j];
+      auto submit = [&]() -> int32_t {
+        const size_t begin = (size_t)band_row(R.height, w.bands, b) * R.width * 6;
+        const size_t end = (size_t)band_row(R.height, w.bands, b + 1) * R.width * 6;
+        HIP_TRY(hipStreamWaitEvent(out_stream_, w.band[b], 0));
+        if (end > begin)
+          HIP_TRY(hipMemcpyAsync(w.h_out.as<uint8_t>() + begin,
+                                 reinterpret_cast<const uint8_t*>(H_jobs[j].out) + begin,
+                                 end - begin, hipMemcpyDeviceToHost, out_stream_));
+        HIP_TRY(hipEventRecord(w.copied[b], out_stream_));
+        return DICE_OK;
+      };
+      const int32_t rc = submit();
+      if (rc != DICE_OK) outcome[j].fail(rc);
+      {
+        std::lock_guard<std::mutex> lock(cq.mutex);
+        cq.queued[j] = b + 1;
+        if (rc != DICE_OK) cq.closed[j] = true;
+      }
+      {
+        std::lock_guard<std::mutex> lock(w.mutex);
+        if (rc != DICE_OK)
+          w.aborted = true;
+        else
+          w.published = b + 1;
+      }
+      w.cv.notify_all();
+    }
+  });
   for (std::thread& t : threads) t.join();
   DICE_TRY(wait_stream(stream_));
+  DICE_TRY(wait_stream(out_stream_));
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
-    DICE_TRY(wait_stream(ws_[j]->copy_stream));
     if (options.device_chain) DICE_TRY(wait_stream(ws_[j]->chain_stream));
   }
   DICE_TRY(wait_stream(fetch_stream_));

@@ -121,6 +121,7 @@ class Runner {
   const DeviceTables* tables_;
   hipStream_t stream_ = nullptr;
   hipStream_t fetch_stream_ = nullptr;  // draw schedules to the host (chain)
+  hipStream_t out_stream_ = nullptr;    // output rows to the host, lowest band first
   hipEvent_t ev_[8] = {};
   hipEvent_t fine_[8] = {};
   bool fine_on_ = false;

@@ -317,6 +317,10 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "t_decision": (C.c_int32, [VOIDP, f32p, i64, i64, C.POINTER(DiceParams),
                                u8p]),
     "t_copy": (C.c_int32, [VOIDP, C.c_uint64, C.c_int32, f64p]),
+    "t_writer_chain_host": (C.c_int32, [f64p, f32p, u8p, i64, C.c_float, C.c_float, f32p,
+                                        C.c_uint32, C.c_int32, u32p, u32p, u64p]),
+    "t_writer_site_host": (C.c_int32, [f64p, f32p, C.c_uint8, C.c_float, C.c_float, f32p, u32p,
+                                       i64, C.c_int32, u32p, u8p]),
 }
 
 

@@ -387,6 +387,29 @@ int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height,
                         int64_t width, const dice_params* params,
                         uint8_t* fallback);
 
+/* The serial writer on the host, no device involved: the integer draw
+ * schedule of each site (the kernel's writer_record, compiled for the host)
+ * and the chain over them, exactly as the replay runs it; `generic` != 0
+ * forces the per-channel path for every site.  Sites: cand n*3 (binary64),
+ * orig n*3, floor n (writer floor gate on/off).  Writes each site's start
+ * state.  Stands in for the writer loop of write_band
+ * (fast_cpu/kernels.py:1139-1204) in the L1 tests. */
+int32_t dice_t_writer_chain_host(const double* cand, const float* orig,
+                                 const uint8_t* floor, int64_t count, float low,
+                                 float high, const float* scales,
+                                 uint32_t state_in, int32_t generic,
+                                 uint32_t* states, uint32_t* state_out,
+                                 uint64_t* advances_out);
+
+/* One site's serial writer step from each of `count` start states (the
+ * same schedule builder and walk as dice_t_writer_chain_host): the end
+ * state and the advances taken from each. */
+int32_t dice_t_writer_site_host(const double* cand, const float* orig,
+                                uint8_t floor, float low, float high,
+                                const float* scales, const uint32_t* states_in,
+                                int64_t count, int32_t generic,
+                                uint32_t* states_out, uint8_t* advances);
+
 /* Calibration: `repeats` launches of a plain 16-byte-vector copy kernel over
  * `bytes` (device buffers owned by the call); average milliseconds per copy. */
 int32_t dice_t_copy(dice_ctx* ctx, uint64_t bytes, int32_t repeats,

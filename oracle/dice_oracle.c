@@ -727,6 +727,45 @@ static int64_t write_pixel(const double candidate[3], const float original[3],
   return advances;
 }
 
+/* The serial writer over n sites (host_writer.py:73-93 order): each site's
+ * start state, its written values, the final state and the advances. */
+void dor_write_pixels(const double* cand, const float* orig, const uint8_t* floor,
+                      int64_t count, float low, float high, const float* scales,
+                      uint32_t state_in, uint32_t* states, float* values,
+                      uint32_t* state_out, uint64_t* advances_out) {
+  writer_consts K;
+  K.low = (double)low;
+  K.high = (double)high;
+  K.low_lt_high = K.low < K.high;
+  for (int c = 0; c < 3; ++c) K.scales[c] = scales[c];
+  uint32_t state = state_in & LCG24_MASK;
+  uint64_t advances = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    states[i] = state;
+    advances += (uint64_t)write_pixel(cand + i * 3, orig + i * 3, floor[i], &K, &state,
+                                      values + i * 3);
+  }
+  *state_out = state;
+  *advances_out = advances;
+}
+
+/* One site's writer step from each of n start states. */
+void dor_write_pixel_states(const double* cand, const float* orig, uint8_t floor, float low,
+                            float high, const float* scales, const uint32_t* states_in,
+                            int64_t count, uint32_t* states_out, uint8_t* advances) {
+  writer_consts K;
+  K.low = (double)low;
+  K.high = (double)high;
+  K.low_lt_high = K.low < K.high;
+  for (int c = 0; c < 3; ++c) K.scales[c] = scales[c];
+  for (int64_t i = 0; i < count; ++i) {
+    uint32_t state = states_in[i] & LCG24_MASK;
+    float values[3];
+    advances[i] = (uint8_t)write_pixel(cand, orig, floor, &K, &state, values);
+    states_out[i] = state;
+  }
+}
+
 /* ------------------------------------------------------------------------
  * Emit (output.py:61-94)
  * ---------------------------------------------------------------------- */

@@ -63,6 +63,13 @@ void dor_rgb_unscaled_averages(const float* patches, int64_t count,
 void dor_dither_chain(const double* values, const float* scales, int64_t count,
                       uint32_t state_in, float low, float high, double* deltas,
                       uint32_t* state_out, uint64_t* advances_out);
+void dor_write_pixels(const double* cand, const float* orig, const uint8_t* floor,
+                      int64_t count, float low, float high, const float* scales,
+                      uint32_t state_in, uint32_t* states, float* values,
+                      uint32_t* state_out, uint64_t* advances_out);
+void dor_write_pixel_states(const double* cand, const float* orig, uint8_t floor, float low,
+                            float high, const float* scales, const uint32_t* states_in,
+                            int64_t count, uint32_t* states_out, uint8_t* advances);
 int32_t dor_emit(const float* values, int64_t count, uint16_t* out);
 int32_t dor_planes(const uint16_t* rgbi, int64_t height, int64_t width,
                    const float* lut, const dice_params* params,

@@ -68,6 +68,10 @@ ORACLE_SIGNATURES = {
     "rgb_unscaled_averages": (None, [f32p, i64, f64p]),
     "dither_chain": (None, [f64p, f32p, i64, C.c_uint32, C.c_float, C.c_float,
                             f64p, u32p, u64p]),
+    "write_pixels": (None, [f64p, f32p, u8p, i64, C.c_float, C.c_float, f32p, C.c_uint32,
+                            u32p, f32p, u32p, u64p]),
+    "write_pixel_states": (None, [f64p, f32p, C.c_uint8, C.c_float, C.c_float, f32p, u32p, i64,
+                                  u32p, u8p]),
     "emit": (C.c_int32, [f32p, i64, u16p]),
     "planes": (C.c_int32, [u16p, i64, i64, f32p, C.POINTER(DiceParams), f32p,
                            u8p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p]),
