@@ -796,7 +796,7 @@ struct SitesShared {
 };
 
 template <int TY, int TX>
-__global__ __launch_bounds__(kThreads) void k_sites(JobSet set) {
+__global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
   using Sh = SitesShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC;
