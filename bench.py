@@ -27,9 +27,13 @@ Four legs:
 Every timed job's output SHA-256 and counters are compared with the compiled
 CPU oracle (the C restatement of the reference CPU path) run on the same
 frames.  That oracle run is also the CPU baseline, timed on this host's
-cores.  The roofline is that of the streaming analysis kernel k_scan:
-14 algorithmic bytes per pixel (8 B RGBI in + 6 B RGB out) over its
-HIP-event time on the library's stream.
+cores.  The roofline is that of the kernel with the largest share of device
+time, the reconstruction-site kernel k_sites: its algorithmic bytes (8 B
+RGBI read per pixel of the frame, one 40 B site record written per attempted
+site) over its HIP-event time on the library's stream, with its binary64
+work (about 1750 operations per site: 13 nine-column averages, the record
+divisions and the combiner) against the FP64 vector peak beside it.  The
+streaming kernel k_scan (14 B per pixel: 8 B in, 6 B out) is reported too.
 """
 
 from __future__ import annotations
@@ -50,7 +54,20 @@ sys.path.insert(0, str(ROOT / "digital-fauxice_amd"))
 
 METRIC = "bit-exact Mpix/s at 5782x3946 RGBI16 (batched vs plain); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_PIXEL = 14  # 8 B RGBI uint16 in + 6 B RGB uint16 out
+SITE_READ_BYTES_PER_PIXEL = 8  # k_sites: the RGBI of every pixel of its tiles
+SITE_RECORD_BYTES = 40  # k_sites: one Site record per attempted site
+# binary64 operations per attempted site in k_sites: 13 unscaled 9x9
+# averages (9 column scratches of 12 operations + 17 to combine them), the
+# RGB Q16 rows (13 each), 15 record divisions, 9 candidate scalings and the
+# combiner (about 50)
+SITE_FP64_OPS = 13 * (9 * 12 + 17) + 3 * 13 + 15 + 9 + 50
+# HBM bytes one k_sites<16, 64> launch moves on this workload: rocprofv3
+# FETCH_SIZE and WRITE_SIZE (two --pmc runs over the plain leg, one frame per
+# dispatch), each scaled by what it reads for the plain copy kernel k_copy
+# over 1 GiB (x2.000 and x1.000)
+SITES_TRAFFIC_BYTES = 339_300_000
 
 
 def parse_args() -> argparse.Namespace:
@@ -176,6 +193,11 @@ def main() -> int:
                 for k in stage_rows[0]}
     scan_s = stage_ms["scan_kernel_ms"] / 1e3
     achieved = BYTES_PER_PIXEL * height * width / scan_s / 1e9 if scan_s > 0 else 0.0
+    sites_s = stage_ms["sites_kernel_ms"] / 1e3
+    sites0 = int(plain_results[0][1])
+    sites_bytes = SITE_READ_BYTES_PER_PIXEL * height * width + SITE_RECORD_BYTES * sites0
+    sites_achieved = sites_bytes / sites_s / 1e9 if sites_s > 0 else 0.0
+    sites_tflops = SITE_FP64_OPS * sites0 / sites_s / 1e12 if sites_s > 0 else 0.0
 
     # ---- batched leg: B full frames per call
     batch_timings: list[dict] = []
@@ -507,12 +529,26 @@ def main() -> int:
         if stage_ms["total"] else None,
         "roofline": {
             "bound": "hbm",
+            "kernel": "k_sites",
+            "achieved": round(sites_achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(sites_achieved / HBM_PEAK_GBS, 4),
+            "traffic": SITES_TRAFFIC_BYTES,
+            "kernel_ms": round(stage_ms["sites_kernel_ms"], 4),
+            "algorithmic_bytes": sites_bytes,
+            "fp64_tflops": round(sites_tflops, 3),
+            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+            "fp64_frac": round(sites_tflops / FP64_PEAK_TFLOPS, 4),
+        },
+        "roofline_scan": {
+            "bound": "hbm",
             "kernel": "k_scan",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "kernel_ms": round(stage_ms["scan_kernel_ms"], 4),
         },
         "cpu_baseline": cpu_baseline,
     }
