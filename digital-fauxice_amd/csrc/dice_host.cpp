@@ -28,6 +28,7 @@
 #include "dice_device.hpp"
 #include "dice_kernels.hpp"
 #include "dice_runtime.hpp"
+#include "dice_routing.hpp"
 
 #pragma clang fp contract(off)
 
@@ -783,6 +784,78 @@ int32_t dice_t_writer_site_host(const double* cand, const float* orig, uint8_t f
     advances[i] = (uint8_t)draws;
   }
   return DICE_OK;
+}
+
+struct dice_routing {
+  int device = 0;
+  dice::RoutingOutput out;
+};
+
+int32_t dice_route_at_floor(dice_ctx* ctx, const uint8_t* at_floor, int64_t height,
+                            int64_t width, uint32_t flags, int64_t min_area, int64_t min_radius,
+                            int64_t margin, dice_routing** out) {
+  if (!out) return set_error(DICE_INVALID, "dice_route_at_floor: null output");
+  *out = nullptr;
+  if (!at_floor || height < 1 || width < 1 || height * width >= INT32_MAX)
+    return set_error(DICE_INVALID, "at_floor_mask must be a non-empty HxW plane below 2^31 pixels");
+  if (min_area < 1 || min_radius < 1 || margin < 0)
+    return set_error(DICE_INVALID, "routing policy needs min_area >= 1, min_radius >= 1, "
+                                   "margin >= 0");
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    auto routing = std::make_unique<dice_routing>();
+    routing->device = ctx->device;
+    const size_t n = (size_t)height * width;
+    DeviceBuffer staged;
+    const uint8_t* mask = at_floor;
+    if (!(flags & DICE_ROUTING_MASK_ON_DEVICE)) {
+      HIP_TRY(staged.ensure(n, ctx->stream));
+      HIP_TRY(hipMemcpyAsync(staged.ptr, at_floor, n, hipMemcpyHostToDevice, ctx->stream));
+      mask = staged.as<uint8_t>();
+    }
+    DICE_TRY(route_at_floor(ctx->stream, mask, height, width, min_area, min_radius, margin,
+                            routing->out));
+    *out = routing.release();
+    return DICE_OK;
+  });
+}
+
+int32_t dice_routing_summary_get(const dice_routing* routing, dice_routing_summary* summary) {
+  if (!routing || !summary) return set_error(DICE_INVALID, "null routing or summary");
+  *summary = routing->out.summary;
+  return DICE_OK;
+}
+
+int32_t dice_routing_tables(const dice_routing* routing, dice_provisional_region* provisional,
+                            dice_final_region* finals, int32_t* pairs) {
+  if (!routing) return set_error(DICE_INVALID, "null routing");
+  const RoutingOutput& o = routing->out;
+  if (provisional && !o.provisional.empty())
+    memcpy(provisional, o.provisional.data(), o.provisional.size() * sizeof(*provisional));
+  if (finals && !o.finals.empty())
+    memcpy(finals, o.finals.data(), o.finals.size() * sizeof(*finals));
+  if (pairs && !o.pairs.empty()) memcpy(pairs, o.pairs.data(), o.pairs.size() * sizeof(*pairs));
+  return DICE_OK;
+}
+
+int32_t dice_routing_planes(const dice_routing* routing, int32_t* provisional_labels,
+                            uint8_t* synthesis_mask, int32_t* final_labels) {
+  if (!routing) return set_error(DICE_INVALID, "null routing");
+  HIP_TRY(hipSetDevice(routing->device));
+  const RoutingOutput& o = routing->out;
+  const size_t n = (size_t)o.height * o.width;
+  if (provisional_labels)
+    HIP_TRY(hipMemcpy(provisional_labels, o.provisional_labels.ptr, n * 4, hipMemcpyDeviceToHost));
+  if (synthesis_mask)
+    HIP_TRY(hipMemcpy(synthesis_mask, o.synthesis_mask.ptr, n, hipMemcpyDeviceToHost));
+  if (final_labels)
+    HIP_TRY(hipMemcpy(final_labels, o.final_labels.ptr, n * 4, hipMemcpyDeviceToHost));
+  return DICE_OK;
+}
+
+void dice_routing_free(dice_routing* routing) {
+  if (!routing) return;
+  (void)hipSetDevice(routing->device);
+  delete routing;
 }
 
 int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count, uint16_t* out) {

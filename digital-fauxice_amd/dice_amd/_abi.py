@@ -257,6 +257,42 @@ class DiceDeviceInfo(C.Structure):
     ]
 
 
+class DiceRoutingSummary(C.Structure):
+    _fields_ = [
+        ("height", C.c_int64),
+        ("width", C.c_int64),
+        ("provisional_regions", C.c_int64),
+        ("final_regions", C.c_int64),
+        ("pairs", C.c_int64),
+        ("at_floor_pixels", C.c_int64),
+        ("synthesis_pixels", C.c_int64),
+        ("non_floor_synthesis_pixels", C.c_int64),
+        ("perimeter_excluded_id", C.c_int64),
+        ("perimeter_suppressed_halo_pixels", C.c_int64),
+    ]
+
+
+class DiceProvisionalRegion(C.Structure):
+    _fields_ = [
+        ("y0", C.c_int32), ("x0", C.c_int32), ("y1", C.c_int32), ("x1", C.c_int32),
+        ("area", C.c_int64),
+        ("max_chessboard_radius", C.c_int64),
+        ("synthesized_pixels", C.c_int64),
+        ("routed", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class DiceFinalRegion(C.Structure):
+    _fields_ = [
+        ("y0", C.c_int32), ("x0", C.c_int32), ("y1", C.c_int32), ("x1", C.c_int32),
+        ("area", C.c_int64),
+        ("at_floor_pixels", C.c_int64),
+    ]
+
+
+DICE_ROUTING_MASK_ON_DEVICE = 0x1
+
 VOIDP = C.c_void_p
 
 # Signatures of the HIP library entry points (symbol prefix "dice_").
@@ -319,6 +355,13 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "t_copy": (C.c_int32, [VOIDP, C.c_uint64, C.c_int32, f64p]),
     "t_writer_chain_host": (C.c_int32, [f64p, f32p, u8p, i64, C.c_float, C.c_float, f32p,
                                         C.c_uint32, C.c_int32, u32p, u32p, u64p]),
+    "route_at_floor": (C.c_int32, [VOIDP, VOIDP, i64, i64, C.c_uint32, i64, i64, i64,
+                                   C.POINTER(VOIDP)]),
+    "routing_summary_get": (C.c_int32, [VOIDP, C.POINTER(DiceRoutingSummary)]),
+    "routing_tables": (C.c_int32, [VOIDP, C.POINTER(DiceProvisionalRegion),
+                                   C.POINTER(DiceFinalRegion), i32p]),
+    "routing_planes": (C.c_int32, [VOIDP, i32p, u8p, i32p]),
+    "routing_free": (None, [VOIDP]),
     "t_writer_site_host": (C.c_int32, [f64p, f32p, C.c_uint8, C.c_float, C.c_float, f32p, u32p,
                                        i64, C.c_int32, u32p, u8p]),
 }

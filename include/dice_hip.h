@@ -410,6 +410,64 @@ int32_t dice_t_writer_site_host(const double* cand, const float* orig,
                                 int64_t count, int32_t generic,
                                 uint32_t* states_out, uint8_t* advances);
 
+/* ---- At-floor routing ------------------------------------------------------
+ * route_at_floor_mask (hybrid/src/fauxce_hybrid/routing.py:447-687) on the
+ * device, for the hybrid step after the core:
+ *   * provisional regions: the 8-connected components of the at-floor mask,
+ *     numbered in raster order of their first pixel, with area, half-open
+ *     bounding box and largest chessboard distance to in-frame background;
+ *   * routed = (area >= min_area || radius >= min_radius), except for a
+ *     component holding every perimeter pixel;
+ *   * the synthesis mask: the routed union dilated by a square of radius
+ *     `margin` (clipped to the frame), minus that perimeter component;
+ *   * final regions: its 8-connected components.  Those whose box grown by
+ *     128 px passes 2048 x 2048 pixels are cut on a 1792 px grid from the
+ *     box origin.  They are numbered by (y0, x0, y1, x1, creation order).
+ * The selection reasons, dispositions, JSON census and synthesis budget are
+ * assembled from these tables by the caller (dice_amd/routing.py).  The
+ * mask must have at least one pixel off the floor.  A routing holds its
+ * planes in HBM until dice_routing_free. */
+#define DICE_ROUTING_MASK_ON_DEVICE 0x1u
+
+typedef struct dice_routing dice_routing;
+
+typedef struct dice_routing_summary {
+  int64_t height, width;
+  int64_t provisional_regions, final_regions, pairs;
+  int64_t at_floor_pixels, synthesis_pixels, non_floor_synthesis_pixels;
+  int64_t perimeter_excluded_id;              /* 0: none */
+  int64_t perimeter_suppressed_halo_pixels;   /* dilated halo removed from it */
+} dice_routing_summary;
+
+typedef struct dice_provisional_region {      /* id = index + 1 */
+  int32_t y0, x0, y1, x1;
+  int64_t area;
+  int64_t max_chessboard_radius;
+  int64_t synthesized_pixels;                 /* its pixels inside the synthesis mask */
+  int32_t routed;                             /* directly routed */
+  int32_t reserved;
+} dice_provisional_region;
+
+typedef struct dice_final_region {            /* id = index + 1 */
+  int32_t y0, x0, y1, x1;
+  int64_t area;
+  int64_t at_floor_pixels;
+} dice_final_region;
+
+int32_t dice_route_at_floor(dice_ctx* ctx, const uint8_t* at_floor, int64_t height,
+                            int64_t width, uint32_t flags, int64_t min_area,
+                            int64_t min_radius, int64_t margin, dice_routing** out);
+int32_t dice_routing_summary_get(const dice_routing* routing, dice_routing_summary* summary);
+/* provisional[provisional_regions], finals[final_regions], pairs[2 * pairs]
+ * ((provisional id, final id) of every pair that shares an at-floor pixel
+ * of the synthesis mask, sorted); any pointer may be null. */
+int32_t dice_routing_tables(const dice_routing* routing, dice_provisional_region* provisional,
+                            dice_final_region* finals, int32_t* pairs);
+/* Host buffers of H*W; any may be null. */
+int32_t dice_routing_planes(const dice_routing* routing, int32_t* provisional_labels,
+                            uint8_t* synthesis_mask, int32_t* final_labels);
+void dice_routing_free(dice_routing* routing);
+
 /* Calibration: `repeats` launches of a plain 16-byte-vector copy kernel over
  * `bytes` (device buffers owned by the call); average milliseconds per copy. */
 int32_t dice_t_copy(dice_ctx* ctx, uint64_t bytes, int32_t repeats,
