@@ -10,15 +10,20 @@ the timed region.  One job covers:
 * the SHA-256 of the public RGB16 output, which needs the output bytes on
   the host.
 
-Four legs:
+Legs:
   plain    one job per call (dice_process_job), W warm-up + K timed jobs:
            p50/p99 latency and Mpix/s of the plain single-frame path;
-  batched  the headline `value`: B frames per dice_process_batch call,
-           persistent launches over the batch, device phases replayed from
-           captured hipGraphs, tile shape autotuned on the batch:
-           W warm-up + K timed batches;
+  batched  B frames per dice_process_batch call: persistent launches over
+           the batch, device phases replayed from captured hipGraphs, tile
+           shape autotuned on the batch; W warm-up + K timed batches;
+  pipelined  the headline `value`: K x B x rounds frames through the serving
+           engine (batches of B, `drivers` batches in flight, so the device
+           phases of one batch overlap the host work of the others);
   crops    16 crops of uneven size (sides 512-3000 px, aspect 0.5-2) as one
            batch (same machinery): Mpix/s and batch latency.
+  routing  the hybrid's at-floor routing of frame 0's at-floor mask on the
+           device (components, radii, dilation, partition), checked against
+           the SciPy oracle.
   serving  Poisson arrivals (70 % 1024-2048 px crops, 30 % full frames) at
            80 % of the batched capacity into the serving engine (continuous
            batching, HBM-budget admission, two drivers): p50/p99 latency and
@@ -85,6 +90,8 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     parser.add_argument("--serving-jobs", type=int, default=40)
+    parser.add_argument("--no-routing", action="store_true",
+                        help="skip the at-floor routing leg")
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -412,6 +419,33 @@ def main() -> int:
         del lat
         log(f"serving: {serving}")
 
+    # ---- at-floor routing of frame 0's at-floor mask (the hybrid's next
+    # step), on the device; the oracle (SciPy) checks the census
+    routing = None
+    if not args.no_routing:
+        from dice_amd.hip_backend import routing as hip_routing
+        from dice_amd.hip_backend.process import process_hip
+        from dice_amd.profile import selector8_job
+
+        at_floor = process_hip(selector8_job(full.mains[0], full.prepasses[0]),
+                               export_diagnostics=True).diagnostics.at_floor_mask
+        for _ in range(2):
+            hip_routing.route_at_floor_mask(at_floor)
+        route_lat = []
+        for _ in range(max(3, args.steps)):
+            a = time.perf_counter()
+            routed = hip_routing.route_at_floor_mask(at_floor)
+            route_lat.append((time.perf_counter() - a) * 1e3)
+        route_ms = statistics.median(route_lat)
+        routing = {
+            "at_floor_pixels": routed.at_floor_pixel_count,
+            "provisional_regions": len(routed.provisional_regions),
+            "final_regions": len(routed.final_regions),
+            "ms": round(route_ms, 3),
+            "mpix_per_s": round(frame_mpix / route_ms * 1e3, 2),
+        }
+        log(f"routing: {routing}")
+
     # ---- CPU baseline (the oracle), which also checks every output
     cpu_baseline = None
     bit_exact = None
@@ -453,6 +487,16 @@ def main() -> int:
                                 for p, (f, i) in zip(served_prints, picks))
             serving["bit_exact"] = serving_exact
             bit_exact = bit_exact and serving_exact
+        if routing is not None:
+            # the routing census against the SciPy oracle of the same mask
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import routing_oracle
+
+            a = time.perf_counter()
+            want = routing_oracle.route(at_floor)[0]
+            routing["cpu_ms"] = round((time.perf_counter() - a) * 1e3, 1)
+            routing["bit_exact"] = want == hip_routing.routing_json_document(routed)
+            bit_exact = bit_exact and routing["bit_exact"]
         timed = runs[1:] if len(runs) > 1 else runs
         median = statistics.median(timed)
         cpu_baseline = {
@@ -523,6 +567,7 @@ def main() -> int:
         "batched_vs_plain": round(value / plain_value, 3),
         "crops": crops,
         "serving": serving,
+        "routing": routing,
         "bit_exact": bit_exact,
         "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
         "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)

@@ -17,7 +17,7 @@ import numpy as np
 import numpy.typing as npt
 
 from ._abi import DicePrepassParams
-from .contracts import DualRGBIAcquisition
+from .contracts import AcquisitionEpoch, DualRGBIAcquisition, RGBI16Frame
 from .parameters import (
     AuxiliaryParameters,
     DecisionParameters,
@@ -213,3 +213,23 @@ def neutral_prepass_record() -> PrepassRecord:
     payload = bytearray(0x40)
     struct.pack_into("<f", payload, 0x04, 0.17)
     return PrepassRecord(bytes(payload))
+
+
+def selector8_job(main: npt.NDArray[np.uint16], prepass: npt.NDArray[np.uint16],
+                  frame_id: str = "job") -> ProcessingJob:
+    """The supported acquisition of one frame: a 4000 dpi main pass and its
+    285 dpi prepass (contracts.py:25-74), LS-5000 selector-8 Normal."""
+
+    return ProcessingJob(
+        acquisition=DualRGBIAcquisition(
+            prepass=RGBI16Frame(prepass, AcquisitionEpoch.PREPASS, 285, f"{frame_id}-prepass"),
+            main=RGBI16Frame(main, AcquisitionEpoch.MAIN, 4000, f"{frame_id}-main"),
+            same_frame_id=frame_id,
+        ),
+        scanner_model=ScannerModel.NIKON_SUPER_COOLSCAN_5000_ED,
+        mode=ProcessingMode.NORMAL,
+        selector=8,
+        resolution_metric=4000,
+        bit_depth=16,
+        focus_exposure_locked=True,
+    )
