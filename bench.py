@@ -24,6 +24,10 @@ Legs:
   routing  the hybrid's at-floor routing of frame 0's at-floor mask on the
            device (components, radii, dilation, partition), checked against
            the SciPy oracle.
+  stream   frame 0 handed over as 723 blocks of 8 rows (the scanner's
+           transport) through a frame stream: the device phases and the
+           serial writer run as the blocks arrive; the latency from the
+           last block to the result, against the plain whole-frame latency.
   serving  Poisson arrivals (70 % 1024-2048 px crops, 30 % full frames) at
            80 % of the batched capacity into the serving engine (continuous
            batching, HBM-budget admission, two drivers): p50/p99 latency and
@@ -92,6 +96,8 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--serving-jobs", type=int, default=40)
     parser.add_argument("--no-routing", action="store_true",
                         help="skip the at-floor routing leg")
+    parser.add_argument("--no-stream", action="store_true",
+                        help="skip the 8-row block streaming leg")
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -446,6 +452,66 @@ def main() -> int:
         }
         log(f"routing: {routing}")
 
+    # ---- frame 0 arriving as 8-row blocks through a frame stream
+    streamed = None
+    if not args.no_stream:
+        handle = C.c_void_p()
+        engine.raise_for(lib.dice_stream_create(ctx, C.byref(handle)))
+        main0 = np.ascontiguousarray(full.mains[0])
+        out0 = np.empty((height, width, 3), np.uint16)
+        prepass0 = np.ascontiguousarray(full.prepasses[0])
+        blocks = [(row, np.ascontiguousarray(main0[row:row + 8])) for row in range(0, height, 8)]
+
+        def stream_frame():
+            res = DiceReplayResult()
+            job = DiceJob()
+            job.main_rgbi = None
+            job.height, job.width = height, width
+            job.prepass_rgbi = prepass0.ctypes.data
+            job.prepass_height, job.prepass_width = prepass0.shape[:2]
+            job.params = C.pointer(params)
+            job.prepass_params = C.pointer(prepass_params)
+            job.flags = 0
+            job.out_rgb16 = out0.ctypes.data
+            job.result = C.pointer(res)
+            engine.raise_for(lib.dice_stream_begin(handle, C.byref(job), None))
+            pushes = []
+            frame_start = time.perf_counter()
+            for row, block in blocks:
+                a = time.perf_counter()
+                engine.raise_for(lib.dice_stream_push(handle, ptr(block, C.c_uint16), row,
+                                                      block.shape[0]))
+                pushes.append(time.perf_counter() - a)
+            a = time.perf_counter()
+            engine.raise_for(lib.dice_stream_finish(handle))
+            done = time.perf_counter()
+            return (fingerprint(res), pushes, (done - a + pushes[-1]) * 1e3,
+                    (done - frame_start) * 1e3)
+
+        stream_frame()
+        runs = [stream_frame() for _ in range(max(3, min(args.steps, 5)))]
+        progress = _abi.DiceStreamProgress()
+        lib.dice_stream_progress_get(handle, C.byref(progress))
+        lib.dice_stream_destroy(handle)
+        stream_print = runs[0][0]
+        tails = [r[2] for r in runs]
+        frames_ms = [r[3] for r in runs]
+        pushes = [p * 1e3 for r in runs for p in r[1]]
+        streamed = {
+            "blocks": len(blocks),
+            "rows_per_block": 8,
+            "windows": int(progress.windows),
+            "last_block_to_result_ms_p50": round(statistics.median(tails), 3),
+            "last_block_to_result_ms_max": round(max(tails), 3),
+            "push_ms_mean": round(statistics.mean(pushes), 4),
+            "push_ms_p99": round(float(np.percentile(pushes, 99)), 4),
+            "frame_ms_back_to_back": round(statistics.median(frames_ms), 2),
+            "ingest_mpix_per_s": round(frame_mpix / statistics.median(frames_ms) * 1e3, 2),
+            "plain_p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+            "identical_to_plain": all(r[0] == plain_results[0] for r in runs),
+        }
+        log(f"stream: {streamed}")
+
     # ---- CPU baseline (the oracle), which also checks every output
     cpu_baseline = None
     bit_exact = None
@@ -487,6 +553,9 @@ def main() -> int:
                                 for p, (f, i) in zip(served_prints, picks))
             serving["bit_exact"] = serving_exact
             bit_exact = bit_exact and serving_exact
+        if streamed is not None:
+            streamed["bit_exact"] = stream_print == expected[0]
+            bit_exact = bit_exact and streamed["bit_exact"]
         if routing is not None:
             # the routing census against the SciPy oracle of the same mask
             sys.path.insert(0, str(ROOT / "oracle"))
@@ -568,6 +637,7 @@ def main() -> int:
         "crops": crops,
         "serving": serving,
         "routing": routing,
+        "stream": streamed,
         "bit_exact": bit_exact,
         "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
         "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)

@@ -315,7 +315,41 @@ struct JobDev {
   double* cumulative;           // H * 3
   double* epoch_wide;           // epochs * 2 (den_w, num_w)
   float* epoch_store;           // epochs * 2 (den32, num32)
+  // The part of the job the device phases cover.  A whole-frame run covers
+  // everything; a frame streamed in as row blocks advances the window as its
+  // rows arrive (every phase only reads rows, records and stages that are
+  // complete when it runs).
+  struct Window {
+    int32_t block_lo, block_n;    // 8-row producer blocks
+    int32_t rowsum_lo, rowsum_n;  // row-sum units of kRowSumRows rows
+    int32_t chain_lo, chain_hi;   // rows of the cumulative chain (epochs follow)
+    int32_t stage_lo, stage_hi;   // stage-table entries
+    int32_t trow_lo, trow_n;      // analysis tile rows
+    uint32_t site_lo;             // first main site of the draw schedules
+    int32_t startup;              // 1: the hidden startup runs in this window
+  } win;
 };
+
+// Rows per k_row_sums work unit.
+constexpr int kRowSumRows = 16;
+
+// The window of a whole-frame run.
+inline JobDev::Window full_window(int32_t height, int32_t tiles_y) {
+  JobDev::Window w{};
+  w.block_lo = 0;
+  w.block_n = (height + 7) / 8;
+  w.rowsum_lo = 0;
+  w.rowsum_n = (height + kRowSumRows - 1) / kRowSumRows;
+  w.chain_lo = 0;
+  w.chain_hi = height;
+  w.stage_lo = 0;
+  w.stage_hi = height + 6;
+  w.trow_lo = 0;
+  w.trow_n = tiles_y;
+  w.site_lo = 0;
+  w.startup = 1;
+  return w;
+}
 
 // A batch of jobs for one launch: work unit u of the launch belongs to the
 // job j with base[j] <= u < base[j+1] (empty ranges allowed), and is unit

@@ -29,6 +29,7 @@
 #include "dice_kernels.hpp"
 #include "dice_runtime.hpp"
 #include "dice_routing.hpp"
+#include "dice_stream.hpp"
 
 #pragma clang fp contract(off)
 
@@ -856,6 +857,108 @@ void dice_routing_free(dice_routing* routing) {
   if (!routing) return;
   (void)hipSetDevice(routing->device);
   delete routing;
+}
+
+// ---- frame streams (contracts.py:108-158, 224-261)
+
+struct dice_stream {
+  int device = 0;
+  std::unique_ptr<FrameStream> frames;
+  dice_job* job = nullptr;
+  double start = 0.0;
+};
+
+namespace {
+// The frame's status into its dice_job (a refused block is the caller's
+// error, not the frame's: it leaves the frame as it was).
+int32_t report_stream(dice_stream* stream, int32_t rc) {
+  if (dice_job* job = stream->job) {
+    const JobRequest& R = stream->frames->request();
+    job->status = R.code;
+    snprintf(job->error, sizeof job->error, "%s", R.error.c_str());
+  }
+  return rc;
+}
+
+// A frame that could not begin.
+int32_t refuse_begin(dice_job* job, int32_t rc) {
+  job->status = rc;
+  snprintf(job->error, sizeof job->error, "%s", error_buffer());
+  return rc;
+}
+}  // namespace
+
+int32_t dice_stream_create(dice_ctx* ctx, dice_stream** out) {
+  if (!out) return set_error(DICE_INVALID, "null stream pointer");
+  *out = nullptr;
+  DICE_TRY(activate(ctx));
+  auto stream = std::make_unique<dice_stream>();
+  stream->device = ctx->device;
+  stream->frames = std::make_unique<FrameStream>(ctx->device, &ctx->tables);
+  DICE_TRY(stream->frames->init());
+  *out = stream.release();
+  return DICE_OK;
+}
+
+int32_t dice_stream_begin(dice_stream* stream, dice_job* job, dice_diagnostics* diagnostics) {
+  if (!stream || !job) return set_error(DICE_INVALID, "null stream or job");
+  HIP_TRY(hipSetDevice(stream->device));
+  stream->job = nullptr;  // until the frame has begun
+  stream->start = now_seconds();
+  job->status = DICE_OK;
+  job->error[0] = 0;
+  if (job->timings) memset(job->timings, 0, sizeof *job->timings);
+  if (!job->params || !job->prepass_params || !job->result || !job->out_rgb16)
+    return refuse_begin(job, set_error(DICE_INVALID, "stream job is missing a required pointer"));
+  if (job->main_rgbi)
+    return refuse_begin(job, set_error(DICE_INVALID,
+                                       "a streamed job's main rows arrive through "
+                                       "dice_stream_push (main_rgbi must be null)"));
+  memset(job->result, 0, sizeof *job->result);
+  double prepass_seconds = 0.0;
+  if (int32_t rc = admit_job(job->prepass_rgbi, job->prepass_height, job->prepass_width,
+                             job->height, job->width, *job->params, *job->prepass_params,
+                             &prepass_seconds))
+    return refuse_begin(job, rc);
+  if (job->timings) job->timings->prepass = prepass_seconds;
+  JobRequest R;
+  R.height = job->height;
+  R.width = job->width;
+  R.flags = job->flags;
+  R.out = job->out_rgb16;
+  R.params = *job->params;
+  R.diag = diagnostics;
+  R.result = job->result;
+  R.timings = job->timings;
+  if (int32_t rc = stream->frames->begin(R)) return refuse_begin(job, rc);
+  stream->job = job;
+  return DICE_OK;
+}
+
+int32_t dice_stream_push(dice_stream* stream, const uint16_t* rgbi, int64_t start_row,
+                         int64_t rows) {
+  if (!stream) return set_error(DICE_INVALID, "null stream");
+  return report_stream(stream, stream->frames->push(rgbi, start_row, rows));
+}
+
+int32_t dice_stream_finish(dice_stream* stream) {
+  if (!stream) return set_error(DICE_INVALID, "null stream");
+  const int32_t rc = report_stream(stream, stream->frames->finish());
+  if (stream->job && stream->job->timings)
+    stream->job->timings->total = now_seconds() - stream->start;
+  return rc;
+}
+
+int32_t dice_stream_progress_get(const dice_stream* stream, dice_stream_progress* progress) {
+  if (!stream || !progress) return set_error(DICE_INVALID, "null stream or progress");
+  *progress = stream->frames->progress();
+  return DICE_OK;
+}
+
+void dice_stream_destroy(dice_stream* stream) {
+  if (!stream) return;
+  (void)hipSetDevice(stream->device);
+  delete stream;
 }
 
 int32_t dice_t_emit(dice_ctx* ctx, const float* values, int64_t count, uint16_t* out) {

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobSet set
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const int H = J.height, W = J.width;
     const int cells = W / 8;
-    const int b = unit;
+    const int b = J.win.block_lo + unit;
     const bool epoch = b < H / 8;
     const int tid = threadIdx.x;
     const double threshold = (double)kProducerThreshold;
@@ -347,7 +347,6 @@ __global__ __launch_bounds__(kProducerThreads) void k_producer_blocks(JobSet set
 // A rejected pixel has weight 0 and three +0.0 terms, the exact identity of
 // the non-negative sums; so do the padding columns and rows past the frame,
 // whose loads are clamped into the frame.
-constexpr int kRowSumRows = 16;
 constexpr int kRowSumLanes = kRowSumRows * 3;
 constexpr int kRowSumPitch = kRowSumLanes + 1;  // odd pitch: conflict-free column stores
 constexpr int kRowSumCols = 128;
@@ -367,7 +366,7 @@ __global__ __launch_bounds__(kRowSumThreads) void k_row_sums(JobSet set) {
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const int H = J.height, W = J.width;
     const int cells = W / 8, active = cells * 8;
-    const int row0 = unit * kRowSumRows;
+    const int row0 = (J.win.rowsum_lo + unit) * kRowSumRows;
     const int tid = threadIdx.x;
     const int chunks = (active + kRowSumCols - 1) / kRowSumCols;
     const int steps = chunks == 0 ? 0 : chunks + 3;
@@ -474,7 +473,10 @@ __global__ __launch_bounds__(kRowSumThreads) void k_row_sums(JobSet set) {
 // 358-360, 437-456).  Workgroup 0: the cumulative row sums (lanes 0..2);
 // workgroup 1: the epoch accumulators (lane 0).  Each chain reads its
 // inputs through LDS in chunks the whole wave loads (the next chunk in
-// flight while the current one is summed).
+// flight while the current one is summed).  The chains cover the window's
+// rows [chain_lo, chain_hi) and its whole epochs, resuming from their own
+// outputs of the row / epoch before (a streamed frame extends them as its
+// rows arrive).
 constexpr int kChainChunk = 512;
 
 __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
@@ -485,9 +487,10 @@ __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
     const int lane = threadIdx.x;
     constexpr int kPer = kChainChunk * 3 / 64;
     if (unit == 0) {
-      const int H = J.height;
+      const int H = J.win.chain_hi;
       const double* __restrict__ source = J.row_sums;
-      double total = 0.0;
+      const int first = J.win.chain_lo;
+      double total = first > 0 && lane < 3 ? J.cumulative[(size_t)(first - 1) * 3 + lane] : 0.0;
       double next[kPer];
       auto load = [&](int base) {
         const int n = min(kChainChunk, H - base) * 3;
@@ -497,8 +500,8 @@ __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
           next[j] = i < n ? source[(size_t)base * 3 + i] : 0.0;
         }
       };
-      load(0);
-      for (int base = 0; base < H; base += kChainChunk) {
+      if (first < H) load(first);
+      for (int base = first; base < H; base += kChainChunk) {
         const int n = min(kChainChunk, H - base);
 #pragma unroll
         for (int j = 0; j < kPer; ++j) in_buffer[lane + j * 64] = next[j];
@@ -516,10 +519,12 @@ __global__ __launch_bounds__(64) void k_producer_chain(JobSet set) {
         __syncthreads();
       }
     } else {
-      const int epochs = J.height / 8;
+      const int epochs = J.win.chain_hi / 8;
+      const int first = J.win.chain_lo / 8;
       const double* __restrict__ source = J.epoch_adds;
-      float denominator = 0.0f, numerator = 0.0f;
-      for (int base = 0; base < epochs; base += kChainChunk) {
+      float denominator = first > 0 ? J.epoch_store[(size_t)(first - 1) * 2 + 0] : 0.0f;
+      float numerator = first > 0 ? J.epoch_store[(size_t)(first - 1) * 2 + 1] : 0.0f;
+      for (int base = first; base < epochs; base += kChainChunk) {
         const int n = min(kChainChunk, epochs - base);
         for (int i = lane; i < n * 2; i += 64) in_buffer[i] = source[(size_t)base * 2 + i];
         __syncthreads();
@@ -610,9 +615,8 @@ __global__ __launch_bounds__(kThreads) void k_schedule_rows(JobDev J,
 __global__ __launch_bounds__(kThreads) void k_stage_table(JobSet set) {
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const float factor_b = J.factor_b;
-    const int i = unit * blockDim.x + threadIdx.x;
-    const int count = J.height + 6;
-    if (i >= count) return;
+    const int i = J.win.stage_lo + unit * blockDim.x + threadIdx.x;
+    if (i >= J.win.stage_hi) return;
     const int row = i < J.height - 1 ? i : J.height - 1;
     RowRecord r = row_record(J, row);
     float p1 = r.p1, p2 = r.p2, p3 = r.mean_visible, p4 = r.mean_infrared;
@@ -690,7 +694,7 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
   __shared__ Sh S;
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
-    const int t = unit;
+    const int t = J.win.trow_lo * J.tiles_x + unit;
     const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
     const int H = J.height, W = J.width;
     const int y0 = tile_y * TY, x0 = tile_x * TX;
@@ -803,7 +807,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
   __shared__ Sh S;
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
-    const int t = unit;
+    const int t = J.win.trow_lo * J.tiles_x + unit;
     const uint32_t total = J.tile_count[t];
     if (total == 0) return;
     const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
@@ -954,11 +958,21 @@ __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
 // (and writing the total), and the per-block rescan with its offset.
 constexpr int kSegPerBlock = 1024;
 
+// The window's segments: rows [trow_lo, trow_lo + trow_n) x TY, every tile
+// column; the scan continues from the sites of the rows before
+// (counters[0], zero for a whole frame).
+__device__ __forceinline__ long window_segment_lo(const JobDev& J) {
+  return (long)J.win.trow_lo * J.tile_rows * J.tiles_x;
+}
+__device__ __forceinline__ long window_segments(const JobDev& J) {
+  return (long)J.win.trow_n * J.tile_rows * J.tiles_x;
+}
+
 __global__ __launch_bounds__(kThreads) void k_seg_partials(JobSet set) {
   __shared__ uint32_t wave_sums[kThreads / 64];
   for_each_unit(set, [&](const JobDev& J, const int unit) {
-    const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
-    const long base = (long)unit * kSegPerBlock;
+    const long n = window_segment_lo(J) + window_segments(J);
+    const long base = window_segment_lo(J) + (long)unit * kSegPerBlock;
     uint32_t sum = 0;
 #pragma unroll
     for (int j = 0; j < kSegPerBlock / kThreads; ++j) {
@@ -974,9 +988,10 @@ __global__ __launch_bounds__(kThreads) void k_seg_partials(JobSet set) {
 __global__ __launch_bounds__(kThreads) void k_seg_top(JobSet set) {
   __shared__ uint32_t wave_sums[kThreads / 64];
   for_each_unit(set, [&](const JobDev& J, const int unit) {
-    const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+    const long segments = window_segments(J);
     const int blocks = (int)((segments + kSegPerBlock - 1) / kSegPerBlock);
-    uint32_t carry = 0;
+    uint32_t carry = J.counters[0];
+    __syncthreads();  // every thread has read the carry before thread 0 replaces it
     for (int base = 0; base < blocks; base += kThreads) {
       const int i = base + threadIdx.x;
       const uint32_t value = i < blocks ? J.seg_block_sums[i] : 0u;
@@ -995,10 +1010,11 @@ __global__ __launch_bounds__(kThreads) void k_seg_top(JobSet set) {
 __global__ __launch_bounds__(kThreads) void k_seg_offsets(JobSet set) {
   __shared__ uint32_t wave_sums[kThreads / 64];
   for_each_unit(set, [&](const JobDev& J, const int unit) {
-    const long n = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+    const long n = window_segment_lo(J) + window_segments(J);
     constexpr int kPer = kSegPerBlock / kThreads;
     // each thread owns kPer consecutive segments so the rescan stays ordered
-    const long first = (long)unit * kSegPerBlock + (long)threadIdx.x * kPer;
+    const long first =
+        window_segment_lo(J) + (long)unit * kSegPerBlock + (long)threadIdx.x * kPer;
     uint32_t values[kPer];
     uint32_t sum = 0;
 #pragma unroll
@@ -1177,7 +1193,8 @@ __global__ __launch_bounds__(kThreads) void k_zero_jobs(JobSet set) {
   });
 }
 
-// Integer draw schedule (writer_record) of every main site and every
+// Integer draw schedule (writer_record) of the main sites [win.site_lo,
+// counters[0]) and, when the window holds the start of the frame, of every
 // hidden-startup slot of every job; the main site count is read on the
 // device, so the launch geometry does not depend on it.
 __global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
@@ -1186,11 +1203,12 @@ __global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
   for (int j = 0; j < set.count; ++j) {
     const JobDev& J = job_at(set, j);
     if (J.skip) continue;
-    const uint32_t count = J.counters[0];
-    const uint32_t total = count + 6u * (uint32_t)J.width;
+    const uint32_t lo = J.win.site_lo;
+    const uint32_t count = J.counters[0] - lo;
+    const uint32_t total = count + (J.win.startup ? 6u * (uint32_t)J.width : 0u);
     for (uint32_t i = first; i < total; i += stride) {
       if (i < count) {
-        J.writer_records[i] = writer_record(J.sites[i], true, J.K);
+        J.writer_records[lo + i] = writer_record(J.sites[lo + i], true, J.K);
       } else {
         const uint32_t k = i - count;
         J.su_writer_records[k] = writer_record(J.su_sites[k], J.su_attempted[k] != 0, J.K);
@@ -1568,7 +1586,7 @@ void launch_persistent(Kernel kernel, const JobSet& set, int threads, hipStream_
 }
 
 uint32_t segment_blocks(const JobDev& J) {
-  const long segments = (long)J.tiles_y * J.tile_rows * J.tiles_x;
+  const long segments = (long)J.win.trow_n * J.tile_rows * J.tiles_x;
   return (uint32_t)((segments + kSegPerBlock - 1) / kSegPerBlock);
 }
 }  // namespace
@@ -1584,16 +1602,16 @@ hipError_t launch_producer(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                            hipStream_t stream) {
   auto when = [](const JobDev& J, long units) { return J.run_producer ? units : 0L; };
   launch_persistent(k_producer_blocks, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
-                      return when(J, (J.height + 7) / 8);
+                      return when(J, J.win.block_n);
                     }), kProducerThreads, stream);
   launch_persistent(k_row_sums, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
-                      return when(J, blocks_for(J.height, kRowSumRows));
+                      return when(J, J.win.rowsum_n);
                     }), kRowSumThreads, stream);
   launch_persistent(k_producer_chain, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
-                      return when(J, 2);
+                      return when(J, J.win.chain_hi > J.win.chain_lo ? 2 : 0);
                     }), 64, stream);
   launch_persistent(k_stage_table, make_set(d_jobs, h_jobs, n, [&](const JobDev& J) {
-                      return when(J, blocks_for(J.height + 6, kThreads));
+                      return when(J, blocks_for(J.win.stage_hi - J.win.stage_lo, kThreads));
                     }), kThreads, stream);
   return hipGetLastError();
 }
@@ -1663,7 +1681,7 @@ hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStr
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
   if (!any) return hipSuccess;
   const JobSet set = make_set(d_jobs, h_jobs, n,
-                              [](const JobDev& J) { return J.tiles_x * J.tiles_y; });
+                              [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
 #define DICE_MATCH(a, b)                 \
   if (ty == a && tx == b) {              \
     launch_scan_shape<a, b>(set, stream); \
@@ -1690,7 +1708,7 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipSt
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
   if (!any) return hipSuccess;
   const JobSet set = make_set(d_jobs, h_jobs, n,
-                              [](const JobDev& J) { return J.tiles_x * J.tiles_y; });
+                              [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
 #define DICE_MATCH(a, b)                  \
   if (ty == a && tx == b) {               \
     launch_sites_shape<a, b>(set, stream); \
@@ -1704,10 +1722,12 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipSt
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                           hipStream_t stream) {
   launch_persistent(k_startup_rows, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
-                      return blocks_for((long)(J.height < 5 ? J.height : 5) * J.width, kThreads);
+                      return J.win.startup ? blocks_for((long)(J.height < 5 ? J.height : 5) *
+                                                            J.width, kThreads)
+                                           : 0u;
                     }), kThreads, stream);
   launch_persistent(k_startup_sites, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
-                      return blocks_for(6L * J.width, kThreads);
+                      return J.win.startup ? blocks_for(6L * J.width, kThreads) : 0u;
                     }), kThreads, stream);
   return hipGetLastError();
 }

@@ -18,13 +18,13 @@
 #include <thread>
 
 #include "dice_kernels.hpp"
+#include "dice_workspace.hpp"
 
 #pragma clang fp contract(off)
 
 namespace dice {
 
 namespace {
-constexpr int kMaxBands = 64;
 
 // DICE_TRACE=1 prints one timeline line per batch to stderr.
 bool trace_enabled() {
@@ -79,25 +79,6 @@ WriterConsts writer_consts(const dice_params& p) {
   return K;
 }
 
-namespace {
-
-int32_t flag_error(uint32_t flags) {
-  if (flags & kErrNonfiniteAux)
-    return set_error(DICE_NONFINITE_AUX, "auxiliary score input must be a finite HxW plane");
-  if (flags & kErrSetterScaleOne) return set_error(DICE_INVALID, "X3A setter scale cannot equal one");
-  if (flags & kErrStageNonfinite)
-    return set_error(DICE_INVALID, "stage calibration contains a non-finite value");
-  if (flags & kErrNonfiniteWork) return set_error(DICE_NONFINITE_WORK, "work RGB must be finite");
-  if (flags & kErrWriterChain)
-    return set_error(DICE_INTERNAL, "writer state chain disagrees with the per-site writer");
-  return DICE_OK;
-}
-
-// Band b of H rows in `bands` equal parts.
-inline int64_t band_row(int64_t H, int bands, int b) { return H * b / bands; }
-
-}  // namespace
-
 int32_t DeviceTables::upload() {
   DICE_TRY(require_tables());
   const Tables& t = tables();
@@ -112,79 +93,6 @@ int32_t DeviceTables::upload() {
   HIP_TRY(hipMemcpy(noop.ptr, t.noop_emit.data(), 65536 * 2, hipMemcpyHostToDevice));
   return DICE_OK;
 }
-
-// --------------------------------------------------------------------------
-// Workspace: the device and pinned buffers of one job slot, grown on demand
-// and kept for the next batch.
-// --------------------------------------------------------------------------
-
-struct Workspace {
-  DeviceBuffer rgbi, out, site_lists, tile_count, seg_count, seg_offset, seg_block_sums,
-      row_start, counters, flags;
-  DeviceBuffer diag_score, diag_floor, diag_changed;
-  DeviceBuffer su_aux, su_score, su_work, su_sites, su_attempted;
-  DeviceBuffer failpos, row_sums, accepted, epoch_adds, cumulative, epoch_wide, epoch_store;
-  DeviceBuffer st_alpha, st_offset, st_replacement, st_has, st_score_base, st_coarse, st_gate,
-      st_row_gate;
-  DeviceBuffer sites, records, su_records, states, writer_counts;
-  DeviceBuffer chain;        // ChainState, then the LCG state after each band
-  PinnedBuffer h_chain;      // the same on the host
-  PinnedBuffer h_small;      // [0,64) counters, [64,128) flags, [128, ...) writer slots
-  PinnedBuffer h_records;    // 6W startup records, then the main records
-  PinnedBuffer h_row_start;  // H+1
-  PinnedBuffer h_states;     // 6W startup states, then the main states
-  PinnedBuffer h_out;        // staging of the output rows on their way to the hash and caller
-  hipEvent_t band[kMaxBands] = {};     // writer values of band b done (runner stream)
-  hipEvent_t fetched[kMaxBands] = {};  // draw schedules of band b on the host (copy stream)
-  hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (output stream)
-  hipEvent_t startup_fetched = nullptr;
-  // the device writer chain of this job and its writer values, band by band
-  hipStream_t chain_stream = nullptr;
-  int bands = 1;
-  bool events = false;
-  // chain -> copy scheduler -> hash hand-off: `published` counts the bands
-  // whose output copy is queued, `aborted` that the chain failed
-  std::mutex mutex;
-  std::condition_variable cv;
-  int published = 0;
-  bool aborted = false;
-
-  ~Workspace() {
-    for (hipEvent_t* set : {band, fetched, copied})
-      for (int b = 0; b < kMaxBands; ++b)
-        if (set[b]) (void)hipEventDestroy(set[b]);
-    if (startup_fetched) (void)hipEventDestroy(startup_fetched);
-    if (chain_stream) {
-      (void)hipStreamSynchronize(chain_stream);
-      (void)hipStreamDestroy(chain_stream);
-    }
-  }
-  int32_t init_events() {
-    if (events) return DICE_OK;
-    // worker threads wait on these: block instead of spinning, so waiting
-    // threads leave the cores to the chains and hashes
-    const unsigned flags = hipEventDisableTiming | hipEventBlockingSync;
-    for (hipEvent_t* set : {band, fetched, copied})
-      for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
-    HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
-    HIP_TRY(hipStreamCreateWithFlags(&chain_stream, hipStreamNonBlocking));
-    events = true;
-    return DICE_OK;
-  }
-  uint64_t bytes() const {
-    const DeviceBuffer* all[] = {&rgbi, &out, &site_lists, &tile_count, &seg_count, &seg_offset,
-                                 &seg_block_sums, &row_start, &counters, &flags, &diag_score,
-                                 &diag_floor, &diag_changed, &su_aux, &su_score, &su_work,
-                                 &su_sites, &su_attempted, &failpos, &row_sums, &accepted,
-                                 &epoch_adds, &cumulative, &epoch_wide, &epoch_store,
-                                 &st_alpha, &st_offset, &st_replacement, &st_has,
-                                 &st_score_base, &st_coarse, &st_gate, &st_row_gate, &sites,
-                                 &records, &su_records, &states, &writer_counts, &chain};
-    uint64_t total = 0;
-    for (const DeviceBuffer* b : all) total += b->bytes;
-    return total;
-  }
-};
 
 // --------------------------------------------------------------------------
 // Runner
@@ -257,6 +165,7 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     J.tile_cols = TX;
     J.tiles_x = (int32_t)((W + TX - 1) / TX);
     J.tiles_y = (int32_t)((H + TY - 1) / TY);
+    J.win = full_window(J.height, J.tiles_y);
     const size_t pixels = (size_t)H * W;
     const size_t tiles = (size_t)J.tiles_x * J.tiles_y;
     const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
@@ -347,7 +256,8 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
       J.rgbi = R.main;
     } else {
       HIP_TRY(w.rgbi.ensure(pixels * 8, stream_));
-      HIP_TRY(hipMemcpyAsync(w.rgbi.ptr, R.main, pixels * 8, hipMemcpyHostToDevice, stream_));
+      if (!R.streamed)
+        HIP_TRY(hipMemcpyAsync(w.rgbi.ptr, R.main, pixels * 8, hipMemcpyHostToDevice, stream_));
       J.rgbi = w.rgbi.as<uint16_t>();
     }
     if (R.flags & DICE_OUT_ON_DEVICE) {
@@ -761,19 +671,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         const double f0 = now_seconds();
         HIP_TRY(hipEventSynchronize(w.startup_fetched));
         fetch_wait += now_seconds() - f0;
-        for (int stage = 0; stage < 6; ++stage) {
-          uint64_t draws = 0;
-          int64_t attempted = 0;
-          for (int64_t x = 0; x < W; ++x) {
-            const size_t i = (size_t)stage * W + x;
-            states[i] = state;
-            attempted += record_attempted(records[i]) ? 1 : 0;
-            state = chain_site(records[i], state, draws);
-          }
-          res->startup_attempted[stage] = attempted;
-          res->startup_advances[stage] = (int64_t)draws;
-        }
-        res->startup_final_rng_state = state;
+        state = walk_startup(records, W, state, states, res);
         HIP_TRY(hipMemcpyAsync(J.su_site_state, states, su * 4, hipMemcpyHostToDevice, stream_));
         HIP_TRY(launch_writer_startup_check(J, state, nullptr, stream_));
         uint64_t advances = 0;

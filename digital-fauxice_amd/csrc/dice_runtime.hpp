@@ -45,6 +45,7 @@ struct JobRequest {
   Callbacks cb;
   dice_replay_result* result = nullptr;
   dice_stage_timings* timings = nullptr;
+  bool streamed = false;  // main rows arrive later, block by block (FrameStream)
   // outcome
   int32_t code = DICE_OK;
   std::string error;
@@ -100,6 +101,7 @@ class Runner {
   uint64_t workspace_bytes() const;
 
  private:
+  friend class FrameStream;
   struct Graph {
     std::string key;
     hipGraphExec_t exec = nullptr;

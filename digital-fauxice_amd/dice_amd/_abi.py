@@ -293,6 +293,20 @@ class DiceFinalRegion(C.Structure):
 
 DICE_ROUTING_MASK_ON_DEVICE = 0x1
 
+
+class DiceStreamProgress(C.Structure):
+    _fields_ = [
+        ("rows_received", C.c_int64),
+        ("rows_analyzed", C.c_int64),
+        ("rows_final", C.c_int64),
+        ("sites_written", C.c_int64),
+        ("windows", C.c_int64),
+        ("push_seconds", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
 VOIDP = C.c_void_p
 
 # Signatures of the HIP library entry points (symbol prefix "dice_").
@@ -362,6 +376,12 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
                                    C.POINTER(DiceFinalRegion), i32p]),
     "routing_planes": (C.c_int32, [VOIDP, i32p, u8p, i32p]),
     "routing_free": (None, [VOIDP]),
+    "stream_create": (C.c_int32, [VOIDP, C.POINTER(VOIDP)]),
+    "stream_begin": (C.c_int32, [VOIDP, C.POINTER(DiceJob), C.POINTER(DiceDiagnostics)]),
+    "stream_push": (C.c_int32, [VOIDP, u16p, i64, i64]),
+    "stream_finish": (C.c_int32, [VOIDP]),
+    "stream_progress_get": (C.c_int32, [VOIDP, C.POINTER(DiceStreamProgress)]),
+    "stream_destroy": (None, [VOIDP]),
     "t_writer_site_host": (C.c_int32, [f64p, f32p, C.c_uint8, C.c_float, C.c_float, f32p, u32p,
                                        i64, C.c_int32, u32p, u8p]),
 }

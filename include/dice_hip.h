@@ -468,6 +468,50 @@ int32_t dice_routing_planes(const dice_routing* routing, int32_t* provisional_la
                             uint8_t* synthesis_mask, int32_t* final_labels);
 void dice_routing_free(dice_routing* routing);
 
+/* ---- Frame streams (8-row scanner blocks) ---------------------------------
+ * The reference replays a frame only once all of it is in memory
+ * (streaming.py:365-552 takes main_rgbi whole); its transport hands the main
+ * pass over as valid-row-only blocks with an absolute start row
+ * (contracts.py:108-158 RGBI16Block / BlockSchedule, 8 rows per block, the
+ * last one shorter) that assemble_rgbi16_blocks (contracts.py:224-261)
+ * joins when they are ordered, gap-free, non-overlapping and of one width.
+ * A stream takes those blocks as they arrive and runs every device phase
+ * whose inputs are complete: producer blocks, row sums and the cumulative
+ * chains, stage entries, analysis tile rows, sites, the writer draw
+ * schedules, the serial writer chain over the new sites, their values, and
+ * the copy and SHA-256 of the output rows that are final.  The result is
+ * the one dice_process_batch gives for the whole frame (bit-exact output,
+ * hash, counters and receipts).
+ *
+ * dice_stream_begin takes a dice_job whose main_rgbi is null (the frame
+ * arrives through dice_stream_push) and whose prepass frame is complete;
+ * DICE_MAIN_ON_DEVICE is rejected.  height/width are the block schedule's.
+ * Pushes must continue at the first row not yet received; a push that
+ * completes the frame runs the rest of it.  dice_stream_finish waits for
+ * nothing more: it fills job->result, status and error (and the optional
+ * diagnostics) and returns the job's status.  After an error every later
+ * call returns it (a refused block is not an error of the frame).  A stream
+ * is reusable: begin may follow finish, or abandon a frame in progress.  One
+ * caller thread at a time per stream. */
+typedef struct dice_stream dice_stream;
+
+typedef struct dice_stream_progress {
+  int64_t rows_received;   /* main rows pushed so far */
+  int64_t rows_analyzed;   /* rows whose analysis tiles and sites are done */
+  int64_t rows_final;      /* output rows written, copied and hashed */
+  int64_t sites_written;   /* main sites through the writer */
+  int64_t windows;         /* device windows run */
+  double push_seconds;     /* wall time inside the last push */
+} dice_stream_progress;
+
+int32_t dice_stream_create(dice_ctx* ctx, dice_stream** out);
+int32_t dice_stream_begin(dice_stream* stream, dice_job* job, dice_diagnostics* diagnostics);
+int32_t dice_stream_push(dice_stream* stream, const uint16_t* rgbi, int64_t start_row,
+                         int64_t rows);
+int32_t dice_stream_finish(dice_stream* stream);
+int32_t dice_stream_progress_get(const dice_stream* stream, dice_stream_progress* progress);
+void dice_stream_destroy(dice_stream* stream);
+
 /* Calibration: `repeats` launches of a plain 16-byte-vector copy kernel over
  * `bytes` (device buffers owned by the call); average milliseconds per copy. */
 int32_t dice_t_copy(dice_ctx* ctx, uint64_t bytes, int32_t repeats,
