@@ -788,22 +788,40 @@ struct SitesShared {
   static constexpr int AR = TY + 10, AC = TX + 12;  // aux, edge-clamped, +-5/+-6
   static constexpr int HR = TY + 10, HC = TX + 10;  // score / waux history, +-5
   static constexpr int RR = TY + 8, RC = TX + 8;    // weighted RGB history, +-4
+  // feature-record positions: the tile and a one-pixel ring (the cross
+  // neighbours of its sites)
+  static constexpr int PR = TY + 2, PC = TX + 2, NP = PR * PC;
+  static constexpr int kPerThread = (NP + kThreads - 1) / kThreads;
   float aux[AR][AC];
   float score[HR][HC];
   float waux[HR][HC];
   float wrgb[3][RR][RC];
+  float record_w[3][NP];      // f32 of the unrounded score averages
+  float record_ratio[3][NP];  // f32(waux average / unrounded score average)
+  uint8_t record_valid[NP];   // bit l: score average l > 0 (else the fallback)
+  uint32_t needed_bits[(NP + 31) / 32];
+  uint16_t needed_list[NP];
   RowParams rows[AR];
   float writer_coarse[TY];
   uint8_t writer_floor[TY];
   uint32_t row_start[TY];
   uint32_t row_offset[TY];
+  uint32_t wave_sums[kThreads / 64];
 };
 
+// Reconstruction sites of one tile.  A feature record (streaming.py:236-308)
+// is a function of its position alone, up to the fallback that replaces an
+// empty average (the centre row's writer coarse reference); so every
+// position a site of the tile needs -- the site itself and, in cross-
+// neighbour mode, its four neighbours -- gets its averages once, in a table
+// of raw ratios and validity bits, and each site applies its own fallback
+// when it reads them.  Adjacent sites share most of their records.
 template <int TY, int TX>
 __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
   using Sh = SitesShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
-  constexpr int RR = Sh::RR, RC = Sh::RC;
+  constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
+  constexpr int kWords = (NP + 31) / 32;
   __shared__ Sh S;
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
@@ -815,8 +833,10 @@ __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
     const int y0 = tile_y * TY, x0 = tile_x * TX;
     const int tid = threadIdx.x;
     const float floor = P.score_floor;
+    const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
 
     for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
+    for (int i = tid; i < kWords; i += kThreads) S.needed_bits[i] = 0u;
     if (tid == 0) {
       uint32_t start = 0;
       for (int i = 0; i < TY; ++i) {
@@ -830,6 +850,24 @@ __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
       }
     }
     __syncthreads();
+
+    // The record positions the sites need (record index = (ty+1)*PC + tx+1).
+    // Neighbours outside the frame's columns are all-zero records, not
+    // averages (streaming.py:259-308).
+    for (uint32_t q = tid; q < total; q += kThreads) {
+      const int li = list[q];
+      const int ty = li / TX, tx = li % TX;
+      const int x = x0 + tx;
+      const int centre = (ty + 1) * PC + tx + 1;
+      auto mark = [&](int i) { atomicOr(&S.needed_bits[i >> 5], 1u << (i & 31)); };
+      mark(centre);
+      if (P.cross_neighbor) {
+        if (x > 0) mark(centre - 1);
+        if (x < W - 1) mark(centre + 1);
+        mark(centre - PC);
+        mark(centre + PC);
+      }
+    }
 
     // Working lanes and aux on the clamped halo; working RGB staged for the
     // weighted-RGB history.
@@ -893,46 +931,92 @@ __global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
         S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
       }
     }
+
+    __syncthreads();
+
+    // Compact the needed positions (each thread owns a run of them).
+    uint32_t needed = 0;
+    {
+      const int first = tid * Sh::kPerThread;
+      uint32_t count = 0;
+#pragma unroll
+      for (int k = 0; k < Sh::kPerThread; ++k) {
+        const int i = first + k;
+        if (i < NP && ((S.needed_bits[i >> 5] >> (i & 31)) & 1u)) ++count;
+      }
+      uint32_t at = block_exclusive_scan(count, S.wave_sums, &needed);
+#pragma unroll
+      for (int k = 0; k < Sh::kPerThread; ++k) {
+        const int i = first + k;
+        if (i < NP && ((S.needed_bits[i >> 5] >> (i & 31)) & 1u)) S.needed_list[at++] = (uint16_t)i;
+      }
+    }
+    __syncthreads();
+
+    // The averages of every needed position (streaming.py:236-256; the A
+    // lanes divide by the unrounded binary64 W, reconstruction.py:365-371).
+    // Position (pr, pc) has its patch at history rows pr..pr+8, columns
+    // pc..pc+8.
+    for (uint32_t k = tid; k < needed; k += kThreads) {
+      const int i = S.needed_list[k];
+      const int pr = i / PC, pc = i % PC;
+      double unrounded[3], numerator[3];
+      unscaled_averages([&](int dy, int dx) { return S.score[pr + dy][pc + dx]; }, unrounded);
+      unscaled_averages([&](int dy, int dx) { return S.waux[pr + dy][pc + dx]; }, numerator);
+      uint8_t valid = 0;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        S.record_w[l][i] = (float)unrounded[l];
+        const bool positive = unrounded[l] > 0.0;
+        S.record_ratio[l][i] = positive ? (float)(numerator[l] / unrounded[l]) : 0.0f;
+        valid |= positive ? (uint8_t)(1u << l) : (uint8_t)0;
+      }
+      S.record_valid[i] = valid;
+    }
     __syncthreads();
 
     // One site per thread.  Eligibility already implies the driver gate
     // passes (reconstruction.py:890-904 re-tests the same row gate and the
     // same point score), so every listed centre is attempted.
-    const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
     for (uint32_t q = tid; q < total; q += kThreads) {
       const int li = list[q];
       const int ty = li / TX, tx = li % TX;
       const int y = y0 + ty, x = x0 + tx;
       const int cr = ty + 5, cc = tx + 5;  // history index of the centre
       const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
+      const int centre = (ty + 1) * PC + tx + 1;
       const float fallback = S.writer_coarse[ty];
-      auto record = [&](int r, int c, float point_aux, float point_score, float w[4],
-                        float f[4]) {
-        auto score_at = [&](int dy, int dx) { return S.score[r - 4 + dy][c - 4 + dx]; };
-        auto waux_at = [&](int dy, int dx) { return S.waux[r - 4 + dy][c - 4 + dx]; };
-        feature_record(score_at, waux_at, point_aux, point_score, fallback, w, f);
+      // record at table index i with point values (aux, score) of its pixel
+      auto feature = [&](int i, float point_aux, float point_score, float f[4]) {
+        const uint8_t valid = S.record_valid[i];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) f[l] = (valid >> l) & 1u ? S.record_ratio[l][i] : fallback;
+        f[3] = point_score > 0.0f ? point_aux : fallback;
       };
       float weights[4], features[5][4];
-      record(cr, cc, S.aux[ar][ac], S.score[cr][cc], weights, features[0]);
+      weights[0] = S.record_w[0][centre];
+      weights[1] = S.record_w[1][centre];
+      weights[2] = S.record_w[2][centre];
+      weights[3] = S.score[cr][cc];
+      feature(centre, S.aux[ar][ac], S.score[cr][cc], features[0]);
       int record_count = 1;
       if (P.cross_neighbor) {
         // streaming.py:259-308: neighbours share the centre row's fallback;
         // out-of-row neighbours are all-zero records; rows -1 and H take the
         // pseudo-row / repeated-row point values already in the halo.
         record_count = 5;
-        float unused[4];
         if (x > 0) {
-          record(cr, cc - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], unused, features[1]);
+          feature(centre - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], features[1]);
         } else {
           features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
         }
         if (x < W - 1) {
-          record(cr, cc + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], unused, features[2]);
+          feature(centre + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], features[2]);
         } else {
           features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
         }
-        record(cr - 1, cc, S.aux[ar - 1][ac], S.score[cr - 1][cc], unused, features[3]);
-        record(cr + 1, cc, S.aux[ar + 1][ac], S.score[cr + 1][cc], unused, features[4]);
+        feature(centre - PC, S.aux[ar - 1][ac], S.score[cr - 1][cc], features[3]);
+        feature(centre + PC, S.aux[ar + 1][ac], S.score[cr + 1][cc], features[4]);
       }
       auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
       float candidates[3][3];
