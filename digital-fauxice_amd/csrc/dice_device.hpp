@@ -360,6 +360,10 @@ struct JobSet {
   const JobDev* jobs;  // device array of `count` descriptors
   int32_t count;
   uint32_t base[kMaxBatch + 1];
+  // {next ticket, finished workgroups} of a dynamically scheduled launch:
+  // zero before the launch, and zero again after it (the last workgroup
+  // resets them); one pair per stream that launches such kernels
+  uint32_t* queue;
 };
 
 }  // namespace dice

@@ -122,6 +122,37 @@ __device__ __forceinline__ void for_each_unit(const JobSet& set, Body&& body) {
   }
 }
 
+// Persistent loop that deals the units out one at a time from a ticket
+// counter, for kernels whose units differ widely in cost (a tile of the
+// reconstruction kernel holds anywhere from 0 to TY*TX sites): a workgroup
+// takes the next unit as soon as it is free, so the launch ends about one
+// unit after the average workgroup instead of with the unluckiest one.
+// Every workgroup leaves once the tickets pass the unit count; the last one
+// to leave resets the counter pair for the next launch on the stream.
+template <class Body>
+__device__ __forceinline__ void for_each_unit_dynamic(const JobSet& set, Body&& body) {
+  __shared__ uint32_t ticket;
+  const uint32_t total = set.base[set.count];
+  uint32_t* queue = set.queue;
+  for (;;) {
+    if (threadIdx.x == 0) ticket = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const uint32_t u = ticket;
+    if (u >= total) break;
+    const int j = job_of(set, u);
+    const JobDev& J = job_at(set, j);
+    if (!J.skip) body(J, (int)(u - set.base[j]));
+    __syncthreads();  // every thread has read the ticket and left the unit's LDS
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
 // streaming.py:157-198: four nine-sample bars, edge-clamped by `aux_at`.
 template <class AuxAt>
 __device__ __forceinline__ bool decision_fallback(const AuxAt& aux_at,
@@ -816,14 +847,23 @@ struct SitesShared {
 // neighbour mode, its four neighbours -- gets its averages once, in a table
 // of raw ratios and validity bits, and each site applies its own fallback
 // when it reads them.  Adjacent sites share most of their records.
+// Workgroups per CU the LDS of a tile shape allows (160 KB per CU), as
+// the occupancy target of its register allocation.
 template <int TY, int TX>
-__global__ __launch_bounds__(kThreads, 3) void k_sites(JobSet set) {
+constexpr int sites_occupancy() {
+  return sizeof(SitesShared<TY, TX>) * 3 <= 160 * 1024   ? 3
+         : sizeof(SitesShared<TY, TX>) * 2 <= 160 * 1024 ? 2
+                                                         : 1;
+}
+
+template <int TY, int TX>
+__global__ __launch_bounds__(kThreads, (sites_occupancy<TY, TX>())) void k_sites(JobSet set) {
   using Sh = SitesShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
   constexpr int kWords = (NP + 31) / 32;
   __shared__ Sh S;
-  for_each_unit(set, [&](const JobDev& J, const int unit) {
+  for_each_unit_dynamic(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
     const int t = J.win.trow_lo * J.tiles_x + unit;
     const uint32_t total = J.tile_count[t];
@@ -1656,6 +1696,7 @@ JobSet make_set(const JobDev* d_jobs, const JobDev* h_jobs, int n, Units&& units
     set.base[j + 1] = set.base[j] + u;
   }
   for (int j = n + 1; j <= kMaxBatch; ++j) set.base[j] = set.base[n];
+  set.queue = nullptr;
   return set;
 }
 
@@ -1786,13 +1827,15 @@ hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int 
   return hipGetLastError();
 }
 
-hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream) {
+hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint32_t* queue,
+                        hipStream_t stream) {
   int ty = 0, tx = 0;
   bool any = false;
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
   if (!any) return hipSuccess;
-  const JobSet set = make_set(d_jobs, h_jobs, n,
-                              [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
+  JobSet set =
+      make_set(d_jobs, h_jobs, n, [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
+  set.queue = queue;
 #define DICE_MATCH(a, b)                  \
   if (ty == a && tx == b) {               \
     launch_sites_shape<a, b>(set, stream); \
@@ -1821,6 +1864,7 @@ hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream) {
   set.jobs = d_jobs;
   set.count = n;
   for (int j = 0; j <= kMaxBatch; ++j) set.base[j] = 0;
+  set.queue = nullptr;
   const unsigned grid = resident_blocks(reinterpret_cast<const void*>(k_writer_prep), kThreads);
   hipLaunchKernelGGL(k_writer_prep, dim3(grid), dim3(kThreads), 0, stream, set);
   return hipGetLastError();

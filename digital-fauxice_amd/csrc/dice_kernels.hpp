@@ -39,7 +39,10 @@ int tile_shapes(int (*out)[2], int max);
 hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream);
 hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                 hipStream_t stream);
-hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStream_t stream);
+// `queue`: two zeroed uint32 the launch uses as its ticket counter (and
+// leaves zeroed); one pair per stream.
+hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint32_t* queue,
+                        hipStream_t stream);
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                           hipStream_t stream);
 hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream);

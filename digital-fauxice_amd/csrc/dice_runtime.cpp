@@ -127,6 +127,8 @@ int32_t Runner::init() {
   for (hipEvent_t& e : fine_) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventCreateWithFlags(&sync_event_, hipEventDisableTiming | hipEventBlockingSync));
   HIP_TRY(d_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
+  HIP_TRY(queue_.ensure(256));
+  HIP_TRY(hipMemset(queue_.ptr, 0, 256));
   HIP_TRY(h_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
   return DICE_OK;
 }
@@ -320,7 +322,7 @@ int32_t Runner::launch_b(int n) {
   const JobDev* d = d_jobs_.as<JobDev>();
   const JobDev* h = h_jobs_.as<JobDev>();
   DICE_TRY(mark(4));
-  HIP_TRY(launch_sites(d, h, n, stream_));
+  HIP_TRY(launch_sites(d, h, n, queue_.as<uint32_t>(), stream_));
   DICE_TRY(mark(5));
   HIP_TRY(launch_startup(d, h, n, stream_));
   DICE_TRY(mark(6));

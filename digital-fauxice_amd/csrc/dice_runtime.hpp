@@ -131,6 +131,7 @@ class Runner {
   int tile_rows_ = 16, tile_cols_ = 64;
   std::vector<std::unique_ptr<Workspace>> ws_;
   DeviceBuffer d_jobs_;
+  DeviceBuffer queue_;  // ticket counter of the dynamically scheduled launches (zero at rest)
   PinnedBuffer h_jobs_;
   Graph graph_a_, graph_b_;
   std::map<std::string, std::pair<int, int>> tuned_;
