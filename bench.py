@@ -67,16 +67,18 @@ FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_PIXEL = 14  # 8 B RGBI uint16 in + 6 B RGB uint16 out
 SITE_READ_BYTES_PER_PIXEL = 8  # k_sites: the RGBI of every pixel of its tiles
 SITE_RECORD_BYTES = 40  # k_sites: one Site record per attempted site
-# binary64 operations per attempted site in k_sites: 13 unscaled 9x9
-# averages (9 column scratches of 12 operations + 17 to combine them), the
-# RGB Q16 rows (13 each), 15 record divisions, 9 candidate scalings and the
-# combiner (about 50)
+# binary64 operations per attempted site of the reference algorithm: 13
+# unscaled 9x9 averages (9 column scratches of 12 operations + 17 to combine
+# them), the RGB Q16 rows (13 each), 15 record divisions, 9 candidate
+# scalings and the combiner (about 50).  k_sites shares feature records
+# between neighbouring sites and executes about half of these; the rate is
+# quoted on the algorithmic count.
 SITE_FP64_OPS = 13 * (9 * 12 + 17) + 3 * 13 + 15 + 9 + 50
 # HBM bytes one k_sites<16, 64> launch moves on this workload: rocprofv3
 # FETCH_SIZE and WRITE_SIZE (two --pmc runs over the plain leg, one frame per
 # dispatch), each scaled by what it reads for the plain copy kernel k_copy
 # over 1 GiB (x2.000 and x1.000)
-SITES_TRAFFIC_BYTES = 339_300_000
+SITES_TRAFFIC_BYTES = 388_900_000
 
 
 def parse_args() -> argparse.Namespace:
