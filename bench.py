@@ -287,6 +287,7 @@ def main() -> int:
             served.wait(t)
         synchronize()
         served.reset_stats()
+        cpu_start = os.times()
         start = time.perf_counter()
         tickets = [served.submit_device(full.d_main[k % args.batch].address, height, width,
                                         full.prepasses[k % args.batch],
@@ -298,7 +299,11 @@ def main() -> int:
             pipe_prints.append((r.output_sha256, r.attempted_pixels, r.written_pixels,
                                 r.public_rng_advances, r.final_rng_state, r.changed_pixels))
         pipe_elapsed = time.perf_counter() - start
+        cpu_end = os.times()
         pipe_stats = served.stats()
+    # host cores the pipelined leg kept busy (user + system time of every
+    # thread of this process over the leg's wall time)
+    pipe_cores = ((cpu_end.user - cpu_start.user) + (cpu_end.system - cpu_start.system)) / pipe_elapsed
     del pipe_outs
     value = jobs_total * frame_mpix / pipe_elapsed
     log(f"pipelined batches: {jobs_total} frames in {pipe_elapsed:.3f} s "
@@ -627,6 +632,8 @@ def main() -> int:
             "mean_batch": round(pipe_stats["mean_batch"], 2),
             "job_latency_p50_ms": round(pipe_stats["latency_p50_ms"], 3),
             "job_latency_p99_ms": round(pipe_stats["latency_p99_ms"], 3),
+            "host_cores_busy": round(pipe_cores, 2),
+            "host_cores": len(os.sched_getaffinity(0)),
         },
         "batched_single_call": batched_single_call,
         "plain": {
