@@ -202,3 +202,91 @@ def test_block_rules_in_python(stream, synthetic):
     stream.finish()
     with pytest.raises(ValueError, match="no frame"):
         stream.finish()
+
+
+def c_stream_job(main, prepass, out_address, flags, result):
+    """A dice_job for the C stream entry points (main rows come by push)."""
+
+    from dice_amd._abi import DiceJob
+
+    job = DiceJob()
+    job.main_rgbi = None
+    job.height, job.width = main.shape[:2]
+    job.prepass_rgbi = prepass.ctypes.data
+    job.prepass_height, job.prepass_width = prepass.shape[:2]
+    job.params = C.pointer(stable_profile_params())
+    job.prepass_params = C.pointer(DEFAULT_PROFILE.prepass_parameters())
+    job.flags = flags
+    job.out_rgb16 = out_address
+    job.result = C.pointer(result)
+    return job
+
+
+def push_all(lib, handle, main, rows_per_block=8):
+    for row in range(0, main.shape[0], rows_per_block):
+        block = np.ascontiguousarray(main[row:row + rows_per_block])
+        code = lib.dice_stream_push(handle, block.ctypes.data_as(C.POINTER(C.c_uint16)), row,
+                                    block.shape[0])
+        assert code == 0, lib.dice_last_error()
+
+
+def test_stream_flags_device_output_and_no_hash(synthetic):
+    """DICE_OUT_ON_DEVICE writes the output into the caller's device buffer;
+    DICE_SKIP_SHA256 leaves the hash empty and everything else unchanged."""
+    from dice_amd._abi import DiceReplayResult
+    from dice_amd.hip_backend.device import DeviceArray
+    from dice_amd.hip_backend.engine import context
+
+    main, prepass, cpu = synthetic[2]
+    prepass = np.ascontiguousarray(prepass)
+    lib = library()
+    handle = C.c_void_p()
+    assert lib.dice_stream_create(context(), C.byref(handle)) == 0
+    try:
+        for flags in (_abi.DICE_OUT_ON_DEVICE, _abi.DICE_OUT_ON_DEVICE | _abi.DICE_SKIP_SHA256,
+                      _abi.DICE_SKIP_SHA256):
+            device_out = DeviceArray(cpu.output.nbytes)
+            host_out = np.zeros_like(cpu.output)
+            on_device = bool(flags & _abi.DICE_OUT_ON_DEVICE)
+            result = DiceReplayResult()
+            job = c_stream_job(main, prepass,
+                               device_out.address if on_device else host_out.ctypes.data,
+                               flags, result)
+            assert lib.dice_stream_begin(handle, C.byref(job), None) == 0, lib.dice_last_error()
+            push_all(lib, handle, main)
+            assert lib.dice_stream_finish(handle) == 0, lib.dice_last_error()
+            assert job.status == 0
+            got = device_out.download(np.empty_like(cpu.output)) if on_device else host_out
+            assert np.array_equal(got, cpu.output), flags
+            assert (result.attempted_pixels, result.written_pixels, result.final_rng_state,
+                    result.changed_pixels) == (cpu.attempted_pixels, cpu.written_pixels,
+                                               cpu.final_rng_state, cpu.changed_pixels)
+            want = b"" if flags & _abi.DICE_SKIP_SHA256 else cpu.output_sha256.encode()
+            assert result.output_sha256 == want
+    finally:
+        lib.dice_stream_destroy(handle)
+
+
+def test_streams_on_two_threads(synthetic):
+    """Two streams, each on its own thread (each has its own runner, HIP
+    streams and workspace), give the results each gives alone."""
+    import threading
+
+    outcomes = {}
+
+    def run(k):
+        main, prepass, cpu = synthetic[k]
+        with HipFrameStream() as s:
+            results = [run_stream(s, main, prepass, check_lag=False) for _ in range(2)]
+        outcomes[k] = (cpu, results)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in (0, 3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert sorted(outcomes) == [0, 3]
+    for cpu, results in outcomes.values():
+        for result in results:
+            assert result.replay.output_sha256 == cpu.output_sha256
+            assert np.array_equal(result.output_rgb16, cpu.output)
