@@ -28,6 +28,9 @@ Legs:
            transport) through a frame stream: the device phases and the
            serial writer run as the blocks arrive; the latency from the
            last block to the result, against the plain whole-frame latency.
+  large    one 8818x13228 large-format frame (1500 disks up to r 30 px, 120
+           scratches), one job per call: p50 latency and Mpix/s, checked
+           against the oracle in the CPU leg.
   serving  Poisson arrivals (70 % 1024-2048 px crops, 30 % full frames) at
            80 % of the batched capacity into the serving engine (continuous
            batching, HBM-budget admission, two drivers): p50/p99 latency and
@@ -100,6 +103,8 @@ def parse_args() -> argparse.Namespace:
                         help="skip the at-floor routing leg")
     parser.add_argument("--no-stream", action="store_true",
                         help="skip the 8-row block streaming leg")
+    parser.add_argument("--no-large", action="store_true",
+                        help="skip the 8818x13228 large-format leg")
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -519,6 +524,39 @@ def main() -> int:
         }
         log(f"stream: {streamed}")
 
+    # ---- one large-format frame per call (device-resident, like plain)
+    large = None
+    if not args.no_large:
+        big = Frames([synth.config4_large()])
+        bh, bw = big.specs[0].height, big.specs[0].width
+
+        def large_step() -> DiceReplayResult:
+            res = DiceReplayResult()
+            code = lib.dice_process_job(ctx, C.c_void_p(big.d_main[0].address), bh, bw,
+                                        ptr(big.prepasses[0], C.c_uint16),
+                                        big.prepasses[0].shape[0], big.prepasses[0].shape[1],
+                                        C.byref(params), C.byref(prepass_params), device_flags,
+                                        C.c_void_p(big.d_out[0].address), C.byref(res), None,
+                                        None, None)
+            engine.raise_for(code)
+            return res
+
+        large_step()
+        large_lat, large_prints = [], []
+        for _ in range(max(3, min(args.steps, 5))):
+            a = time.perf_counter()
+            large_prints.append(fingerprint(large_step()))
+            large_lat.append((time.perf_counter() - a) * 1e3)
+        large = {
+            "height": bh,
+            "width": bw,
+            "attempted_sites": int(large_prints[0][1]),
+            "p50_ms": round(float(np.percentile(large_lat, 50)), 3),
+            "mpix_per_s": round(big.mpix[0] / float(np.percentile(large_lat, 50)) * 1e3, 2),
+            "repeatable": all(p == large_prints[0] for p in large_prints),
+        }
+        log(f"large: {large}")
+
     # ---- CPU baseline (the oracle), which also checks every output
     cpu_baseline = None
     bit_exact = None
@@ -563,6 +601,15 @@ def main() -> int:
         if streamed is not None:
             streamed["bit_exact"] = stream_print == expected[0]
             bit_exact = bit_exact and streamed["bit_exact"]
+        if large is not None:
+            a = time.perf_counter()
+            cpu = oracle_lib.process_job(big.mains[0], big.prepasses[0], params, prepass_params,
+                                         threads=threads)
+            large["cpu_s"] = round(time.perf_counter() - a, 3)
+            large["bit_exact"] = large_prints[0] == (
+                cpu.output_sha256, cpu.attempted_pixels, cpu.written_pixels,
+                cpu.public_rng_advances, cpu.final_rng_state, cpu.changed_pixels)
+            bit_exact = bit_exact and large["bit_exact"]
         if routing is not None:
             # the routing census against the SciPy oracle of the same mask
             sys.path.insert(0, str(ROOT / "oracle"))
@@ -647,6 +694,7 @@ def main() -> int:
         "serving": serving,
         "routing": routing,
         "stream": streamed,
+        "large": large,
         "bit_exact": bit_exact,
         "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
         "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)
