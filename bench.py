@@ -321,13 +321,13 @@ def main() -> int:
         for _ in range(2):
             run_batch(crop_frames, batch_flags)
         synchronize()
-        lat = []
-        crop_prints = None
+        lat, crop_runs = [], []
         start = time.perf_counter()
         for _ in range(args.steps):
             a = time.perf_counter()
             crop_prints, crop_report = run_batch(crop_frames, batch_flags)
             lat.append((time.perf_counter() - a) * 1e3)
+            crop_runs.append(crop_prints)
         crop_elapsed = time.perf_counter() - start
         crops = {
             "jobs": len(crop_frames.specs),
@@ -601,6 +601,17 @@ def main() -> int:
         if streamed is not None:
             streamed["bit_exact"] = stream_print == expected[0]
             bit_exact = bit_exact and streamed["bit_exact"]
+        if crops is not None:
+            # every crop of every timed batch against the oracle on that crop
+            crop_expected = []
+            for main, prepass in zip(crop_frames.mains, crop_frames.prepasses):
+                cpu = oracle_lib.process_job(main, prepass, params, prepass_params,
+                                             threads=threads)
+                crop_expected.append((cpu.output_sha256, cpu.attempted_pixels,
+                                      cpu.written_pixels, cpu.public_rng_advances,
+                                      cpu.final_rng_state, cpu.changed_pixels))
+            crops["bit_exact"] = all(run == crop_expected for run in crop_runs)
+            bit_exact = bit_exact and crops["bit_exact"]
         if large is not None:
             a = time.perf_counter()
             cpu = oracle_lib.process_job(big.mains[0], big.prepasses[0], params, prepass_params,

@@ -11,6 +11,7 @@ validation inside a batch fails alone.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 
 import numpy as np
 import pytest
@@ -210,3 +211,49 @@ def test_batch_device_resident_buffers(frames, oracle_fingerprints):
         assert results[i].output_sha256.decode() == oracle_fingerprints[i]["output_sha256"]
         host = outs[i].download(np.empty(main.shape[:2] + (3,), np.uint16))
         assert hashlib.sha256(host.tobytes()).hexdigest() == oracle_fingerprints[i]["output_sha256"]
+
+
+def test_uneven_crops_workload_matches_oracle():
+    """The 16-crop workload (sides 512-3000 px, aspect 0.5-2, defects scaled
+    to area) as one batch with graph replay and autotuned tiles: every crop
+    equals the oracle, on the run that captures the graphs and on a replay."""
+    specs = synth.config3_crops()
+    frames = [(synth.frame(s), None) for s in specs]
+    frames = [(main, synth.prepass_for(main)) for main, _ in frames]
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    expected = []
+    for main, prepass in frames:
+        cpu = oracle_lib.process_job(main, prepass, params, prepass_params, threads=16)
+        expected.append({
+            "output_sha256": cpu.output_sha256,
+            "attempted_pixels": cpu.attempted_pixels,
+            "written_pixels": cpu.written_pixels,
+            "public_rng_advances": cpu.public_rng_advances,
+            "final_rng_state": cpu.final_rng_state,
+            "changed_pixels": cpu.changed_pixels,
+        })
+    jobs = [make_job(main, prepass, f"crop-{i}") for i, (main, prepass) in enumerate(frames)]
+    for run in range(3):  # direct, capture, replay
+        outcome = process_batch_hip(jobs, graph=True, autotune=True)
+        outcome.raise_first()
+        for i, result in enumerate(outcome.results):
+            assert _core(replay_fingerprint(result.replay)) == expected[i], (run, i)
+            assert hashlib.sha256(result.output_rgb16.tobytes()).hexdigest() == \
+                expected[i]["output_sha256"]
+    assert outcome.report["graph_replayed"] == 1
+
+
+def test_seed0_crop_matches_oracle():
+    """The 256 x 256 seed-0 crop (12 disks r 2-12, 3 scratches 50-200 px)."""
+    main = synth.frame(synth.config1_crop())
+    prepass = synth.prepass_for(main)
+    cpu = oracle_lib.process_job(main, prepass, stable_profile_params(),
+                                 DEFAULT_PROFILE.prepass_parameters(), diagnostics=True)
+    result = process_hip(make_job(main, prepass, "seed0"), export_diagnostics=True)
+    assert result.replay.output_sha256 == cpu.output_sha256
+    assert np.array_equal(result.output_rgb16, cpu.output)
+    assert result.replay.attempted_pixels == cpu.attempted_pixels
+    assert np.array_equal(result.diagnostics.score_plane.view(np.uint32),
+                          cpu.score_plane.view(np.uint32))
+    assert np.array_equal(result.diagnostics.changed_mask, cpu.changed_mask)
