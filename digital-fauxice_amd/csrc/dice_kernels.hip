@@ -877,17 +877,28 @@ __global__ __launch_bounds__(kThreads, (sites_occupancy<TY, TX>())) void k_sites
 
     for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
     for (int i = tid; i < kWords; i += kThreads) S.needed_bits[i] = 0u;
-    if (tid == 0) {
-      uint32_t start = 0;
-      for (int i = 0; i < TY; ++i) {
+    // Per tile row (one lane each, in the first wave): its first entry in
+    // the site list (an exclusive scan of the row counts), its first place
+    // in the site array, and its writer stage.
+    static_assert(TY <= 64, "one lane per tile row");
+    if (tid < 64) {
+      const int i = tid;
+      uint32_t count = 0;
+      if (i < TY) {
         const size_t seg = (size_t)(y0 + i) * J.tiles_x + tile_x;
         const int y = min(y0 + i, H - 1);
-        S.row_start[i] = start;
+        count = J.seg_count[seg];
         S.row_offset[i] = J.seg_offset[seg];
-        start += J.seg_count[seg];
         S.writer_coarse[i] = J.stages.coarse[y + 6];
         S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
       }
+      uint32_t inclusive = count;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inclusive, d, 64);
+        if (i >= d) inclusive += t;
+      }
+      if (i < TY) S.row_start[i] = inclusive - count;
     }
     __syncthreads();
 
@@ -1026,6 +1037,11 @@ __global__ __launch_bounds__(kThreads, (sites_occupancy<TY, TX>())) void k_sites
       const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
       const int centre = (ty + 1) * PC + tx + 1;
       const float fallback = S.writer_coarse[ty];
+      // the site's own working RGB, loaded first so that its latency
+      // overlaps the averaging below
+      uint32_t lanes[4];
+      load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+      const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
       // record at table index i with point values (aux, score) of its pixel
       auto feature = [&](int i, float point_aux, float point_score, float f[4]) {
         const uint8_t valid = S.record_valid[i];
@@ -1061,9 +1077,6 @@ __global__ __launch_bounds__(kThreads, (sites_occupancy<TY, TX>())) void k_sites
       auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
       float candidates[3][3];
       rgb_candidates(wrgb_at, weights, candidates);
-      uint32_t lanes[4];
-      load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-      const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
       Site site;
       combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
                         features, record_count, fallback, P.combine, site.cand);
