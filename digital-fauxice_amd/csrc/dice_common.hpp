@@ -147,11 +147,14 @@ inline bool sha256_hex(const void* data, size_t bytes, char out[72]) {
 }
 
 // The response LUT, the inverse factor tables and the no-op emit table
-// (emit of LUT[i]), generated once and pinned by SHA-256.
+// (emit of LUT[i]), generated once and pinned by SHA-256.  The no-op emit
+// stays within a few units of the raw value, so it also travels as 4-bit
+// deltas, eight to a word (noop_delta, kNoopDeltaBias in dice_device.hpp).
 struct Tables {
   std::vector<float> lut;
   std::vector<uint32_t> high, low;
   std::vector<uint16_t> noop_emit;
+  std::vector<uint32_t> noop_delta;  // 8192 words; nibble r & 7 of word r >> 3
   int32_t status = -1;
 };
 const Tables& tables();

@@ -67,6 +67,10 @@ DICE_HD uint32_t chain_channel(uint32_t low, uint32_t high, uint32_t s, uint32_t
   return drawn ? (keep ? t : t2) : s;
 }
 
+// The no-op emit of raw value r is r + d - kNoopDeltaBias with the 4-bit d
+// stored as nibble r & 7 of word r >> 3 of the delta table.
+constexpr int kNoopDeltaBias = 6;
+
 constexpr uint32_t kSitePixelMask = 0x7FFFFFFFu;
 constexpr uint32_t kSiteFloorBit = 0x80000000u;
 
@@ -278,7 +282,8 @@ struct JobDev {
   const uint16_t* rgbi;         // H*W*4
   uint16_t* out;                // H*W*3 (noop emit; patched after the writer)
   const float* lut;             // 65536
-  const uint16_t* noop_emit;    // 65536: emit of LUT[i], the output of an unchanged value
+  const uint32_t* noop_delta;   // 8192: emit of LUT[i], the output of an unchanged value,
+                                // as 4-bit deltas from i (kNoopDeltaBias)
   const uint32_t* factor_high;  // 256
   const uint32_t* factor_low;   // 256
   StageDev stages;

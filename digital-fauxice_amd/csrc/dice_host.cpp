@@ -82,8 +82,16 @@ const Tables& tables() {
       return;
     }
     t.noop_emit.resize(65536);
-    for (int i = 0; i < 65536; ++i)
+    t.noop_delta.assign(8192, 0u);
+    for (int i = 0; i < 65536; ++i) {
       t.noop_emit[i] = emit_value(t.lut[i], t.high.data(), t.low.data());
+      const int delta = (int)t.noop_emit[i] - i + kNoopDeltaBias;
+      if (delta < 0 || delta > 15) {  // the nibble form would not reproduce it
+        t.status = DICE_INTERNAL;
+        return;
+      }
+      t.noop_delta[i >> 3] |= (uint32_t)delta << ((i & 7) * 4);
+    }
     t.status = DICE_OK;
   });
   return t;

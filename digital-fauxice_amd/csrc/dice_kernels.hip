@@ -709,6 +709,7 @@ __device__ __forceinline__ RowParams load_row_params(const JobDev& J, int sy) {
 template <int TY, int TX>
 struct ScanShared {
   static constexpr int AR = TY + 8, AC = TX + 8;  // aux, edge-clamped, +-4
+  uint32_t noop_delta[8192];  // the no-op emit table as 4-bit deltas (32 KB)
   float aux[AR][AC];
   RowParams rows[AR];
   uint8_t writer_floor[TY];
@@ -723,6 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
   constexpr int AR = Sh::AR, AC = Sh::AC;
   static_assert((TY * TX) % kThreads == 0, "tile must be a multiple of the block");
   __shared__ Sh S;
+  bool table_staged = false;  // uniform: every job of a device shares the tables
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
     const int t = J.win.trow_lo * J.tiles_x + unit;
@@ -732,6 +734,12 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
     const int tid = threadIdx.x;
     const float floor = P.score_floor;
 
+    if (!table_staged) {  // once per workgroup; the barrier below orders it
+      const uint4* table = reinterpret_cast<const uint4*>(J.noop_delta);
+      for (int i = tid; i < 8192 / 4; i += kThreads)
+        reinterpret_cast<uint4*>(S.noop_delta)[i] = table[i];
+      table_staged = true;
+    }
     for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
     for (int i = tid; i < TY; i += kThreads) {
       const int y = min(y0 + i, H - 1);
@@ -759,10 +767,14 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
       const int ty = r - 4, tx = c - 4;
       if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
         // emit(LUT[raw]) is a function of the raw sample alone
+        auto noop = [&](uint32_t raw) {
+          const uint32_t delta = (S.noop_delta[raw >> 3] >> ((raw & 7u) * 4u)) & 15u;
+          return (uint16_t)(raw + delta - (uint32_t)kNoopDeltaBias);
+        };
         uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
-        o[0] = J.noop_emit[lanes[0]];
-        o[1] = J.noop_emit[lanes[1]];
-        o[2] = J.noop_emit[lanes[2]];
+        o[0] = noop(lanes[0]);
+        o[1] = noop(lanes[1]);
+        o[2] = noop(lanes[2]);
       }
     }
     if (aux_error) atomicOr(J.error_flags, aux_error);

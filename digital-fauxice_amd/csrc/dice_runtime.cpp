@@ -82,15 +82,15 @@ WriterConsts writer_consts(const dice_params& p) {
 int32_t DeviceTables::upload() {
   DICE_TRY(require_tables());
   const Tables& t = tables();
-  if (noop.bytes >= 65536 * 2) return DICE_OK;
+  if (noop_delta.bytes >= 8192 * 4) return DICE_OK;
   HIP_TRY(lut.ensure(65536 * 4));
   HIP_TRY(high.ensure(256 * 4));
   HIP_TRY(low.ensure(256 * 4));
-  HIP_TRY(noop.ensure(65536 * 2));
+  HIP_TRY(noop_delta.ensure(8192 * 4));
   HIP_TRY(hipMemcpy(lut.ptr, t.lut.data(), 65536 * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(high.ptr, t.high.data(), 256 * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(low.ptr, t.low.data(), 256 * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(noop.ptr, t.noop_emit.data(), 65536 * 2, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(noop_delta.ptr, t.noop_delta.data(), 8192 * 4, hipMemcpyHostToDevice));
   return DICE_OK;
 }
 
@@ -215,7 +215,7 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     const bool want_changed = R.diag && R.diag->changed_mask;
     if (want_changed) HIP_TRY(w.diag_changed.ensure(pixels, stream_));
     J.lut = tables_->lut.as<float>();
-    J.noop_emit = tables_->noop.as<uint16_t>();
+    J.noop_delta = tables_->noop_delta.as<uint32_t>();
     J.factor_high = tables_->high.as<uint32_t>();
     J.factor_low = tables_->low.as<uint32_t>();
     J.stages.alpha = w.st_alpha.as<float>();

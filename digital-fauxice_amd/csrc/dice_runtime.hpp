@@ -69,7 +69,7 @@ struct RunReport {
 
 // Device-resident tables shared by every job of a device.
 struct DeviceTables {
-  DeviceBuffer lut, high, low, noop;
+  DeviceBuffer lut, high, low, noop_delta;
   int32_t upload();
 };
 
