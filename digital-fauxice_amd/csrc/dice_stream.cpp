@@ -124,6 +124,9 @@ int32_t FrameStream::advance() {
     if (received < need_rows || staged < need_stages) break;
     ++trows;
   }
+  // Producer work alone waits for the next window that completes a tile row
+  // (or the frame): nothing downstream could use it sooner.
+  if (trows == trows_done_ && !all_rows) return DICE_OK;
   if (blocks == blocks_done_ && trows == trows_done_ && rowsums == rowsums_done_) return DICE_OK;
 
   JobDev::Window& win = J.win;

@@ -17,8 +17,10 @@
 //   writer chain, values    the new sites, in raster order, on the host chain
 //   output copy, SHA-256    rows [0, min(H, TY * tile rows done))
 //
-// Each window ends with the stream idle, so the caller's block buffer and
-// the descriptors are free again when push returns.
+// A window opens when a push completes a tile row (or the frame); producer
+// work alone waits for it, since nothing downstream could use it sooner.
+// Each push ends with the stream idle, so the caller's block buffer and the
+// descriptors are free again when push returns.
 #pragma once
 
 #include <memory>
