@@ -911,7 +911,8 @@ int32_t dice_stream_create(dice_ctx* ctx, dice_stream** out) {
 int32_t dice_stream_begin(dice_stream* stream, dice_job* job, dice_diagnostics* diagnostics) {
   if (!stream || !job) return set_error(DICE_INVALID, "null stream or job");
   HIP_TRY(hipSetDevice(stream->device));
-  stream->job = nullptr;  // until the frame has begun
+  stream->frames->abandon();  // a frame in progress ends here, whether or not this one begins
+  stream->job = nullptr;      // until the frame has begun
   stream->start = now_seconds();
   job->status = DICE_OK;
   job->error[0] = 0;

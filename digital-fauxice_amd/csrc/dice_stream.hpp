@@ -43,6 +43,8 @@ class FrameStream {
   int32_t push(const uint16_t* rgbi, int64_t start_row, int64_t rows);
   // Completes the result of a fully received frame; returns its status.
   int32_t finish();
+  // Drops a frame in progress (the stream is idle between calls).
+  void abandon() { active_ = false; }
   const JobRequest& request() const { return R_; }
   const dice_stream_progress& progress() const { return progress_; }
 

@@ -290,3 +290,23 @@ def test_streams_on_two_threads(synthetic):
         for result in results:
             assert result.replay.output_sha256 == cpu.output_sha256
             assert np.array_equal(result.output_rgb16, cpu.output)
+
+
+def test_refused_begin_abandons_the_frame_in_progress(stream, synthetic):
+    """A begin that fails validation still ends the frame before it: later
+    pushes are refused instead of feeding the old frame."""
+    from dice_amd._abi import DiceReplayResult
+
+    main, prepass, _ = synthetic[1]
+    stream.begin(prepass_frame(prepass), BlockSchedule.for_frame(*main.shape[:2]))
+    stream.push(split_rgbi16_frame(main_frame(main), 8)[0])
+    lib = library()
+    result = DiceReplayResult()
+    bad = c_stream_job(main, np.ascontiguousarray(prepass), 0, 0, result)
+    bad.height = 3  # below the 5-row minimum: refused before the device
+    assert lib.dice_stream_begin(stream._handle, C.byref(bad), None) == _abi.DICE_INVALID
+    assert bad.status == _abi.DICE_INVALID
+    block = np.ascontiguousarray(main[8:16])
+    assert lib.dice_stream_push(stream._handle, block.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                8, 8) == _abi.DICE_INVALID
+    stream._frame = None
