@@ -28,6 +28,9 @@ Legs:
            transport) through a frame stream: the device phases and the
            serial writer run as the blocks arrive; the latency from the
            last block to the result, against the plain whole-frame latency.
+  device_chain  frame 0 with its serial writer chain on one device
+           wavefront instead of a host core (the opt-in mode): ms per frame
+           and ns per site beside the host walk, checked against the oracle.
   large    one 8818x13228 large-format frame (1500 disks up to r 30 px, 120
            scratches), one job per call: p50 latency and Mpix/s, checked
            against the oracle in the CPU leg.
@@ -105,6 +108,8 @@ def parse_args() -> argparse.Namespace:
                         help="skip the 8-row block streaming leg")
     parser.add_argument("--no-large", action="store_true",
                         help="skip the 8818x13228 large-format leg")
+    parser.add_argument("--no-device-chain-leg", action="store_true",
+                        help="skip the single-wavefront device writer chain leg")
     parser.add_argument("--no-cpu-baseline", action="store_true",
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -524,6 +529,28 @@ def main() -> int:
         }
         log(f"stream: {streamed}")
 
+    # ---- frame 0 with the writer chain on one device wavefront
+    device_chain = None
+    if not args.no_device_chain_leg:
+        chain_prints, chain_ms = [], []
+        for _ in range(3):
+            array, results = batch_jobs(full)  # the first job of the array: frame 0
+            report = DiceBatchReport()
+            a = time.perf_counter()
+            code = lib.dice_process_batch(ctx, array, 1, _abi.DICE_BATCH_DEVICE_CHAIN,
+                                          C.byref(report))
+            chain_ms.append((time.perf_counter() - a) * 1e3)
+            engine.raise_for(code)
+            chain_prints.append(fingerprint(results[0]))
+        sites0 = int(plain_results[0][1])
+        device_chain = {
+            "ms_per_frame": round(min(chain_ms), 2),
+            "ns_per_site": round(min(chain_ms) * 1e6 / sites0, 1),
+            "host_walk_ms_plain": round(stage_ms["writer_host"], 2),
+            "identical_to_plain": all(p == plain_results[0] for p in chain_prints),
+        }
+        log(f"device chain: {device_chain}")
+
     # ---- one large-format frame per call (device-resident, like plain)
     large = None
     if not args.no_large:
@@ -612,6 +639,9 @@ def main() -> int:
                                       cpu.final_rng_state, cpu.changed_pixels))
             crops["bit_exact"] = all(run == crop_expected for run in crop_runs)
             bit_exact = bit_exact and crops["bit_exact"]
+        if device_chain is not None:
+            device_chain["bit_exact"] = chain_prints[0] == expected[0]
+            bit_exact = bit_exact and device_chain["bit_exact"]
         if large is not None:
             a = time.perf_counter()
             cpu = oracle_lib.process_job(big.mains[0], big.prepasses[0], params, prepass_params,
@@ -706,6 +736,7 @@ def main() -> int:
         "routing": routing,
         "stream": streamed,
         "large": large,
+        "device_chain": device_chain,
         "bit_exact": bit_exact,
         "stage_ms_plain": {k: round(v, 3) for k, v in stage_ms.items()},
         "writer_share_plain": round(stage_ms["writer_host"] / stage_ms["total"], 4)
