@@ -114,6 +114,8 @@ def parse_args() -> argparse.Namespace:
                         help="skip the CPU oracle leg (and its bit-exact check)")
     parser.add_argument("--cpu-seconds", type=float, default=20.0,
                         help="approximate CPU time budget of the baseline sample")
+    parser.add_argument("--no-one-thread", action="store_true",
+                        help="skip the one-thread oracle frame of the CPU leg")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                         help="only run the plain copy kernel over BYTES (counter calibration)")
     return parser.parse_args()
@@ -121,6 +123,18 @@ def parse_args() -> argparse.Namespace:
 
 def log(message: str) -> None:
     print(f"bench: {message}", file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str | None:
+    """The host CPU's model name, from /proc/cpuinfo (None where it is not there)."""
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8", errors="replace") as info:
+            for line in info:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -162,9 +176,9 @@ def fingerprint(res) -> tuple:
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(jobs shard across GPUs by job, without collectives; see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(jobs would shard across GPUs by job, one process per GPU, without "
+              "collectives)", file=sys.stderr)
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -663,6 +677,14 @@ def main() -> int:
             bit_exact = bit_exact and routing["bit_exact"]
         timed = runs[1:] if len(runs) > 1 else runs
         median = statistics.median(timed)
+        one_thread_s = None
+        if not args.no_one_thread:
+            a = time.perf_counter()
+            cpu = oracle_lib.process_job(full.mains[0], full.prepasses[0], params,
+                                         prepass_params, threads=1)
+            one_thread_s = time.perf_counter() - a
+            bit_exact = bit_exact and cpu.output_sha256 == expected[0][0]
+            log(f"cpu oracle frame 0 on one thread: {one_thread_s:.2f} s")
         cpu_baseline = {
             "value": round(frame_mpix / median, 4),
             "unit": "Mpix/s",
@@ -672,6 +694,12 @@ def main() -> int:
                        f"oracle, OpenMP row-parallel analysis + serial writer; median "
                        f"{median:.3f} s; it also checked {checked} frames of every timed batch "
                        f"and every plain job bit for bit"),
+            "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)),
+            "one_thread_s": None if one_thread_s is None else round(one_thread_s, 3),
+            "one_thread_mpix_per_s": (None if one_thread_s is None
+                                      else round(frame_mpix / one_thread_s, 4)),
         }
 
     plain_value = args.steps * frame_mpix / plain_elapsed
