@@ -212,6 +212,45 @@ DICE_HD uint32_t chain_site_drawing(const WriterRecord& r, uint32_t s, uint64_t&
   return c3 ? kept : redrawn;
 }
 
+// chain_site_drawing spread over the lanes of one wavefront, as the device
+// chain runs it.  Lane i < 6 tests channel spec_channel(i) on the state
+// spec_steps(i) LCG steps on (one 24-bit multiply-add from s): channel 1 on
+// t1, channel 2 on t2 / t3, channel 3 on t3 / t4 / t5; lane 6 carries t6.
+// The six keep bits fix the site's draws (spec_draws) and the lane holding
+// its end state (spec_end_lane).  Two exact shortcuts keep a lane at four
+// operations: the state is never masked (the 24-bit multiply reads only
+// its low 24 bits and the test masks the difference), and the masked
+// difference, with the draws flag set, is compared with the high word as
+// stored, len | kChanDraws (both channels draw, len <= 2^24 < kChanDraws).
+constexpr int kSpecLanes = 7;
+DICE_HD constexpr int spec_steps(int lane) { return lane < 3 ? lane + 1 : lane; }
+DICE_HD constexpr int spec_channel(int lane) { return lane == 0 ? 0 : (lane < 3 ? 1 : 2); }
+DICE_HD constexpr uint32_t spec_draws(uint32_t keep_bits) {
+  const uint32_t d0 = (keep_bits & 1u) ? 1u : 2u;
+  const uint32_t d1 = ((keep_bits >> d0) & 1u) ? 1u : 2u;             // lane 1 or 2
+  const uint32_t d2 = ((keep_bits >> (d0 + d1 + 1u)) & 1u) ? 1u : 2u;  // lane 3, 4 or 5
+  return d0 + d1 + d2;
+}
+DICE_HD constexpr int spec_end_lane(uint32_t draws) { return draws == 3u ? 2 : (int)draws; }
+DICE_HD bool spec_keep(uint32_t t, uint32_t low, uint32_t high) {
+  return (((t - low) & kLcgMask) | kChanDraws) < high;
+}
+
+// The lane form evaluated on the host, lane by lane (the low 32 bits of the
+// product of the low 24 bits, as the device multiply gives them).
+inline uint32_t chain_site_lanes(const WriterRecord& r, uint32_t s, uint64_t& draws) {
+  uint32_t t[kSpecLanes], keep_bits = 0;
+  for (int i = 0; i < kSpecLanes; ++i) {
+    const int n = spec_steps(i);
+    t[i] = (lcg_pow(n) & kLcgMask) * (s & kLcgMask) + lcg_add(n);
+    const uint64_t w = r.ch[spec_channel(i)];
+    if (i < 6 && spec_keep(t[i], (uint32_t)w, (uint32_t)(w >> 32))) keep_bits |= 1u << i;
+  }
+  const uint32_t d = spec_draws(keep_bits);
+  draws += d;
+  return t[spec_end_lane(d)] & kLcgMask;
+}
+
 // The chain over n consecutive records: each record's start state into
 // states[i]; returns the state after the last.
 inline uint32_t chain_records(const WriterRecord* r, size_t n, uint32_t s, uint32_t* states,

@@ -753,7 +753,14 @@ int32_t dice_t_writer_chain_host(const double* cand, const float* orig, const ui
   }
   uint64_t draws = 0;
   uint32_t state = state_in & kLcgMask;
-  if (generic) {
+  if (generic == 2) {  // the device chain's lane form where all channels draw
+    for (int64_t i = 0; i < count; ++i) {
+      states[i] = state;
+      const WriterRecord& r = records[(size_t)i];
+      state = draws_all_channels(r) ? chain_site_lanes(r, state, draws)
+                                    : chain_site(r, state, draws);
+    }
+  } else if (generic) {
     for (int64_t i = 0; i < count; ++i) {
       states[i] = state;
       state = chain_site(records[(size_t)i], state, draws);
@@ -785,11 +792,15 @@ int32_t dice_t_writer_site_host(const double* cand, const float* orig, uint8_t f
   }
   site.pixel = floor ? kSiteFloorBit : 0u;
   const WriterRecord record = writer_record(site, true, K);
-  const bool fast = !generic && draws_all_channels(record);
+  const bool drawing = draws_all_channels(record);
+  const bool fast = generic == 0 && drawing;
+  const bool lanes = generic == 2 && drawing;
   for (int64_t i = 0; i < count; ++i) {
     uint64_t draws = 0;
     const uint32_t s = states_in[i] & kLcgMask;
-    states_out[i] = fast ? chain_site_drawing(record, s, draws) : chain_site(record, s, draws);
+    states_out[i] = fast    ? chain_site_drawing(record, s, draws)
+                    : lanes ? chain_site_lanes(record, s, draws)
+                            : chain_site(record, s, draws);
     advances[i] = (uint8_t)draws;
   }
   return DICE_OK;

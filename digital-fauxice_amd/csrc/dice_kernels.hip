@@ -1446,68 +1446,64 @@ __global__ __launch_bounds__(kThreads) void k_writer_startup_check(JobDev J,
   if (state != expected) atomicOr(J.error_flags, kErrWriterChain);
 }
 
-// The writer's serial LCG chain on one wavefront (host_writer.py:73-93,
-// reconstruction.py:916-982 through chain_channel): records [begin, end)
-// of one job, in order, from the state carried in J.chain.  Every value the
-// chain touches is wave-uniform (read through readfirstlane), so the chain
-// runs on the scalar unit.  The 64 lanes stage the records through LDS (the
-// next chunk loads while the current one is chained), the next site's words
-// are read one site ahead, and the start state of every site is collected
-// one per lane and stored 64 at a time.  kStartup: the six hidden rows
-// (6*W slots, with the per-stage receipt); otherwise main sites, and the
-// state after the range goes to band_end[band].
+// The writer's serial LCG chain on one wavefront per job (host_writer.py:73-93,
+// reconstruction.py:916-982): records [begin, end) of the job, in order,
+// from the state carried in J.chain.  The 64 lanes stage the records
+// through LDS (the next chunk loads while the current one is chained) and
+// collect the start state of every site, one per lane, to store 64 at a
+// time.  The chain value itself is wave-uniform and lives in a scalar
+// register.
 constexpr int kWriterChunk = 512;
+constexpr int kChunkWords = kWriterChunk * 6;
+constexpr int kChunkPerLane = kChunkWords / 64;
 
 __device__ __forceinline__ uint32_t uniform32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-template <bool kStartup>
-__global__ __launch_bounds__(64) void k_writer_chain(JobDev J, uint32_t begin, uint32_t end,
-                                                     int band) {
-  __shared__ uint32_t words[2][kWriterChunk * 6];
+// v_writelane_b32: `value` (wave-uniform) into lane `lane` of `old` (the
+// compiler intrinsic; this compiler has no builtin spelling for it).
+extern "C" __device__ int dice_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
+
+// Records [first, first + min(kWriterChunk, end - first)) as words, kChunkPerLane per lane.
+__device__ __forceinline__ void fetch_chunk(const WriterRecord* records, uint32_t first,
+                                            uint32_t end, int lane, uint32_t next[kChunkPerLane]) {
+  const uint32_t* source = reinterpret_cast<const uint32_t*>(records + first);
+  const uint32_t n = min((uint32_t)kWriterChunk, end - first) * 6;
+#pragma unroll
+  for (int j = 0; j < kChunkPerLane; ++j) {
+    const uint32_t e = (uint32_t)(lane + j * 64);
+    next[j] = e < n ? source[e] : 0u;
+  }
+}
+
+// The six hidden startup rows of every job (6*W slots, launched with one
+// workgroup per job), per channel, with the per-stage receipt.
+__global__ __launch_bounds__(64) void k_writer_startup_chain(const JobDev* jobs) {
+  const JobDev& J = jobs[blockIdx.x];
+  if (J.skip) return;
+  __shared__ uint32_t words[2][kChunkWords];
   const int lane = threadIdx.x;
-  const WriterRecord* records = kStartup ? J.su_writer_records : J.writer_records;
-  uint32_t* states = kStartup ? J.su_site_state : J.site_state;
+  const WriterRecord* records = J.su_writer_records;
+  uint32_t* states = J.su_site_state;
   const uint32_t W = (uint32_t)J.width;
+  const uint32_t end = 6u * W;
   uint32_t state = uniform32(J.chain->state);
-  unsigned long long advances = 0;
   long long stage_attempted = 0, stage_draws = 0;
   uint32_t stage = 0, stage_end = W;
-  constexpr int kPer = kWriterChunk * 6 / 64;
-  uint32_t next[kPer];
-  auto fetch = [&](uint32_t first) {
-    const uint32_t* source = reinterpret_cast<const uint32_t*>(records + first);
-    const uint32_t n = min((uint32_t)kWriterChunk, end - first) * 6;
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const uint32_t e = (uint32_t)(lane + j * 64);
-      next[j] = e < n ? source[e] : 0u;
-    }
-  };
-  if (begin < end) fetch(begin);
+  uint32_t next[kChunkPerLane];
+  fetch_chunk(records, 0u, end, lane, next);
   int buffer = 0;
-  for (uint32_t first = begin; first < end; first += kWriterChunk) {
+  for (uint32_t first = 0; first < end; first += kWriterChunk) {
     const uint32_t n = min((uint32_t)kWriterChunk, end - first);
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) words[buffer][lane + j * 64] = next[j];
+    for (int j = 0; j < kChunkPerLane; ++j) words[buffer][lane + j * 64] = next[j];
     __syncthreads();
-    if (first + kWriterChunk < end) fetch(first + kWriterChunk);
+    if (first + kWriterChunk < end) fetch_chunk(records, first + kWriterChunk, end, lane, next);
     const uint32_t* w = words[buffer];
     uint32_t collected = 0;
-    uint32_t v0 = w[0], v1 = w[1], v2 = w[2], v3 = w[3], v4 = w[4], v5 = w[5];
     for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t l0 = uniform32(v0), h0 = uniform32(v1);
-      const uint32_t l1 = uniform32(v2), h1 = uniform32(v3);
-      const uint32_t l2 = uniform32(v4), h2 = uniform32(v5);
-      const uint32_t ahead = (k + 1 < n ? k + 1 : k) * 6;  // next site's words, in flight
-      v0 = w[ahead];
-      v1 = w[ahead + 1];
-      v2 = w[ahead + 2];
-      v3 = w[ahead + 3];
-      v4 = w[ahead + 4];
-      v5 = w[ahead + 5];
-      if (kStartup && first + k == stage_end) {  // next hidden stage
+      if (first + k == stage_end) {  // next hidden stage
         if (lane == 0) {
           J.chain->startup_attempted[stage] = stage_attempted;
           J.chain->startup_advances[stage] = stage_draws;
@@ -1517,17 +1513,15 @@ __global__ __launch_bounds__(64) void k_writer_chain(JobDev J, uint32_t begin, u
         ++stage;
         stage_end += W;
       }
+      const uint32_t* r = w + k * 6;
       collected = lane == (int)(k & 63u) ? state : collected;
+      const uint32_t l0 = uniform32(r[0]);
       uint32_t d0, d1, d2;
-      state = chain_channel(l0, h0, state, d0);
-      state = chain_channel(l1, h1, state, d1);
-      state = chain_channel(l2, h2, state, d2);
-      if (kStartup) {
-        stage_draws += (long long)(d0 + d1 + d2);
-        stage_attempted += (l0 >> 31) & 1u;
-      } else {
-        advances += d0 + d1 + d2;
-      }
+      state = chain_channel(l0, uniform32(r[1]), state, d0);
+      state = chain_channel(uniform32(r[2]), uniform32(r[3]), state, d1);
+      state = chain_channel(uniform32(r[4]), uniform32(r[5]), state, d2);
+      stage_draws += (long long)(d0 + d1 + d2);
+      stage_attempted += (l0 >> 31) & 1u;
       if ((k & 63u) == 63u || k + 1 == n) {
         const uint32_t base = first + (k & ~63u);
         if (lane <= (int)(k & 63u)) states[base + lane] = collected;
@@ -1537,15 +1531,105 @@ __global__ __launch_bounds__(64) void k_writer_chain(JobDev J, uint32_t begin, u
     buffer ^= 1;
   }
   if (lane == 0) {
-    if (kStartup) {
-      J.chain->startup_attempted[stage] = stage_attempted;
-      J.chain->startup_advances[stage] = stage_draws;
-      J.chain->startup_final = state;
-    } else {
-      J.chain->advances += advances;
-      J.band_end[band] = state;
-    }
+    J.chain->startup_attempted[stage] = stage_attempted;
+    J.chain->startup_advances[stage] = stage_draws;
+    J.chain->startup_final = state;
     J.chain->state = state;
+  }
+}
+
+// Band `band` of `bands` of every job's main sites (one workgroup per job;
+// a job with fewer rows than bands has one band per row).  Where all three
+// channels of a site draw, the lanes evaluate chain_site_drawing's six
+// tests at once (chain_site_lanes): a 24-bit multiply-add, a subtract, an
+// and-or and a compare per lane, then the ballot of the six keep bits reads
+// the site's draws and end lane from a 64-entry table held one entry per
+// lane, and that lane's state becomes the chain value.  Other sites take
+// the per-channel path on the scalar unit.  The chain value is kept
+// unmasked (only its low 24 bits are ever read) and masked when stored.
+__global__ __launch_bounds__(64) void k_writer_chain(const JobDev* jobs, int band, int bands) {
+  const JobDev& J = jobs[blockIdx.x];
+  if (J.skip) return;
+  const int64_t H = J.height;
+  const int job_bands = (int)max((int64_t)1, min((int64_t)bands, H));
+  if (band >= job_bands) return;
+  // (read through readfirstlane, so that the loops below are uniform and
+  // the chain value stays in a scalar register)
+  const uint32_t begin = uniform32(J.row_start[H * band / job_bands]);
+  const uint32_t end = uniform32(J.row_start[H * (band + 1) / job_bands]);
+  __shared__ __align__(16) uint32_t words[2][kChunkWords];
+  const int lane = threadIdx.x;
+  // this lane's speculation: steps, channel word, and its entry of the
+  // keep-bits table (entry m = end lane | draws << 8 for keep bits m)
+  const int spec = lane < kSpecLanes ? lane : kSpecLanes - 1;
+  uint32_t mul = 0, add = 0;
+  switch (spec_steps(spec)) {
+    case 1: mul = lcg_pow(1); add = lcg_add(1); break;
+    case 2: mul = lcg_pow(2); add = lcg_add(2); break;
+    case 3: mul = lcg_pow(3); add = lcg_add(3); break;
+    case 4: mul = lcg_pow(4); add = lcg_add(4); break;
+    case 5: mul = lcg_pow(5); add = lcg_add(5); break;
+    default: mul = lcg_pow(6); add = lcg_add(6); break;
+  }
+  const uint32_t word = 2u * (uint32_t)spec_channel(spec);
+  const uint32_t draws_of = spec_draws((uint32_t)lane);
+  const uint32_t table = (uint32_t)spec_end_lane(draws_of) | (draws_of << 8);
+  const WriterRecord* records = J.writer_records;
+  uint32_t* states = J.site_state;
+  uint32_t s = uniform32(J.chain->state);
+  unsigned long long advances = 0;
+  uint32_t next[kChunkPerLane];
+  if (begin < end) fetch_chunk(records, begin, end, lane, next);
+  int buffer = 0;
+  for (uint32_t first = begin; first < end; first += kWriterChunk) {
+    const uint32_t n = min((uint32_t)kWriterChunk, end - first);
+#pragma unroll
+    for (int j = 0; j < kChunkPerLane; ++j) words[buffer][lane + j * 64] = next[j];
+    __syncthreads();
+    if (first + kWriterChunk < end) fetch_chunk(records, first + kWriterChunk, end, lane, next);
+    const uint32_t* w = words[buffer];
+    for (uint32_t g = 0; g < n; g += 64) {
+      const uint32_t gn = min(64u, n - g);
+      // the sites of this group whose three channels all draw
+      const uint32_t mine = min(g + (uint32_t)lane, n - 1u) * 6u;
+      const bool drawing =
+          g + (uint32_t)lane < n && ((w[mine + 1] & w[mine + 3] & w[mine + 5]) & kChanDraws) != 0;
+      const unsigned long long fast = __ballot(drawing);
+      // this lane's channel words for every site of the group, in registers
+      // before the chain starts (no LDS read is then in flight on it)
+      uint2 cw[64];
+#pragma unroll
+      for (uint32_t j = 0; j < 64; ++j)
+        cw[j] = *reinterpret_cast<const uint2*>(w + min(g + j, n - 1u) * 6u + word);
+      uint32_t collected = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < 64; ++j) {
+        if (j >= gn) continue;  // uniform: the group's last sites
+        collected = (uint32_t)dice_writelane((int)(s & kLcgMask), (int)j, (int)collected);
+        if ((fast >> j) & 1ull) {
+          const uint32_t t = __umul24(mul, s) + add;
+          const uint32_t m = (uint32_t)__ballot(spec_keep(t, cw[j].x, cw[j].y)) & 63u;
+          const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)table, (int)m);
+          s = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)(e & 0xFFu));
+          advances += e >> 8;
+        } else {
+          const uint32_t* r = w + (g + j) * 6u;
+          uint32_t d0, d1, d2;
+          s = chain_channel(uniform32(r[0]), uniform32(r[1]), s, d0);
+          s = chain_channel(uniform32(r[2]), uniform32(r[3]), s, d1);
+          s = chain_channel(uniform32(r[4]), uniform32(r[5]), s, d2);
+          advances += d0 + d1 + d2;
+        }
+      }
+      if ((uint32_t)lane < gn) states[first + g + lane] = collected;
+    }
+    __syncthreads();
+    buffer ^= 1;
+  }
+  if (lane == 0) {
+    J.chain->advances += advances;
+    J.band_end[band] = s & kLcgMask;
+    J.chain->state = s & kLcgMask;
   }
 }
 
@@ -1911,12 +1995,14 @@ hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
   return hipGetLastError();
 }
 
-hipError_t launch_writer_chain(const JobDev& J, uint32_t begin, uint32_t end, bool startup,
-                               int band, hipStream_t stream) {
-  if (startup)
-    hipLaunchKernelGGL(k_writer_chain<true>, dim3(1), dim3(64), 0, stream, J, begin, end, band);
-  else
-    hipLaunchKernelGGL(k_writer_chain<false>, dim3(1), dim3(64), 0, stream, J, begin, end, band);
+hipError_t launch_writer_startup_chain(const JobDev* d_jobs, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_writer_startup_chain, dim3(n), dim3(64), 0, stream, d_jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_writer_chain(const JobDev* d_jobs, int n, int band, int bands,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(k_writer_chain, dim3(n), dim3(64), 0, stream, d_jobs, band, bands);
   return hipGetLastError();
 }
 

@@ -54,10 +54,13 @@ hipError_t launch_writer_values(const JobDev& J, uint32_t begin, uint32_t end,
                                 hipStream_t stream);
 hipError_t launch_writer_startup_check(const JobDev& J, uint32_t final_state,
                                        const uint32_t* final_at, hipStream_t stream);
-// The serial writer chain on one wavefront over records [begin, end) of one
-// job (startup: the 6*W hidden slots), carrying its state in J.chain.
-hipError_t launch_writer_chain(const JobDev& J, uint32_t begin, uint32_t end, bool startup,
-                               int band, hipStream_t stream);
+// The serial writer chain on one wavefront per job of the device array
+// d_jobs[0, n) (jobs marked skip return at once), carrying each job's state
+// in its J.chain: the 6*W hidden startup slots, or the main sites of band
+// `band` of min(bands, height) (rows [H*b/bands, H*(b+1)/bands)).
+hipError_t launch_writer_startup_chain(const JobDev* d_jobs, int n, hipStream_t stream);
+hipError_t launch_writer_chain(const JobDev* d_jobs, int n, int band, int bands,
+                               hipStream_t stream);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t stream);
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

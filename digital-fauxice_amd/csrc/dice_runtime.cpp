@@ -599,6 +599,65 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   std::vector<std::thread> threads;
   std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
   std::vector<double> chain_waits(n, 0.0), band_waits(n, 0.0);
+  // The device writer chain (opt-in): after phase B, one wavefront per job
+  // walks its chain, one launch per band for the whole batch, on the fetch
+  // stream (it fetches nothing in this mode), each band followed by its
+  // writer values.  The job descriptors go up again first: jobs that failed
+  // after phase B are marked skip on the host only.
+  int32_t device_chain_rc = DICE_OK;
+  double device_chain_seconds = 0.0;
+  if (options.device_chain) {
+    const double t0 = now_seconds();
+    hipStream_t cs = fetch_stream_;
+    const int bands = std::max(1, std::min(options.bands, kMaxBands));
+    auto enqueue = [&]() -> int32_t {
+      HIP_TRY(hipStreamWaitEvent(cs, ev_[3], 0));
+      for (int j = 0; j < n; ++j) {
+        const JobDev& J = H_jobs[j];
+        if (J.skip) continue;
+        ChainState* init = ws_[j]->h_chain.as<ChainState>();
+        memset(init, 0, sizeof(ChainState));
+        init->state = jobs[j].rng_state;
+        HIP_TRY(hipMemcpyAsync(J.chain, init, sizeof(ChainState), hipMemcpyHostToDevice, cs));
+      }
+      HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
+                             cs));
+      const JobDev* d_jobs = d_jobs_.as<JobDev>();
+      HIP_TRY(launch_writer_startup_chain(d_jobs, n, cs));
+      for (int j = 0; j < n; ++j) {
+        const JobDev& J = H_jobs[j];
+        if (!J.skip) HIP_TRY(launch_writer_startup_check(J, 0, &J.chain->startup_final, cs));
+      }
+      for (int b = 0; b < bands; ++b) {
+        HIP_TRY(launch_writer_chain(d_jobs, n, b, bands, cs));
+        for (int j = 0; j < n; ++j) {
+          const JobDev& J = H_jobs[j];
+          if (J.skip) continue;
+          Workspace& w = *ws_[j];
+          if (b >= w.bands) continue;
+          const uint32_t* row_start = w.h_row_start.as<uint32_t>();
+          const uint32_t s0 = row_start[band_row(J.height, w.bands, b)];
+          const uint32_t s1 = row_start[band_row(J.height, w.bands, b + 1)];
+          HIP_TRY(launch_writer_values(J, s0, s1, 0, J.band_end + b, cs));
+          HIP_TRY(hipEventRecord(w.band[b], cs));
+        }
+      }
+      for (int j = 0; j < n; ++j) {
+        const JobDev& J = H_jobs[j];
+        if (J.skip) continue;
+        Workspace& w = *ws_[j];
+        HIP_TRY(hipMemcpyAsync(w.h_chain.ptr, J.chain, sizeof(ChainState), hipMemcpyDeviceToHost,
+                               cs));
+        HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 128, J.writer_counts,
+                               kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, cs));
+        HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
+                               hipMemcpyDeviceToHost, cs));
+      }
+      return DICE_OK;
+    };
+    device_chain_rc = enqueue();  // a failure fails every job; no hash waits for it
+    device_chain_seconds = now_seconds() - t0;
+  }
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
     Workspace& w = *ws_[j];
@@ -609,39 +668,10 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     }
     const int bands = w.bands;
     if (options.device_chain) {
-      // the serial chain on one wavefront of the device, band by band on
-      // this job's chain stream after phase B; every band is published at
-      // once (its event orders the hash thread's copy behind it)
-      const double t0 = now_seconds();
-      const JobDev& J = H_jobs[j];
-      const JobRequest& R = jobs[j];
-      const uint32_t* row_start = w.h_row_start.as<uint32_t>();
-      ChainState* init = w.h_chain.as<ChainState>();
-      memset(init, 0, sizeof(ChainState));
-      init->state = R.rng_state;
-      hipStream_t cs = w.chain_stream;
-      auto enqueue = [&]() -> int32_t {
-      HIP_TRY(hipStreamWaitEvent(cs, ev_[3], 0));
-      HIP_TRY(hipMemcpyAsync(J.chain, init, sizeof(ChainState), hipMemcpyHostToDevice, cs));
-      HIP_TRY(launch_writer_chain(J, 0, 6u * (uint32_t)J.width, true, 0, cs));
-      HIP_TRY(launch_writer_startup_check(J, 0, &J.chain->startup_final, cs));
-      for (int b = 0; b < bands; ++b) {
-        const uint32_t s0 = row_start[band_row(J.height, bands, b)];
-        const uint32_t s1 = row_start[band_row(J.height, bands, b + 1)];
-        HIP_TRY(launch_writer_chain(J, s0, s1, false, b, cs));
-        HIP_TRY(launch_writer_values(J, s0, s1, 0, J.band_end + b, cs));
-        HIP_TRY(hipEventRecord(w.band[b], cs));
-      }
-      HIP_TRY(hipMemcpyAsync(w.h_chain.ptr, J.chain, sizeof(ChainState), hipMemcpyDeviceToHost,
-                             cs));
-      HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 128, J.writer_counts,
-                             kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, cs));
-      HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
-                             hipMemcpyDeviceToHost, cs));
-      return DICE_OK;
-      };
-      if (int32_t rc = enqueue()) {  // the hash thread must not wait for it
-        outcome[j].fail(rc);
+      // every band is published at once: its event orders the hash
+      // thread's copy behind the chain and values of that band
+      if (device_chain_rc != DICE_OK) {
+        outcome[j].fail(device_chain_rc);
         {
           std::lock_guard<std::mutex> lock(w.mutex);
           w.aborted = true;
@@ -650,7 +680,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       } else {
         values_queued(j, bands);
       }
-      chain_seconds[j] = now_seconds() - t0;
+      chain_seconds[j] = device_chain_seconds;
     } else {
     // chain thread
     threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
@@ -829,7 +859,6 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   DICE_TRY(wait_stream(out_stream_));
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
-    if (options.device_chain) DICE_TRY(wait_stream(ws_[j]->chain_stream));
   }
   DICE_TRY(wait_stream(fetch_stream_));
   const double t_host = now_seconds();

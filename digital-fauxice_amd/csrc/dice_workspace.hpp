@@ -77,8 +77,6 @@ struct Workspace {
   hipEvent_t fetched[kMaxBands] = {};  // draw schedules of band b on the host (copy stream)
   hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (output stream)
   hipEvent_t startup_fetched = nullptr;
-  // the device writer chain of this job and its writer values, band by band
-  hipStream_t chain_stream = nullptr;
   int bands = 1;
   bool events = false;
   // chain -> copy scheduler -> hash hand-off: `published` counts the bands
@@ -93,10 +91,6 @@ struct Workspace {
       for (int b = 0; b < kMaxBands; ++b)
         if (set[b]) (void)hipEventDestroy(set[b]);
     if (startup_fetched) (void)hipEventDestroy(startup_fetched);
-    if (chain_stream) {
-      (void)hipStreamSynchronize(chain_stream);
-      (void)hipStreamDestroy(chain_stream);
-    }
   }
   int32_t init_events() {
     if (events) return DICE_OK;
@@ -106,7 +100,6 @@ struct Workspace {
     for (hipEvent_t* set : {band, fetched, copied})
       for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
     HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
-    HIP_TRY(hipStreamCreateWithFlags(&chain_stream, hipStreamNonBlocking));
     events = true;
     return DICE_OK;
   }

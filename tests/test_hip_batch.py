@@ -242,6 +242,11 @@ def test_uneven_crops_workload_matches_oracle():
             assert hashlib.sha256(result.output_rgb16.tobytes()).hexdigest() == \
                 expected[i]["output_sha256"]
     assert outcome.report["graph_replayed"] == 1
+    # the same batch with its 16 writer chains walked together on the device
+    chained = process_batch_hip(jobs, device_chain=True)
+    chained.raise_first()
+    for i, result in enumerate(chained.results):
+        assert _core(replay_fingerprint(result.replay)) == expected[i], ("device chain", i)
 
 
 def test_seed0_crop_matches_oracle():
