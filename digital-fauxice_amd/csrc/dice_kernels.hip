@@ -715,7 +715,8 @@ struct ScanShared {
   uint8_t writer_floor[TY];
   int32_t writer_gate[TY];
   uint32_t row_count[TY];
-  uint32_t wave_sums[kThreads / 64];
+  // eligible centres per wave in each slab of kThreads centres
+  uint32_t slab_wave[TY * TX / kThreads][kThreads / 64];
 };
 
 template <int TY, int TX>
@@ -781,13 +782,14 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
     __syncthreads();
 
     // Centre score (x3a.py:245-286), decision bars (streaming.py:157-198) and
-    // eligibility (streaming.py:461-466); each thread owns a contiguous run of
-    // centres so the queue comes out row-major.
+    // eligibility (streaming.py:461-466).  Centre k * kThreads + tid: the 64
+    // lanes of a wave read 64 consecutive LDS words (no bank conflicts) and
+    // store consecutive diagnostics.
     constexpr int kPer = TY * TX / kThreads;
-    uint32_t eligible_bits = 0, eligible_count = 0;
+    uint32_t eligible_bits = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int li = tid * kPer + k;
+      const int li = k * kThreads + tid;
       const int ty = li / TX, tx = li % TX;
       const int y = y0 + ty, x = x0 + tx;
       if (y >= H || x >= W) continue;
@@ -806,18 +808,36 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
       auto aux_at = [&](int dy, int dx) { return S.aux[ar + dy][ac + dx]; };
       if (decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit)) continue;
       eligible_bits |= 1u << k;
-      eligible_count += 1;
     }
-    uint32_t total;
-    uint32_t base = block_exclusive_scan(eligible_count, S.wave_sums, &total);
-    uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+    // The queue in row-major order: slab by slab, and within a slab wave by
+    // wave and lane by lane (counts by ballot, one barrier for all slabs).
+    const int lane = tid & 63, wave = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t rank[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      if (eligible_bits & (1u << k)) {
-        const int li = tid * kPer + k;
-        list[base++] = (uint16_t)li;
+      const unsigned long long mask = __ballot((eligible_bits >> k) & 1u);
+      rank[k] = (uint32_t)__popcll(mask & below);
+      if (lane == 0) S.slab_wave[k][wave] = (uint32_t)__popcll(mask);
+    }
+    __syncthreads();
+    uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      uint32_t before = 0, slab = 0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) {
+        const uint32_t c = S.slab_wave[k][w];
+        before += w < wave ? c : 0u;
+        slab += c;
+      }
+      if ((eligible_bits >> k) & 1u) {
+        const int li = k * kThreads + tid;
+        list[total + before + rank[k]] = (uint16_t)li;
         atomicAdd(&S.row_count[li / TX], 1u);
       }
+      total += slab;
     }
     __syncthreads();
     for (int i = tid; i < TY; i += kThreads)
