@@ -97,7 +97,7 @@ def parse_args() -> argparse.Namespace:
                         help="batched legs run each job's writer chain on one device wavefront")
     parser.add_argument("--drivers", type=int, default=2,
                         help="concurrent batches in the pipelined and serving legs")
-    parser.add_argument("--pipeline-rounds", type=int, default=4,
+    parser.add_argument("--pipeline-rounds", type=int, default=8,
                         help="the pipelined leg runs steps x batch x this many frames")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")
