@@ -262,6 +262,21 @@ inline uint32_t chain_records(const WriterRecord* r, size_t n, uint32_t s, uint3
   return s;
 }
 
+// The walk keeps the start state of every kStateGroup-th site only
+// (group_states[g] = state at site g * kStateGroup); the device re-walks each
+// group from it (k_writer_expand).  Sites [first, first + n) of the job's
+// main chain; returns the state after the last.
+constexpr uint32_t kStateGroup = 64;
+
+inline uint32_t chain_records_grouped(const WriterRecord* r, size_t first, size_t n, uint32_t s,
+                                      uint32_t* group_states, uint64_t& draws) {
+  for (size_t i = first; i < first + n; ++i) {
+    if (i % kStateGroup == 0) group_states[i / kStateGroup] = s;
+    s = draws_all_channels(r[i]) ? chain_site_drawing(r[i], s, draws) : chain_site(r[i], s, draws);
+  }
+  return s;
+}
+
 DICE_HD bool record_attempted(const WriterRecord& r) {
   return (r.ch[0] & kRecordAttemptedBit) != 0;
 }
@@ -336,6 +351,7 @@ struct JobDev {
   WriterRecord* writer_records; // one per site, row-major
   WriterRecord* su_writer_records; // 6*W
   uint32_t* site_state;         // LCG state at the start of each site
+  uint32_t* group_state;        // the same at every kStateGroup-th site (host walk)
   uint32_t* su_site_state;      // 6*W, the same for the hidden startup rows
   ChainState* chain;            // the device writer chain's state and receipt
   uint32_t* band_end;           // LCG state after each writer band

@@ -1386,6 +1386,24 @@ __global__ __launch_bounds__(kThreads) void k_writer_prep(JobSet set) {
   }
 }
 
+// Start states of sites [begin, end) of one job from those the host walk
+// kept at every kStateGroup-th site: one thread per group re-walks the
+// integer chain (the walk's own functions) from the group's start.
+__global__ __launch_bounds__(kThreads) void k_writer_expand(JobDev J, uint32_t begin,
+                                                            uint32_t end) {
+  const uint32_t g = begin / kStateGroup + blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t first = g * kStateGroup;
+  if (first >= end) return;
+  const uint32_t last = min(first + kStateGroup, end);
+  uint32_t s = J.group_state[g];
+  uint64_t draws = 0;
+  for (uint32_t i = first; i < last; ++i) {
+    if (i >= begin) J.site_state[i] = s;
+    const WriterRecord r = J.writer_records[i];
+    s = draws_all_channels(r) ? chain_site_drawing(r, s, draws) : chain_site(r, s, draws);
+  }
+}
+
 // The writer itself over sites [begin, end) of one job, every site in
 // parallel from the LCG state the serial integer chain recorded at its
 // start: values, written / changed counts, the patch of the output, and a
@@ -1996,6 +2014,15 @@ hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream) {
   set.queue = nullptr;
   const unsigned grid = resident_blocks(reinterpret_cast<const void*>(k_writer_prep), kThreads);
   hipLaunchKernelGGL(k_writer_prep, dim3(grid), dim3(kThreads), 0, stream, set);
+  return hipGetLastError();
+}
+
+hipError_t launch_writer_expand(const JobDev& J, uint32_t begin, uint32_t end,
+                                hipStream_t stream) {
+  if (end <= begin) return hipSuccess;
+  const long groups = (long)((end - 1) / kStateGroup - begin / kStateGroup + 1);
+  hipLaunchKernelGGL(k_writer_expand, dim3(blocks_for(groups, kThreads)), dim3(kThreads), 0,
+                     stream, J, begin, end);
   return hipGetLastError();
 }
 

@@ -46,6 +46,10 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint3
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                           hipStream_t stream);
 hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream);
+// Start states of sites [begin, end) from J.group_state (the state at every
+// kStateGroup-th site), written to J.site_state.
+hipError_t launch_writer_expand(const JobDev& J, uint32_t begin, uint32_t end,
+                                hipStream_t stream);
 // Per job: the writer over sites [begin, end) and the startup check; the
 // state after the range is end_state, or *end_state_at when that is given
 // (device memory written by the device chain).

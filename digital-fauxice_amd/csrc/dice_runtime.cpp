@@ -472,13 +472,17 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     const size_t su = 6 * (size_t)R.width;
     HIP_TRY(w.sites.ensure((count + 1) * sizeof(Site), stream_));
     HIP_TRY(w.records.ensure((count + 1) * sizeof(WriterRecord), stream_));
-    HIP_TRY(w.states.ensure((count + su + 1) * 4, stream_));
+    // device: startup states, site states, then the group start states;
+    // host: startup states, then the group start states
+    const size_t groups = count / kStateGroup + 2;
+    HIP_TRY(w.states.ensure((su + count + 1 + groups) * 4, stream_));
     HIP_TRY(w.h_records.ensure((count + su + 1) * sizeof(WriterRecord)));
-    HIP_TRY(w.h_states.ensure((count + su + 1) * 4));
+    HIP_TRY(w.h_states.ensure((su + std::max(count + 1, groups)) * 4));
     J.sites = w.sites.as<Site>();
     J.writer_records = w.records.as<WriterRecord>();
     J.su_site_state = w.states.as<uint32_t>();
     J.site_state = J.su_site_state + su;
+    J.group_state = J.site_state + count + 1;
   }
   HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
                          stream_));
@@ -708,7 +712,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         HIP_TRY(launch_writer_startup_check(J, state, nullptr, stream_));
         uint64_t advances = 0;
         const WriterRecord* main = records + su;
-        uint32_t* main_states = states + su;
+        uint32_t* group_states = states + su;
         for (int b = 0; b < bands; ++b) {
           const uint32_t s0 = row_start[band_row(H, bands, b)];
           const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
@@ -717,11 +721,17 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
           HIP_TRY(hipEventSynchronize(w.fetched[b]));
           fetch_wait += now_seconds() - f0;
           const double k0 = now_seconds();
-          state = chain_records(main + s0, s1 - s0, state, main_states + s0, advances);
+          state = chain_records_grouped(main, s0, s1 - s0, state, group_states, advances);
           walking += now_seconds() - k0;
           if (s1 > s0) {
-            HIP_TRY(hipMemcpyAsync(J.site_state + s0, main_states + s0, (size_t)(s1 - s0) * 4,
-                                   hipMemcpyHostToDevice, stream_));
+            // the group starts that fall in this band; the device expands
+            // them to the start state of every site of the band
+            const uint32_t g0 = (s0 + kStateGroup - 1) / kStateGroup;
+            const uint32_t g1 = (s1 + kStateGroup - 1) / kStateGroup;
+            if (g1 > g0)
+              HIP_TRY(hipMemcpyAsync(J.group_state + g0, group_states + g0, (size_t)(g1 - g0) * 4,
+                                     hipMemcpyHostToDevice, stream_));
+            HIP_TRY(launch_writer_expand(J, s0, s1, stream_));
             HIP_TRY(launch_writer_values(J, s0, s1, state, nullptr, stream_));
           }
           HIP_TRY(hipEventRecord(w.band[b], stream_));
