@@ -1579,8 +1579,9 @@ __global__ __launch_bounds__(64) void k_writer_startup_chain(const JobDev* jobs)
 // Band `band` of `bands` of every job's main sites (one workgroup per job;
 // a job with fewer rows than bands has one band per row).  Where all three
 // channels of a site draw, the lanes evaluate chain_site_drawing's six
-// tests at once (chain_site_lanes): a 24-bit multiply-add, a subtract, an
-// and-or and a compare per lane, then the ballot of the six keep bits reads
+// tests at once (chain_site_lanes): per lane, t - low from one 24-bit
+// multiply-add (add - low prepared with the site's words), a 24-bit mask and
+// a compare with the interval length, then the ballot of the six keep bits reads
 // the site's draws and end lane from a 64-entry table held one entry per
 // lane, and that lane's state becomes the chain value.  Other sites take
 // the per-channel path on the scalar unit.  The chain value is kept
@@ -1635,10 +1636,15 @@ __global__ __launch_bounds__(64) void k_writer_chain(const JobDev* jobs, int ban
       const unsigned long long fast = __ballot(drawing);
       // this lane's channel words for every site of the group, in registers
       // before the chain starts (no LDS read is then in flight on it)
+      // (.x holds add - low, so that one multiply-add gives t - low, and .y
+      // the interval length alone: the fast path only sees drawing channels)
       uint2 cw[64];
 #pragma unroll
-      for (uint32_t j = 0; j < 64; ++j)
+      for (uint32_t j = 0; j < 64; ++j) {
         cw[j] = *reinterpret_cast<const uint2*>(w + min(g + j, n - 1u) * 6u + word);
+        cw[j].x = add - cw[j].x;
+        cw[j].y &= kChanLen;
+      }
       uint32_t collected = 0;
 #pragma unroll
       for (uint32_t j = 0; j < 64; ++j) {
@@ -1646,7 +1652,9 @@ __global__ __launch_bounds__(64) void k_writer_chain(const JobDev* jobs, int ban
         collected = (uint32_t)dice_writelane((int)(s & kLcgMask), (int)j, (int)collected);
         if ((fast >> j) & 1ull) {
           const uint32_t t = __umul24(mul, s) + add;
-          const uint32_t m = (uint32_t)__ballot(spec_keep(t, cw[j].x, cw[j].y)) & 63u;
+          // keeps(t, low, len), with t - low from its own multiply-add
+          const uint32_t d = __umul24(mul, s) + cw[j].x;
+          const uint32_t m = (uint32_t)__ballot((d & kLcgMask) < cw[j].y) & 63u;
           const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)table, (int)m);
           s = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)(e & 0xFFu));
           advances += e >> 8;
