@@ -266,7 +266,7 @@ inline uint32_t chain_records(const WriterRecord* r, size_t n, uint32_t s, uint3
 // (group_states[g] = state at site g * kStateGroup); the device re-walks each
 // group from it (k_writer_expand).  Sites [first, first + n) of the job's
 // main chain; returns the state after the last.
-constexpr uint32_t kStateGroup = 64;
+constexpr uint32_t kStateGroup = 16;
 
 inline uint32_t chain_records_grouped(const WriterRecord* r, size_t first, size_t n, uint32_t s,
                                       uint32_t* group_states, uint64_t& draws) {

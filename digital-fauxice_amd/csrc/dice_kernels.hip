@@ -1395,12 +1395,19 @@ __global__ __launch_bounds__(kThreads) void k_writer_expand(JobDev J, uint32_t b
   const uint32_t first = g * kStateGroup;
   if (first >= end) return;
   const uint32_t last = min(first + kStateGroup, end);
+  // the group's records first, so that their loads overlap
+  WriterRecord r[kStateGroup];
+#pragma unroll
+  for (uint32_t k = 0; k < kStateGroup; ++k)
+    if (first + k < last) r[k] = J.writer_records[first + k];
   uint32_t s = J.group_state[g];
   uint64_t draws = 0;
-  for (uint32_t i = first; i < last; ++i) {
+#pragma unroll
+  for (uint32_t k = 0; k < kStateGroup; ++k) {
+    const uint32_t i = first + k;
+    if (i >= last) break;
     if (i >= begin) J.site_state[i] = s;
-    const WriterRecord r = J.writer_records[i];
-    s = draws_all_channels(r) ? chain_site_drawing(r, s, draws) : chain_site(r, s, draws);
+    s = draws_all_channels(r[k]) ? chain_site_drawing(r[k], s, draws) : chain_site(r[k], s, draws);
   }
 }
 
