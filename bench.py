@@ -84,7 +84,7 @@ SITE_FP64_OPS = 13 * (9 * 12 + 17) + 3 * 13 + 15 + 9 + 50
 # FETCH_SIZE and WRITE_SIZE (two --pmc runs over the plain leg, one frame per
 # dispatch), each scaled by what it reads for the plain copy kernel k_copy
 # over 1 GiB (x2.000 and x1.000)
-SITES_TRAFFIC_BYTES = 389_200_000
+SITES_TRAFFIC_BYTES = 389_500_000
 
 
 def parse_args() -> argparse.Namespace:
