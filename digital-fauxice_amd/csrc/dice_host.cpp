@@ -760,6 +760,18 @@ int32_t dice_t_writer_chain_host(const double* cand, const float* orig, const ui
       state = draws_all_channels(r) ? chain_site_lanes(r, state, draws)
                                     : chain_site(r, state, draws);
     }
+  } else if (generic == 3) {  // every kStateGroup-th state kept, each group re-walked
+    std::vector<uint32_t> groups((size_t)count / kStateGroup + 2);
+    state = chain_records_grouped(records.data(), 0, (size_t)count, state, groups.data(), draws);
+    for (int64_t first = 0; first < count; first += kStateGroup) {
+      uint32_t s = groups[(size_t)first / kStateGroup];
+      uint64_t again = 0;
+      for (int64_t i = first; i < std::min<int64_t>(count, first + kStateGroup); ++i) {
+        states[i] = s;
+        const WriterRecord& r = records[(size_t)i];
+        s = draws_all_channels(r) ? chain_site_drawing(r, s, again) : chain_site(r, s, again);
+      }
+    }
   } else if (generic) {
     for (int64_t i = 0; i < count; ++i) {
       states[i] = state;

@@ -392,7 +392,9 @@ int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height,
  * and the chain over them, exactly as the replay runs it; `generic` 1
  * forces the per-channel path for every site, 2 takes the device chain's
  * lane-parallel form (evaluated lane by lane) wherever all three channels
- * draw.  Sites: cand n*3 (binary64),
+ * draw, 3 keeps every 16th start state only and re-walks each group from it
+ * (the host walk and the device expansion of the replay).  Sites: cand n*3
+ * (binary64),
  * orig n*3, floor n (writer floor gate on/off).  Writes each site's start
  * state.  Stands in for the writer loop of write_band
  * (fast_cpu/kernels.py:1139-1204) in the L1 tests. */

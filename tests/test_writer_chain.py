@@ -90,7 +90,7 @@ SCALES = {
 
 
 @pytest.mark.parametrize("scales", sorted(SCALES))
-@pytest.mark.parametrize("generic", [0, 1, 2], ids=["fast", "per-channel", "lanes"])
+@pytest.mark.parametrize("generic", [0, 1, 2, 3], ids=["fast", "per-channel", "lanes", "grouped"])
 def test_writer_chain_matches_serial_writer(scales, generic):
     lut = O.lut()
     low, high = float(lut[655]), float(lut[64879])
