@@ -1653,6 +1653,22 @@ __global__ __launch_bounds__(64) void k_writer_chain(const JobDev* jobs, int ban
         cw[j].y &= kChanLen;
       }
       uint32_t collected = 0;
+      if (gn == 64u && fast == ~0ull) {  // a full group whose sites all draw
+        uint32_t group_draws = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 64; ++j) {
+          collected = (uint32_t)dice_writelane((int)(s & kLcgMask), (int)j, (int)collected);
+          const uint32_t t = __umul24(mul, s) + add;
+          const uint32_t d = __umul24(mul, s) + cw[j].x;
+          const uint32_t m = (uint32_t)__ballot((d & kLcgMask) < cw[j].y) & 63u;
+          const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)table, (int)m);
+          s = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)(e & 0xFFu));
+          group_draws += e >> 8;
+        }
+        advances += group_draws;
+        states[first + g + lane] = collected;
+        continue;
+      }
 #pragma unroll
       for (uint32_t j = 0; j < 64; ++j) {
         if (j >= gn) continue;  // uniform: the group's last sites
