@@ -294,3 +294,80 @@ def producer_frames(height: int, width: int) -> np.ndarray:
         rng.integers(8848, 65536, size=(height, width), dtype=np.uint16),
     )
     return pixels
+
+
+# ---------------------------------------------------------------------------
+# Prepass reductions (prepass.py:96-236) on their own.
+# ---------------------------------------------------------------------------
+
+# (valid_ir_threshold, alpha_scale, ratio_minimum, ratio_maximum,
+#  zero_denominator_alpha, completion_marker); every float is binary32-exact.
+PREPASS_DEFAULT = (8847.2255859375, 1.5, float(np.float32(-0.2)), float(np.float32(0.2)),
+                   float(np.float32(0.17)), 16)
+PREPASS_CUSTOM = (30000.0, 0.75, -1.0, 1.0, 0.5, 7)
+# zero_denominator_alpha = 1 on a frame with no fully valid tile: the record's
+# alpha is exactly 1, which the profile must refuse (profile.py:48-55).
+PREPASS_ALPHA_ONE = (8847.2255859375, 1.5, float(np.float32(-0.2)), float(np.float32(0.2)),
+                     1.0, 16)
+
+
+def _smooth_field(rng, height: int, width: int, low: int, high: int) -> np.ndarray:
+    y, x = np.indices((height, width), dtype=np.float64)
+    field = np.zeros((height, width))
+    for _ in range(4):
+        fy, fx = rng.uniform(0.2, 3.0, 2) * 2 * np.pi / np.array([height, width])
+        field += np.cos(fy * y + fx * x + rng.uniform(0, 2 * np.pi))
+    field = (field - field.min()) / max(np.ptp(field), 1e-9)
+    return low + (high - low) * field
+
+
+def prepass_cases() -> dict[str, tuple[np.ndarray, tuple]]:
+    """Prepass frames and parameters: every job case's prepass, ragged and
+    degenerate geometries (zero padding to a multiple of 8), threshold
+    straddles, flat tiles (zero deviations), a large smooth frame with many
+    in-window ratios, non-default parameters, and a record whose alpha is 1."""
+
+    out: dict[str, tuple[np.ndarray, tuple]] = {}
+    for name, (_, prepass) in job_cases().items():
+        out[f"job_{name}"] = (prepass, PREPASS_DEFAULT)
+
+    def random_frame(name, height, width, invalid=0.0, low=20000, high=60000):
+        rng = np.random.default_rng(_seed(f"prepass-{name}"))
+        px = rng.integers(low, high, size=(height, width, 4), dtype=np.uint16)
+        if invalid:
+            mask = rng.random((height, width)) < invalid
+            px[:, :, 3] = np.where(mask, rng.integers(0, 8848, size=(height, width)),
+                                   px[:, :, 3]).astype(np.uint16)
+        return px
+
+    one = np.zeros((1, 1, 4), np.uint16)
+    one[0, 0] = (31000, 27000, 22000, 50000)
+    out["one_by_one_valid"] = (one, PREPASS_DEFAULT)
+    dark = one.copy()
+    dark[0, 0, 3] = 100
+    out["one_by_one_invalid"] = (dark, PREPASS_DEFAULT)
+    out["ragged_7x13"] = (random_frame("7x13", 7, 13), PREPASS_DEFAULT)
+    out["ragged_3x100"] = (random_frame("3x100", 3, 100), PREPASS_DEFAULT)
+    out["ragged_13x21_mixed"] = (random_frame("13x21", 13, 21, invalid=0.02), PREPASS_DEFAULT)
+    out["all_invalid_16x24"] = (random_frame("invalid", 16, 24, low=0, high=8848),
+                                PREPASS_DEFAULT)
+    flat = np.empty((16, 16, 4), np.uint16)
+    flat[...] = (30000, 31000, 32000, 50000)
+    out["flat_tiles_16x16"] = (flat, PREPASS_DEFAULT)
+    edge = random_frame("edge", 16, 16)
+    edge[0:8, 0:8, 3] = 8848           # lowest valid raw IR: the tile is fully valid
+    edge[8:16, 8:16, 3] = 8848
+    edge[12, 12, 3] = 8847             # highest invalid raw IR: the tile is not
+    out["threshold_edge_16x16"] = (edge, PREPASS_DEFAULT)
+    rng = np.random.default_rng(_seed("prepass-smooth"))
+    smooth = np.empty((413, 282, 4), np.uint16)
+    for lane, (low, high) in enumerate(((9000, 50000), (9000, 50000), (9000, 50000),
+                                        (40000, 60000))):
+        smooth[:, :, lane] = np.clip(np.rint(_smooth_field(rng, 413, 282, low, high)
+                                             + rng.normal(0, 40, (413, 282))), 0, 65535)
+    out["smooth_413x282"] = (smooth, PREPASS_DEFAULT)
+    out["custom_params_40x56"] = (random_frame("custom", 40, 56, invalid=0.01, low=30001),
+                                  PREPASS_CUSTOM)
+    out["alpha_one_zero_denominator"] = (random_frame("alpha-one", 9, 17, low=0, high=8000),
+                                         PREPASS_ALPHA_ONE)
+    return out

@@ -35,7 +35,7 @@ def load_reference():
     pkg.__path__ = [str(REFERENCE_SRC)]
     sys.modules["portable_digital_ice"] = pkg
     names = ["engine", "streaming", "x3a", "reconstruction", "dither", "rng",
-             "contracts", "profile", "producer_parameters"]
+             "contracts", "profile", "producer_parameters", "prepass"]
     return {name: importlib.import_module(f"portable_digital_ice.{name}") for name in names}
 
 
@@ -131,9 +131,51 @@ def producer_vector(R, pixels) -> dict:
     }
 
 
-def main() -> int:
+def prepass_vector(R, pixels, params) -> dict:
+    """The reference's 0x40 record (prepass.py:96-236) and the profile's
+    fail-closed base (profile.py:48-55) for one prepass frame."""
+
+    contracts, prepass, profile = R["contracts"], R["prepass"], R["profile"]
+    frame = contracts.RGBI16Frame(pixels, contracts.AcquisitionEpoch.PREPASS, 285, "pp")
+    keys = ("valid_ir_threshold", "alpha_scale", "ratio_minimum", "ratio_maximum",
+            "zero_denominator_alpha", "completion_marker")
+    reduction = prepass.reduce_prepass_frame(
+        frame, parameters=prepass.PrepassParameters(**dict(zip(keys, params))),
+        frame_index=0, evidence_id="pp")
+    vector = {
+        "prepass_sha256": sha(pixels.astype("<u2")),
+        "parameters": list(params),
+        "payload_hex": reduction.record.payload.hex(),
+        "valid_pixels": int(reduction.valid_pixels),
+        "fully_valid_tiles": int(reduction.fully_valid_tiles),
+    }
+    try:
+        base = profile._base_primary(reduction.record)
+        vector["base_primary_bits"] = int(np.asarray(base, np.float32).view(np.uint32))
+    except ValueError as error:
+        vector["base_primary_error"] = str(error)
+    return vector
+
+
+def prepass_vectors(R) -> dict:
+    out = {}
+    for name, (pixels, params) in cases.prepass_cases().items():
+        out[name] = prepass_vector(R, pixels, params)
+        print("prepass", name, out[name]["payload_hex"][:16], flush=True)
+    return out
+
+
+def main(argv: list[str]) -> int:
     R = load_reference()
+    target = HERE / "reference_vectors.json"
+    if argv == ["--prepass-only"]:
+        # Refresh only the prepass records; the other vectors are unchanged.
+        out = json.loads(target.read_text())
+        out["prepass"] = prepass_vectors(R)
+        target.write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
+        return 0
     out = {"jobs": {}, "streams": {}, "producer": {}}
+    out["prepass"] = prepass_vectors(R)
     for name, (main_px, prepass) in cases.job_cases().items():
         out["jobs"][name] = job_vector(R, main_px, prepass)
         print("job", name, out["jobs"][name]["output_sha256"][:16], flush=True)
@@ -144,9 +186,9 @@ def main() -> int:
         key = f"{height}x{width}"
         out["producer"][key] = producer_vector(R, cases.producer_frames(height, width))
         print("producer", key, flush=True)
-    (HERE / "reference_vectors.json").write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
+    target.write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
     return 0
 
 
 if __name__ == "__main__":
-    raise SystemExit(main())
+    raise SystemExit(main(sys.argv[1:]))

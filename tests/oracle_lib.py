@@ -254,6 +254,17 @@ def schedule_struct(arrays: dict) -> DiceSchedule:
     return s
 
 
+def prepass_params(values) -> DicePrepassParams:
+    """DicePrepassParams from (valid_ir_threshold, alpha_scale, ratio_minimum,
+    ratio_maximum, zero_denominator_alpha, completion_marker)."""
+
+    params = DicePrepassParams()
+    (params.valid_ir_threshold, params.alpha_scale, params.ratio_minimum,
+     params.ratio_maximum, params.zero_denominator_alpha) = (float(v) for v in values[:5])
+    params.completion_marker = int(values[5])
+    return params
+
+
 def prepass(pixels: np.ndarray, params: DicePrepassParams,
             table: np.ndarray | None = None) -> DicePrepassResult:
     pixels = np.ascontiguousarray(pixels, dtype=np.uint16)

@@ -76,6 +76,23 @@ def test_prepass_matches_oracle(name):
     assert got.fully_valid_tiles == want.fully_valid_tiles
 
 
+@pytest.mark.parametrize("name", sorted(cases.prepass_cases()))
+def test_prepass_matches_reference_records(name, reference_vectors):
+    """dice_prepass against the reference's 0x40 records (prepass.py:96-236)."""
+
+    pixels, params = cases.prepass_cases()[name]
+    want = reference_vectors["prepass"][name]
+    got = DicePrepassResult()
+    px = np.ascontiguousarray(pixels)
+    assert hip_engine.library().dice_prepass(ptr(px, C.c_uint16), px.shape[0], px.shape[1],
+                                            ptr(O.lut(), C.c_float),
+                                            C.byref(O.prepass_params(params)),
+                                            C.byref(got)) == 0
+    assert bytes(got.payload).hex() == want["payload_hex"]
+    assert got.valid_pixels == want["valid_pixels"]
+    assert got.fully_valid_tiles == want["fully_valid_tiles"]
+
+
 @pytest.mark.parametrize("shape", cases.PRODUCER_CASES)
 def test_stage_table_from_schedule_matches_oracle(shape):
     pixels = cases.producer_frames(*shape)

@@ -34,7 +34,7 @@ from dice_amd.parameters import (
     generate_nikon_response_lut,
     to_dice_params,
 )
-from dice_amd.profile import DEFAULT_PROFILE, neutral_prepass_record
+from dice_amd.profile import DEFAULT_PROFILE, PrepassRecord, _base_primary, neutral_prepass_record
 from dice_amd.stages import constant_stage_table
 
 
@@ -232,6 +232,43 @@ def test_prepass_and_producer_invariants():
     s = O.producer_schedule(main)
     assert s["active_width"] == 8
     assert np.all(s["accepted"] == 8)
+
+
+PREPASS_CASES = cases.prepass_cases()
+
+
+@pytest.mark.parametrize("name", sorted(PREPASS_CASES))
+def test_prepass_record_matches_reference(name, reference_vectors):
+    """The whole 0x40 record, valid_pixels and fully_valid_tiles against the
+    reference's reduce_prepass_frame (prepass.py:96-236) on ragged, 1x1,
+    all-invalid, flat, threshold-edge, large and non-default-parameter
+    frames; and the profile's fail-closed base (profile.py:48-55)."""
+
+    pixels, params = PREPASS_CASES[name]
+    want = reference_vectors["prepass"][name]
+    assert want["prepass_sha256"] == sha(pixels.astype("<u2"))
+    assert want["parameters"] == list(params)
+    got = O.prepass(pixels, O.prepass_params(params))
+    assert bytes(got.payload).hex() == want["payload_hex"]
+    assert got.valid_pixels == want["valid_pixels"]
+    assert got.fully_valid_tiles == want["fully_valid_tiles"]
+    record = PrepassRecord(bytes(got.payload))
+    if "base_primary_error" in want:
+        with pytest.raises(ValueError, match=want["base_primary_error"]):
+            _base_primary(record)
+    else:
+        assert int(np.asarray(_base_primary(record), np.float32).view(np.uint32)) == (
+            want["base_primary_bits"])
+
+
+def test_prepass_vectors_cover_the_fail_closed_and_degenerate_cases(reference_vectors):
+    vectors = reference_vectors["prepass"]
+    assert vectors["alpha_one_zero_denominator"]["payload_hex"][8:16] == "0000803f"
+    assert "base_primary_error" in vectors["alpha_one_zero_denominator"]
+    assert vectors["all_invalid_16x24"]["valid_pixels"] == 0
+    assert vectors["flat_tiles_16x16"]["fully_valid_tiles"] == 4
+    assert vectors["threshold_edge_16x16"]["fully_valid_tiles"] == 3
+    assert vectors["smooth_413x282"]["fully_valid_tiles"] > 1000
 
 
 def test_all_below_threshold_keeps_initial_means():
