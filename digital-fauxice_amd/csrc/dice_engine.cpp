@@ -27,6 +27,8 @@
 #include "dice_common.hpp"
 #include "dice_runtime.hpp"
 
+#pragma clang fp contract(off)
+
 using namespace dice;
 
 namespace {

@@ -674,6 +674,39 @@ int32_t dice_process_batch(dice_ctx* ctx, dice_job* jobs, int32_t count, uint32_
   return process_jobs(*ctx->runner, jobs, count, batch_flags, report);
 }
 
+int32_t dice_t_fp_canaries(dice_ctx* ctx, const dice_fp_canary* io) {
+  if (!io || io->count < 0) return set_error(DICE_INVALID, "canary arguments are missing");
+  return with_device_buffers(ctx, [&]() -> int32_t {
+    const int64_t n = io->count;
+    if (n == 0) return DICE_OK;
+    Scratch S;
+    double *a = S.alloc<double>(n), *b = S.alloc<double>(n), *c = S.alloc<double>(n);
+    double *mul_add = S.alloc<double>(n), *quotient = S.alloc<double>(n);
+    float *fa = S.alloc<float>(n), *fb = S.alloc<float>(n), *fc = S.alloc<float>(n);
+    float *narrowed = S.alloc<float>(n), *f32_mul_add = S.alloc<float>(n);
+    float *f32_quotient = S.alloc<float>(n), *f32_product = S.alloc<float>(n);
+    if (!a || !b || !c || !mul_add || !quotient || !fa || !fb || !fc || !narrowed ||
+        !f32_mul_add || !f32_quotient || !f32_product)
+      return set_error(DICE_INTERNAL, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(a, io->a, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b, io->b, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c, io->c, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fa, io->fa, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fb, io->fb, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fc, io->fc, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_fp_canaries(n, a, b, c, fa, fb, fc, mul_add, quotient, narrowed, f32_mul_add,
+                               f32_quotient, f32_product, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(io->mul_add, mul_add, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(io->quotient, quotient, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(io->narrowed, narrowed, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(io->f32_mul_add, f32_mul_add, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(io->f32_quotient, f32_quotient, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(io->f32_product, f32_product, n * 4, hipMemcpyDeviceToHost));
+    return DICE_OK;
+  });
+}
+
 int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches, int64_t count,
                                  double* out3) {
   return with_device_buffers(ctx, [&]() -> int32_t {

@@ -68,6 +68,12 @@ hipError_t launch_writer_chain(const JobDev* d_jobs, int n, int band, int bands,
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t stream);
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// dice_canary.hip: the level-0 exactness canaries, element-wise over n.
+hipError_t launch_fp_canaries(int64_t n, const double* a, const double* b, const double* c,
+                              const float* fa, const float* fb, const float* fc,
+                              double* mul_add, double* quotient, float* narrowed,
+                              float* f32_mul_add, float* f32_quotient, float* f32_product,
+                              hipStream_t s);
 hipError_t launch_t_unscaled_averages(const float* patches, long count, double* out,
                                       hipStream_t s);
 hipError_t launch_t_rgb_unscaled_averages(const float* patches, long count,

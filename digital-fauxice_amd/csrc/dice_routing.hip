@@ -20,6 +20,8 @@
 
 #include "dice_routing.hpp"
 
+#pragma clang fp contract(off)
+
 namespace dice {
 namespace {
 

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdlib>
 #include <cstring>
@@ -600,7 +601,18 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     cq.cv.notify_one();
   };
   std::vector<JobThreads> outcome(n);
+  // Set by the calling thread when the job's cancel callback says so; the
+  // job's chain and hash threads stop at their next band.
+  std::unique_ptr<std::atomic<int>[]> cancel(new std::atomic<int>[n]);
+  for (int j = 0; j < n; ++j) cancel[j].store(0);
   std::vector<std::thread> threads;
+  std::atomic<int> threads_done{0};
+  auto spawn = [&threads, &threads_done](auto body) {
+    threads.emplace_back([body, &threads_done]() mutable {
+      body();
+      threads_done.fetch_add(1);
+    });
+  };
   std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
   std::vector<double> chain_waits(n, 0.0), band_waits(n, 0.0);
   // The device writer chain (opt-in): after phase B, one wavefront per job
@@ -687,8 +699,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       chain_seconds[j] = device_chain_seconds;
     } else {
     // chain thread
-    threads.emplace_back([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
-                          &chain_waits, &values_queued, &chain_failed, &fetch_band]() {
+    spawn([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
+                          &chain_waits, &values_queued, &chain_failed, &fetch_band,
+                          &cancel]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const JobDev& J = H_jobs[j];
@@ -714,6 +727,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         const WriterRecord* main = records + su;
         uint32_t* group_states = states + su;
         for (int b = 0; b < bands; ++b) {
+          if (cancel[j].load()) return cancelled_error();
           const uint32_t s0 = row_start[band_row(H, bands, b)];
           const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
           if (b + kFetchAhead < bands) DICE_TRY(fetch_band(J, w, b + kFetchAhead));
@@ -763,8 +777,8 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     });
     }
     // hash / download thread
-    threads.emplace_back([this, j, bands, jobs, &outcome, &hash_seconds, &copy_seconds,
-                          &band_waits]() {
+    spawn([this, j, bands, jobs, &outcome, &hash_seconds, &copy_seconds,
+                          &band_waits, &cancel]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const bool want_sha = (R.flags & DICE_SKIP_SHA256) == 0;
@@ -775,9 +789,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         HIP_TRY(hipSetDevice(device_));
         const int64_t H = R.height, W = R.width;
         uint8_t* staging = w.h_out.as<uint8_t>();
-        uint8_t* caller = host_out ? reinterpret_cast<uint8_t*>(R.out) : nullptr;
         Sha256 sha;
         for (int b = 0; b < bands; ++b) {
+          if (cancel[j].load()) return cancelled_error();
           const double p0 = now_seconds();
           {
             std::unique_lock<std::mutex> lock(w.mutex);
@@ -794,11 +808,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
           const size_t begin = (size_t)band_row(H, bands, b) * W * 6;
           const size_t end = (size_t)band_row(H, bands, b + 1) * W * 6;
           constexpr size_t kPiece = 256 << 10;
-          for (size_t at = begin; at < end; at += kPiece) {
-            const size_t len = std::min(kPiece, end - at);
-            if (want_sha) sha.update(staging + at, len);
-            if (caller) memcpy(caller + at, staging + at, len);
-          }
+          if (want_sha)
+            for (size_t at = begin; at < end; at += kPiece)
+              sha.update(staging + at, std::min(kPiece, end - at));
           copying += c1 - c0;
           hashing += now_seconds() - c1;
         }
@@ -815,7 +827,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     });
   }
   // the copy scheduler: the lowest band ready across all jobs goes next
-  threads.emplace_back([this, n, jobs, H_jobs, &cq, &outcome]() {
+  spawn([this, n, jobs, H_jobs, &cq, &outcome]() {
     (void)hipSetDevice(device_);
     for (;;) {
       int j = -1, b = 0;
@@ -864,7 +876,43 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       w.cv.notify_all();
     }
   });
+  // The calling thread reports each job's reconstruction progress band by
+  // band and polls its cancel callback once per band (the callbacks belong
+  // to the caller's thread).  Bands a job finished before this thread saw
+  // them are reported in order, so every job sees exactly one poll per
+  // band, then one more before its result is committed.
+  std::vector<int> reported(n, 0);
+  auto report_bands = [&](int j) {
+    JobRequest& R = jobs[j];
+    Workspace& w = *ws_[j];
+    int published;
+    {
+      std::lock_guard<std::mutex> lock(w.mutex);
+      published = w.published;
+    }
+    while (reported[j] < published && !cancel[j].load()) {
+      reported[j] += 1;
+      R.cb.progress(3, band_row(R.height, w.bands, reported[j]), R.height, 0, 0);
+      if (R.cb.cancelled()) {
+        cancel[j].store(1);
+        std::lock_guard<std::mutex> lock(w.mutex);
+        w.cv.notify_all();
+      }
+    }
+  };
+  bool any_callbacks = false;
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip && jobs[j].cb.cb) any_callbacks = true;
+  if (any_callbacks) {
+    while (threads_done.load() < (int)threads.size()) {
+      for (int j = 0; j < n; ++j)
+        if (!H_jobs[j].skip) report_bands(j);
+      std::this_thread::sleep_for(std::chrono::microseconds(500));
+    }
+  }
   for (std::thread& t : threads) t.join();
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip && outcome[j].code == DICE_OK) report_bands(j);
   DICE_TRY(wait_stream(stream_));
   DICE_TRY(wait_stream(out_stream_));
   for (int j = 0; j < n; ++j) {
@@ -874,6 +922,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   const double t_host = now_seconds();
 
   // ---- counts, checks, diagnostics, results
+  std::vector<int> commit;  // jobs whose host output is written once every check passed
   for (int j = 0; j < n; ++j) {
     JobRequest& R = jobs[j];
     const JobDev& J = H_jobs[j];
@@ -887,6 +936,11 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
     if (int32_t rc = flag_error(flags)) {
       R.code = rc;
+      R.error = error_buffer();
+      continue;
+    }
+    if (cancel[j].load() || R.cb.cancelled()) {
+      R.code = cancelled_error();
       R.error = error_buffer();
       continue;
     }
@@ -911,6 +965,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       R.result->final_rng_state = chain.state;
     }
     const size_t pixels = (size_t)R.height * R.width;
+    if ((R.flags & DICE_OUT_ON_DEVICE) == 0) commit.push_back(j);
     if (R.diag) {
       if (R.diag->score_plane)
         HIP_TRY(hipMemcpy(R.diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
@@ -942,6 +997,35 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         T.sites_kernel_ms += ms_b;
       }
     }
+  }
+  // The caller's host buffers are written only now, after every check of
+  // their job passed (engine.py:256-260 writes the caller's output once,
+  // on success): pinned staging -> caller, in parallel pieces.
+  if (!commit.empty()) {
+    struct Piece {
+      uint8_t* dst;
+      const uint8_t* src;
+      size_t len;
+    };
+    std::vector<Piece> pieces;
+    constexpr size_t kCommitPiece = 8u << 20;
+    for (int j : commit) {
+      const size_t bytes = (size_t)jobs[j].height * jobs[j].width * 6;
+      uint8_t* dst = reinterpret_cast<uint8_t*>(jobs[j].out);
+      const uint8_t* src = ws_[j]->h_out.as<uint8_t>();
+      for (size_t at = 0; at < bytes; at += kCommitPiece)
+        pieces.push_back({dst + at, src + at, std::min(kCommitPiece, bytes - at)});
+    }
+    std::atomic<size_t> next{0};
+    auto copier = [&]() {
+      for (size_t i; (i = next.fetch_add(1)) < pieces.size();)
+        memcpy(pieces[i].dst, pieces[i].src, pieces[i].len);
+    };
+    const size_t workers = std::min<size_t>(pieces.size(), 8);
+    std::vector<std::thread> copiers;
+    for (size_t t = 1; t < workers; ++t) copiers.emplace_back(copier);
+    copier();
+    for (std::thread& t : copiers) t.join();
   }
   if (trace_enabled()) {
     const double t_end = now_seconds();

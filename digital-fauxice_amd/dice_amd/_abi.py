@@ -120,6 +120,24 @@ class DiceSchedule(C.Structure):
     ]
 
 
+class DiceFpCanary(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64),
+        ("a", C.POINTER(C.c_double)),
+        ("b", C.POINTER(C.c_double)),
+        ("c", C.POINTER(C.c_double)),
+        ("fa", C.POINTER(C.c_float)),
+        ("fb", C.POINTER(C.c_float)),
+        ("fc", C.POINTER(C.c_float)),
+        ("mul_add", C.POINTER(C.c_double)),
+        ("quotient", C.POINTER(C.c_double)),
+        ("narrowed", C.POINTER(C.c_float)),
+        ("f32_mul_add", C.POINTER(C.c_float)),
+        ("f32_quotient", C.POINTER(C.c_float)),
+        ("f32_product", C.POINTER(C.c_float)),
+    ]
+
+
 class DicePrepassParams(C.Structure):
     _fields_ = [
         ("valid_ir_threshold", C.c_float),
@@ -367,6 +385,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "t_decision": (C.c_int32, [VOIDP, f32p, i64, i64, C.POINTER(DiceParams),
                                u8p]),
     "t_copy": (C.c_int32, [VOIDP, C.c_uint64, C.c_int32, f64p]),
+    "t_fp_canaries": (C.c_int32, [VOIDP, C.POINTER(DiceFpCanary)]),
     "t_writer_chain_host": (C.c_int32, [f64p, f32p, u8p, i64, C.c_float, C.c_float, f32p,
                                         C.c_uint32, C.c_int32, u32p, u32p, u64p]),
     "route_at_floor": (C.c_int32, [VOIDP, VOIDP, i64, i64, C.c_uint32, i64, i64, i64,

@@ -82,7 +82,12 @@ def process_hip(
     params = stable_profile_params()
     prepass_params = DEFAULT_PROFILE.prepass_parameters()
     ctx = context()
-    working_output = np.empty(expected_shape, dtype=np.uint16)
+    # The library writes a host output buffer only after the whole job
+    # succeeded, so a C-contiguous caller buffer is handed over directly.
+    if destination is not None and destination.flags.c_contiguous:
+        working_output = destination
+    else:
+        working_output = np.empty(expected_shape, dtype=np.uint16)
     diag = None
     if export_diagnostics:
         score = np.empty((height, width), np.float32)
@@ -121,7 +126,8 @@ def process_hip(
     if destination is None:
         output = working_output
     else:
-        np.copyto(destination, working_output)
+        if working_output is not destination:
+            np.copyto(destination, working_output)
         output = destination
     return ProcessingResult(
         output_rgb16=output,

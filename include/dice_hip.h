@@ -387,6 +387,31 @@ int32_t dice_t_decision(dice_ctx* ctx, const float* aux, int64_t height,
                         int64_t width, const dice_params* params,
                         uint8_t* fallback);
 
+/* Level-0 compiler canaries (no reference entry point: the reference's CUDA
+ * backend runs the same probes at module load, docs/cuda-decision-record.md:
+ * 23-40, and tests/test_fast_cpu_bitexact.py:347-375 checks its CPU build).
+ * Element-wise over `count` operands read from device memory:
+ *   mul_add = a*b + c (binary64, two roundings), quotient = a/b,
+ *   narrowed = (float)a, f32_mul_add = fa*fb + fc (binary32, two roundings),
+ *   f32_quotient = fa/fb, f32_product = fa*fb (denormal results kept). */
+typedef struct dice_fp_canary {
+  int64_t count;
+  const double* a;
+  const double* b;
+  const double* c;
+  const float* fa;
+  const float* fb;
+  const float* fc;
+  double* mul_add;
+  double* quotient;
+  float* narrowed;
+  float* f32_mul_add;
+  float* f32_quotient;
+  float* f32_product;
+} dice_fp_canary;
+
+int32_t dice_t_fp_canaries(dice_ctx* ctx, const dice_fp_canary* io);
+
 /* The serial writer on the host, no device involved: the integer draw
  * schedule of each site (the kernel's writer_record, compiled for the host)
  * and the chain over them, exactly as the replay runs it; `generic` 1
