@@ -116,6 +116,7 @@ int32_t require_tables() {
 struct dice_ctx {
   int device = 0;
   uint64_t hbm_budget = 0;
+  uint8_t* scratch = nullptr;  // this device's copy of `scratch` (dice_process_job_noalloc)
   std::mutex mutex;
   DeviceTables tables;
   std::unique_ptr<Runner> runner;
@@ -457,6 +458,35 @@ int32_t dice_probe(dice_device_info* info) {
   return DICE_OK;
 }
 
+namespace {
+
+// The allocation-free entry point's scratch (dice_process_job_noalloc): one
+// block per GPU, device memory the library holds from its load until the
+// process ends.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) uint8_t scratch[DICE_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+uint8_t* g_scratch_ptr[256] = {};
+
+// The device address of `scratch` on `device` (current), looked up once.
+int32_t scratch_on(int device, uint8_t** out) {
+  if (device < 0 || device >= 256) return set_error(DICE_UNAVAILABLE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(DICE_UNAVAILABLE, "cannot locate the allocation-free scratch on device %d: %s",
+                       device, hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return DICE_OK;
+}
+
+}  // namespace
+
 int32_t dice_ctx_create(int32_t device, uint64_t hbm_budget, dice_ctx** out) {
   *out = nullptr;
   DICE_TRY(require_tables());
@@ -476,6 +506,7 @@ int32_t dice_ctx_create(int32_t device, uint64_t hbm_budget, dice_ctx** out) {
     return set_error(DICE_UNAVAILABLE, "cannot create the HIP stream and events: %s",
                      error_buffer());
   ctx->stream = ctx->runner->stream();
+  if (int32_t rc = scratch_on(device, &ctx->scratch)) return rc;
   for (hipEvent_t& event : ctx->events)
     if (hipEventCreate(&event) != hipSuccess)
       return set_error(DICE_UNAVAILABLE, "cannot create HIP events");
@@ -665,6 +696,31 @@ int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t heigh
   if (rc == DICE_OK) cb.progress(4, 1, 1, 0, 0);
   if (timings) timings->total = now_seconds() - start;
   return rc;
+}
+
+int32_t dice_process_job_noalloc(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t height,
+                                 int64_t width, const uint16_t* prepass_rgbi,
+                                 int64_t prepass_height, int64_t prepass_width,
+                                 const dice_params* params,
+                                 const dice_prepass_params* prepass_params, uint16_t* out_rgb16,
+                                 dice_replay_result* result) {
+  DICE_TRY(activate(ctx));
+  if (!main_rgbi || !prepass_rgbi || !params || !prepass_params || !out_rgb16 || !result)
+    return set_error(DICE_INVALID, "a required pointer is null");
+  std::lock_guard<std::mutex> lock(ctx->mutex);
+  memset(result, 0, sizeof *result);
+  if (!ctx->scratch) return set_error(DICE_UNAVAILABLE, "the allocation-free scratch is not set up");
+  DICE_TRY(admit_job(prepass_rgbi, prepass_height, prepass_width, height, width, *params,
+                     *prepass_params, nullptr));
+  JobRequest R;
+  R.main = main_rgbi;
+  R.height = height;
+  R.width = width;
+  R.flags = DICE_MAIN_ON_DEVICE | DICE_OUT_ON_DEVICE | DICE_SKIP_SHA256;
+  R.out = out_rgb16;
+  R.params = *params;
+  R.result = result;
+  return ctx->runner->run_in_arena(R, ctx->scratch, DICE_NOALLOC_SCRATCH_BYTES);
 }
 
 int32_t dice_process_batch(dice_ctx* ctx, dice_job* jobs, int32_t count, uint32_t batch_flags,

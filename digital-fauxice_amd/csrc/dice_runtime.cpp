@@ -27,6 +27,14 @@ namespace dice {
 
 namespace {
 
+// run_in_arena's pinned host area: the row starts of up to kArenaMaxRows
+// rows, then the chain state, the counters and flags, the writer slots.
+constexpr size_t kArenaRowStartBytes = (size_t)(Runner::kArenaMaxRows + 1) * 4;
+constexpr size_t kArenaChainAt = (kArenaRowStartBytes + 255) & ~(size_t)255;
+constexpr size_t kArenaSmallAt = kArenaChainAt + 256;
+constexpr size_t kArenaSlotsAt = kArenaSmallAt + 128;
+constexpr size_t kArenaIoBytes = kArenaSlotsAt + (size_t)kCountSlots * kCountStride * 8;
+
 // DICE_TRACE=1 prints one timeline line per batch to stderr.
 bool trace_enabled() {
   static const bool on = [] {
@@ -131,6 +139,7 @@ int32_t Runner::init() {
   HIP_TRY(queue_.ensure(256));
   HIP_TRY(hipMemset(queue_.ptr, 0, 256));
   HIP_TRY(h_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
+  HIP_TRY(h_io_.ensure(kArenaIoBytes));
   return DICE_OK;
 }
 
@@ -1084,6 +1093,175 @@ int32_t Runner::producer_schedule(const uint16_t* rgbi, int64_t height, int64_t 
   for (int i = 0; i < 12; ++i)
     HIP_TRY(hipMemcpyAsync(dst[i], sched_[i].ptr, H * sizes[i], hipMemcpyDeviceToHost, stream_));
   DICE_TRY(wait_stream(stream_));
+  return DICE_OK;
+}
+
+namespace {
+
+// Consecutive 256-byte-aligned pieces of one device block; `used` keeps
+// counting past the end, so a failed fit reports what the job needs.
+struct Arena {
+  uint8_t* base;
+  size_t size;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    used = at + std::max<size_t>(count, 1) * sizeof(T);
+    return used <= size ? reinterpret_cast<T*>(base + at) : nullptr;
+  }
+  bool fits() const { return used <= size; }
+};
+
+}  // namespace
+
+int32_t Runner::run_in_arena(JobRequest& R, uint8_t* arena, size_t arena_bytes) {
+  HIP_TRY(hipSetDevice(device_));
+  memset(R.result, 0, sizeof *R.result);
+  const int64_t H = R.height, W = R.width;
+  if (H > kArenaMaxRows)
+    return set_error(DICE_OVER_BUDGET, "the allocation-free entry point takes at most %lld rows",
+                     (long long)kArenaMaxRows);
+  const int TY = tile_rows_, TX = tile_cols_;
+  JobDev& J = *h_jobs_.as<JobDev>();
+  memset(&J, 0, sizeof J);
+  J.P = device_params(R.params);
+  J.K = writer_consts(R.params);
+  J.factor_b = R.factor_b;
+  J.run_producer = 1;
+  J.height = (int32_t)H;
+  J.width = (int32_t)W;
+  J.tile_rows = TY;
+  J.tile_cols = TX;
+  J.tiles_x = (int32_t)((W + TX - 1) / TX);
+  J.tiles_y = (int32_t)((H + TY - 1) / TY);
+  J.win = full_window(J.height, J.tiles_y);
+  const size_t tiles = (size_t)J.tiles_x * J.tiles_y;
+  const size_t segments = (size_t)J.tiles_y * TY * J.tiles_x;
+  const size_t stages = (size_t)H + 6;
+  const size_t su = 6 * (size_t)W;
+  Arena A{arena, arena_bytes};
+  J.rgbi = R.main;
+  J.out = R.out;
+  J.lut = tables_->lut.as<float>();
+  J.noop_delta = tables_->noop_delta.as<uint32_t>();
+  J.factor_high = tables_->high.as<uint32_t>();
+  J.factor_low = tables_->low.as<uint32_t>();
+  J.stages.alpha = A.take<float>(stages);
+  J.stages.offset = A.take<float>(stages);
+  J.stages.replacement = A.take<float>(stages);
+  J.stages.has_replacement = A.take<uint8_t>(stages);
+  J.stages.score_base = A.take<float>(stages);
+  J.stages.coarse = A.take<float>(stages);
+  J.stages.gate_secondary = A.take<uint8_t>(stages);
+  J.stages.row_gate = A.take<int32_t>(stages);
+  J.site_lists = A.take<uint16_t>(tiles * TY * TX);
+  J.tile_count = A.take<uint32_t>(tiles);
+  J.seg_count = A.take<uint32_t>(segments);
+  J.seg_offset = A.take<uint32_t>(segments);
+  J.seg_block_sums = A.take<uint32_t>(segments / 1024 + 2);
+  J.row_start = A.take<uint32_t>(H + 1);
+  J.counters = A.take<uint32_t>(16);
+  J.error_flags = A.take<uint32_t>(16);
+  J.writer_counts = A.take<unsigned long long>((size_t)kCountSlots * kCountStride);
+  J.su_aux = A.take<float>(5 * W);
+  J.su_score = A.take<float>(5 * W);
+  J.su_work = A.take<float>(15 * W);
+  J.su_sites = A.take<Site>(su);
+  J.su_attempted = A.take<uint8_t>(su);
+  J.su_writer_records = A.take<WriterRecord>(su);
+  J.failpos = A.take<uint8_t>(((H + 7) / 8) * (W / 8) + 8);
+  J.row_sums = A.take<double>(H * 3);
+  J.accepted = A.take<uint32_t>(H);
+  J.epoch_adds = A.take<double>((H / 8) * 2 + 2);
+  J.cumulative = A.take<double>(H * 3);
+  J.epoch_wide = A.take<double>((H / 8) * 2 + 2);
+  J.epoch_store = A.take<float>((H / 8) * 2 + 2);
+  uint8_t* chain = A.take<uint8_t>(sizeof(ChainState) + kMaxBands * 4);
+  J.chain = reinterpret_cast<ChainState*>(chain);
+  J.band_end = reinterpret_cast<uint32_t*>(chain + sizeof(ChainState));
+  if (!A.fits())
+    return set_error(DICE_OVER_BUDGET,
+                     "a %lldx%lld job needs %zu bytes of scratch before its sites; the "
+                     "allocation-free entry point holds %zu",
+                     (long long)H, (long long)W, A.used, arena_bytes);
+  const JobDev* d = d_jobs_.as<JobDev>();
+  uint8_t* io = h_io_.as<uint8_t>();
+  uint32_t* h_row_start = reinterpret_cast<uint32_t*>(io);
+  ChainState* h_chain = reinterpret_cast<ChainState*>(io + kArenaChainAt);
+  uint32_t* h_counters = reinterpret_cast<uint32_t*>(io + kArenaSmallAt);
+  uint32_t* h_flags = reinterpret_cast<uint32_t*>(io + kArenaSmallAt + 64);
+  const unsigned long long* h_slots =
+      reinterpret_cast<const unsigned long long*>(io + kArenaSlotsAt);
+  fine_on_ = false;
+
+  // phase A: zero, producer + stage table, analysis scan, segment scan
+  HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev), hipMemcpyHostToDevice,
+                         stream_));
+  DICE_TRY(launch_a(1));
+  HIP_TRY(hipMemcpyAsync(h_counters, J.counters, 64, hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(h_flags, J.error_flags, 4, hipMemcpyDeviceToHost, stream_));
+  DICE_TRY(wait_stream(stream_));
+  DICE_TRY(flag_error(*h_flags));
+  if (R.cb.cancelled()) return cancelled_error();
+
+  // the site arrays, now that the site count is known
+  const size_t count = h_counters[0];
+  const size_t groups = count / kStateGroup + 2;
+  J.sites = A.take<Site>(count + 1);
+  J.writer_records = A.take<WriterRecord>(count + 1);
+  J.su_site_state = A.take<uint32_t>(su + count + 1 + groups);
+  J.site_state = J.su_site_state + su;
+  J.group_state = J.site_state + count + 1;
+  if (!A.fits())
+    return set_error(DICE_OVER_BUDGET,
+                     "a %lldx%lld job with %zu attempted sites needs %zu bytes of scratch; the "
+                     "allocation-free entry point holds %zu",
+                     (long long)H, (long long)W, count, A.used, arena_bytes);
+  HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev), hipMemcpyHostToDevice,
+                         stream_));
+
+  // phase B: sites, hidden startup, draw schedules; then the serial chain on
+  // one device wavefront, band by band, each band followed by its writer
+  DICE_TRY(launch_b(1));
+  HIP_TRY(hipMemcpyAsync(h_row_start, J.row_start, ((size_t)H + 1) * 4, hipMemcpyDeviceToHost,
+                         stream_));
+  memset(h_chain, 0, sizeof(ChainState));
+  h_chain->state = R.rng_state;
+  HIP_TRY(hipMemcpyAsync(J.chain, h_chain, sizeof(ChainState), hipMemcpyHostToDevice, stream_));
+  DICE_TRY(wait_stream(stream_));
+  HIP_TRY(launch_writer_startup_chain(d, 1, stream_));
+  HIP_TRY(launch_writer_startup_check(J, 0, &J.chain->startup_final, stream_));
+  const int bands = std::max<int>(1, std::min<int64_t>(std::min(8, kMaxBands), H));
+  for (int b = 0; b < bands; ++b) {
+    HIP_TRY(launch_writer_chain(d, 1, b, bands, stream_));
+    const uint32_t s0 = h_row_start[band_row(H, bands, b)];
+    const uint32_t s1 = h_row_start[band_row(H, bands, b + 1)];
+    HIP_TRY(launch_writer_values(J, s0, s1, 0, J.band_end + b, stream_));
+  }
+  HIP_TRY(hipMemcpyAsync(h_chain, J.chain, sizeof(ChainState), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(io + kArenaSlotsAt, J.writer_counts,
+                         (size_t)kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(h_flags, J.error_flags, 4, hipMemcpyDeviceToHost, stream_));
+  DICE_TRY(wait_stream(stream_));
+  DICE_TRY(flag_error(*h_flags));
+  if (R.cb.cancelled()) return cancelled_error();
+  uint64_t written = 0, changed = 0;
+  for (int slot = 0; slot < kCountSlots; ++slot) {
+    written += h_slots[slot * kCountStride + 0];
+    changed += h_slots[slot * kCountStride + 1];
+  }
+  dice_replay_result& res = *R.result;
+  res.attempted_pixels = h_row_start[H];
+  res.written_pixels = written;
+  res.changed_pixels = changed;
+  res.public_rng_advances = h_chain->advances;
+  res.final_rng_state = h_chain->state;
+  for (int stage = 0; stage < 6; ++stage) {
+    res.startup_attempted[stage] = h_chain->startup_attempted[stage];
+    res.startup_advances[stage] = h_chain->startup_advances[stage];
+  }
+  res.startup_final_rng_state = h_chain->startup_final;
   return DICE_OK;
 }
 

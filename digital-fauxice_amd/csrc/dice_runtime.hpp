@@ -91,6 +91,16 @@ class Runner {
   // packs the payloads).
   int32_t producer_schedule(const uint16_t* rgbi, int64_t height, int64_t width,
                             const dice_params& params, dice_schedule* out);
+  // One job whose device intermediates are all carved from `arena`
+  // (`arena_bytes` long, device memory the caller keeps) instead of this
+  // runner's workspaces.  The main frame and the output are device memory
+  // (R.main, R.out); the writer chain runs on one device wavefront; no
+  // SHA-256 and no diagnostics.  Allocates nothing: the runner's own
+  // buffers come from init().  Frames of more than kArenaMaxRows rows, or
+  // whose intermediates do not fit the arena, are refused with
+  // DICE_OVER_BUDGET before the device phase that would need the space.
+  int32_t run_in_arena(JobRequest& R, uint8_t* arena, size_t arena_bytes);
+  static constexpr int64_t kArenaMaxRows = 16384;
   void set_tile(int rows, int cols) {
     tile_rows_ = rows;
     tile_cols_ = cols;
@@ -133,6 +143,7 @@ class Runner {
   DeviceBuffer d_jobs_;
   DeviceBuffer queue_;  // ticket counter of the dynamically scheduled launches (zero at rest)
   PinnedBuffer h_jobs_;
+  PinnedBuffer h_io_;  // run_in_arena's host side: row starts, chain state, counts, flags
   Graph graph_a_, graph_b_;
   std::map<std::string, std::pair<int, int>> tuned_;
   DeviceBuffer sched_[12];

@@ -20,6 +20,7 @@ DICE_CANCELLED = 5
 DICE_OVER_BUDGET = 6
 DICE_INTERNAL = 7
 DICE_PENDING = 8
+DICE_NOALLOC_SCRATCH_BYTES = 512 << 20
 
 DICE_MAIN_ON_DEVICE = 0x1
 DICE_OUT_ON_DEVICE = 0x2
@@ -358,6 +359,11 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
          C.POINTER(DicePrepassParams), C.c_uint32, VOIDP,
          C.POINTER(DiceReplayResult), C.POINTER(DiceDiagnostics),
          C.POINTER(DiceCallbacks), C.POINTER(DiceStageTimings)],
+    ),
+    "process_job_noalloc": (
+        C.c_int32,
+        [VOIDP, VOIDP, i64, i64, u16p, i64, i64, C.POINTER(DiceParams),
+         C.POINTER(DicePrepassParams), VOIDP, C.POINTER(DiceReplayResult)],
     ),
     "process_batch": (
         C.c_int32,

@@ -358,6 +358,33 @@ int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi,
                          const dice_callbacks* callbacks,
                          dice_stage_timings* timings);
 
+/* The allocation-free variant of dice_process_job.  It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The job is dice_process_job's with flags DICE_MAIN_ON_DEVICE |
+ * DICE_OUT_ON_DEVICE | DICE_SKIP_SHA256 (identical output and counters,
+ * output_sha256 left empty), except that:
+ *   * every device intermediate lives in one file-scope device array of
+ *     DICE_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's load
+ *     until the process ends, and the writer chain runs on one device
+ *     wavefront, so the call allocates nothing (no hipMalloc, no pinned or
+ *     heap memory);
+ *   * a job of more than 16384 rows, or whose intermediates (about 3 B per
+ *     pixel plus 68 B per attempted site) do not fit the array, is refused
+ *     with DICE_OVER_BUDGET;
+ *   * on failure the device output holds partial values.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from different contexts of that device) race and must not be made.
+ * main_rgbi and out_rgb16 are device pointers; the prepass is host memory. */
+#define DICE_NOALLOC_SCRATCH_BYTES (512ull << 20)
+int32_t dice_process_job_noalloc(dice_ctx* ctx, const uint16_t* main_rgbi,
+                                 int64_t height, int64_t width,
+                                 const uint16_t* prepass_rgbi,
+                                 int64_t prepass_height, int64_t prepass_width,
+                                 const dice_params* params,
+                                 const dice_prepass_params* prepass_params,
+                                 uint16_t* out_rgb16, dice_replay_result* result);
+
 /* Level-1 test entry points: the same device functions the replay kernels
  * use, launched over host-provided vectors. */
 int32_t dice_t_unscaled_averages(dice_ctx* ctx, const float* patches,
