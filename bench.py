@@ -85,6 +85,8 @@ SITE_FP64_OPS = 13 * (9 * 12 + 17) + 3 * 13 + 15 + 9 + 50
 # dispatch), each scaled by what it reads for the plain copy kernel k_copy
 # over 1 GiB (x2.000 and x1.000)
 SITES_TRAFFIC_BYTES = 389_500_000
+TRAFFIC_SOURCE = ("profiles/kernel_stats_full_frame_5782x3946.md, k_sites row: "
+                  "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per dispatch (not measured in this run)")
 
 
 def parse_args() -> argparse.Namespace:
@@ -100,8 +102,14 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--pipeline-rounds", type=int, default=8,
                         help="the pipelined leg runs steps x batch x this many frames")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
+    parser.add_argument("--no-end-to-end", action="store_true",
+                        help="skip the host-buffer (PCIe-inclusive) leg")
     parser.add_argument("--no-serving", action="store_true", help="skip the serving leg")
-    parser.add_argument("--serving-jobs", type=int, default=40)
+    parser.add_argument("--serving-jobs", type=int, default=200)
+    parser.add_argument("--serving-drivers", type=int, default=8,
+                        help="drivers of the serving leg's engine")
+    parser.add_argument("--serving-batch", type=int, default=2,
+                        help="largest batch of the serving leg's engine")
     parser.add_argument("--no-routing", action="store_true",
                         help="skip the at-floor routing leg")
     parser.add_argument("--no-stream", action="store_true",
@@ -277,16 +285,72 @@ def main() -> int:
     start = time.perf_counter()
     for _ in range(args.steps):
         a = time.perf_counter()
-        prints, report = run_batch(full, batch_flags)
+        prints, batch_report = run_batch(full, batch_flags)
         batch_lat.append((time.perf_counter() - a) * 1e3)
         batch_results.append(prints)
         batch_timings.extend(t.as_dict() for t in batch_jobs.timings)
     synchronize()
     batch_elapsed = time.perf_counter() - start
     log(f"batched (one call per batch): {args.steps} batches of {args.batch} in "
-        f"{batch_elapsed:.3f} s (tile {report.tile_rows}x{report.tile_cols}, "
-        f"graph {report.graph_replayed})")
+        f"{batch_elapsed:.3f} s (tile {batch_report.tile_rows}x{batch_report.tile_cols}, "
+        f"graph {batch_report.graph_replayed})")
     single_call_value = args.steps * sum(full.mpix) / batch_elapsed
+
+    # ---- end to end: host numpy frames in, host numpy outputs back (PCIe
+    # and the pageable-memory staging included), as process_hip callers see
+    end_to_end = None
+    if not args.no_end_to_end:
+        host_outs = [np.empty((height, width, 3), np.uint16) for _ in full.specs]
+
+        def host_job(i: int) -> tuple:
+            res = DiceReplayResult()
+            code = lib.dice_process_job(ctx, C.c_void_p(full.mains[i].ctypes.data), height,
+                                        width, ptr(full.prepasses[i], C.c_uint16),
+                                        full.prepasses[i].shape[0], full.prepasses[i].shape[1],
+                                        C.byref(params), C.byref(prepass_params), 0,
+                                        C.c_void_p(host_outs[i].ctypes.data), C.byref(res),
+                                        None, None, None)
+            engine.raise_for(code)
+            return fingerprint(res)
+
+        def host_batch() -> list:
+            array, results = batch_jobs(full)
+            for i in range(len(full.specs)):
+                array[i].main_rgbi = full.mains[i].ctypes.data
+                array[i].out_rgb16 = host_outs[i].ctypes.data
+                array[i].flags = 0
+            code = lib.dice_process_batch(ctx, array, len(full.specs), batch_flags, None)
+            engine.raise_for(code)
+            return [fingerprint(r) for r in results]
+
+        host_job(0)
+        host_batch()
+        e2e_plain, e2e_prints = [], []
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            e2e_prints.append(host_job(0))
+            e2e_plain.append((time.perf_counter() - a) * 1e3)
+        e2e_batch = []
+        for _ in range(max(2, args.steps // 2)):
+            a = time.perf_counter()
+            batch_prints = host_batch()
+            e2e_batch.append((time.perf_counter() - a) * 1e3)
+        plain_p50 = float(np.percentile(e2e_plain, 50))
+        batch_p50 = float(np.percentile(e2e_batch, 50))
+        end_to_end = {
+            "note": "host numpy RGBI in and RGB16 out per call (PCIe and pageable staging "
+                    "included); never the headline value",
+            "plain": {"value": round(frame_mpix / plain_p50 * 1e3, 2), "unit": "Mpix/s",
+                      "p50_ms": round(plain_p50, 3),
+                      "p99_ms": round(float(np.percentile(e2e_plain, 99)), 3)},
+            "batch": {"jobs": len(full.specs),
+                      "value": round(sum(full.mpix) / batch_p50 * 1e3, 2), "unit": "Mpix/s",
+                      "p50_ms": round(batch_p50, 3)},
+            "identical_to_resident": (all(p == plain_results[0] for p in e2e_prints)
+                                      and batch_prints == batch_results[-1]),
+        }
+        log(f"end to end: {end_to_end}")
+        del host_outs
 
     # ---- pipelined batches (the headline): the same frames through the
     # serving engine, batches of B formed by `drivers` concurrent drivers, so
@@ -395,8 +459,8 @@ def main() -> int:
             plain_job_lat.append((time.perf_counter() - a) * 1e3)
             plain_prints.append(fingerprint(res))
         plain_mix_elapsed = time.perf_counter() - a0
-        with HipServingEngine(drivers=args.drivers, max_batch=8, graph=False,
-                              device_chain=args.device_chain) as served:
+        with HipServingEngine(drivers=args.serving_drivers, max_batch=args.serving_batch,
+                              graph=False, device_chain=args.device_chain) as served:
             def closed_loop():
                 a = time.perf_counter()
                 ts = []
@@ -442,8 +506,13 @@ def main() -> int:
             "latency_p99_ms": round(stats["latency_p99_ms"], 3),
             "queue_p50_ms": round(stats["queue_p50_ms"], 3),
             "mean_batch": round(stats["mean_batch"], 2),
+            "drivers": args.serving_drivers,
+            "max_batch": args.serving_batch,
+            "hbm_budget": int(stats["hbm_budget"]),
             "admitted_bytes_peak": int(stats["admitted_bytes_peak"]),
             "workspace_bytes": int(stats["workspace_bytes"]),
+            "workspace_bytes_peak": int(stats["workspace_bytes_peak"]),
+            "capacity_vs_plain": round(capacity / (sum(job_mpix) / plain_mix_elapsed), 3),
             "plain_one_at_a_time": {
                 "value": round(sum(job_mpix) / plain_mix_elapsed, 2),
                 "unit": "Mpix/s",
@@ -549,10 +618,10 @@ def main() -> int:
         chain_prints, chain_ms = [], []
         for _ in range(3):
             array, results = batch_jobs(full)  # the first job of the array: frame 0
-            report = DiceBatchReport()
+            chain_report = DiceBatchReport()
             a = time.perf_counter()
             code = lib.dice_process_batch(ctx, array, 1, _abi.DICE_BATCH_DEVICE_CHAIN,
-                                          C.byref(report))
+                                          C.byref(chain_report))
             chain_ms.append((time.perf_counter() - a) * 1e3)
             engine.raise_for(code)
             chain_prints.append(fingerprint(results[0]))
@@ -713,8 +782,8 @@ def main() -> int:
         "unit": "Mpix/s",
         "batch_p50_ms": round(float(np.percentile(batch_lat, 50)), 3),
         "batch_p99_ms": round(float(np.percentile(batch_lat, 99)), 3),
-        "tile": [report.tile_rows, report.tile_cols],
-        "graph_replayed": bool(report.graph_replayed),
+        "tile": [batch_report.tile_rows, batch_report.tile_cols],
+        "graph_replayed": bool(batch_report.graph_replayed),
     }
     line = {
         "metric": METRIC,
@@ -730,11 +799,13 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{args.steps} x {args.batch} full 5782x3946 RGBI16 frames (400 disks + "
-                         "40 scratches each), resident in HBM, through the batching engine: "
-                         f"batches of {args.batch}, {args.drivers} concurrent drivers, persistent "
-                         "launches over each batch, graph replay, autotuned tiles; SHA-256 of "
-                         "every output"),
+            "workload": (f"{jobs_total} full 5782x3946 RGBI16 frames (steps {args.steps} x batch "
+                         f"{args.batch} x {max(1, args.pipeline_rounds)} rounds; {args.batch} "
+                         "distinct frames with 400 disks + 40 scratches each), resident in HBM, "
+                         f"through the batching engine: batches of {args.batch}, {args.drivers} "
+                         "concurrent drivers, persistent launches over each batch, graph replay, "
+                         "autotuned tiles; SHA-256 of every output"),
+            "frames_timed": jobs_total,
             "drivers": args.drivers,
             "writer_chain": "device wavefront" if args.device_chain else "host core",
             "global_batch": args.batch,
@@ -746,8 +817,8 @@ def main() -> int:
         "pipelined": {
             "jobs": jobs_total,
             "mean_batch": round(pipe_stats["mean_batch"], 2),
-            "job_latency_p50_ms": round(pipe_stats["latency_p50_ms"], 3),
-            "job_latency_p99_ms": round(pipe_stats["latency_p99_ms"], 3),
+            "note": ("throughput leg: every job is submitted at once, so per-job times "
+                     "are queue positions, not latencies (see plain and serving)"),
             "host_cores_busy": round(pipe_cores, 2),
             "host_cores": len(os.sched_getaffinity(0)),
         },
@@ -759,6 +830,7 @@ def main() -> int:
             "p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
         },
         "batched_vs_plain": round(value / plain_value, 3),
+        "end_to_end": end_to_end,
         "crops": crops,
         "serving": serving,
         "routing": routing,
@@ -777,6 +849,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(sites_achieved / HBM_PEAK_GBS, 4),
             "traffic": SITES_TRAFFIC_BYTES,
+            "traffic_source": TRAFFIC_SOURCE,
             "kernel_ms": round(stage_ms["sites_kernel_ms"], 4),
             "algorithmic_bytes": sites_bytes,
             "fp64_tflops": round(sites_tflops, 3),

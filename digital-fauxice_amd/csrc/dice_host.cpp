@@ -323,7 +323,7 @@ namespace dice {
 // job's outcome lands in its dice_job.  Shared by dice_process_batch and
 // the serving engine.
 int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t batch_flags,
-                     dice_batch_report* report) {
+                     dice_batch_report* report, const std::function<void(int)>& on_done) {
   if (report) memset(report, 0, sizeof *report);
   if (count < 1 || count > kMaxBatch)
     return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
@@ -363,16 +363,24 @@ int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t bat
   RunOptions options;
   options.graph = (batch_flags & DICE_BATCH_GRAPH) != 0;
   options.device_chain = (batch_flags & DICE_BATCH_DEVICE_CHAIN) != 0;
-  DICE_TRY(runner.run(requests.data(), count, options, &rr));
-  int32_t first_error = DICE_OK;
-  for (int32_t j = 0; j < count; ++j) {
+  // a job's dice_job is completed once, as soon as its outcome is final
+  std::vector<char> completed(count, 0);
+  auto complete = [&](int j) {
     dice_job& job = jobs[j];
     job.status = requests[j].code;
     snprintf(job.error, sizeof job.error, "%s", requests[j].error.c_str());
     if (job.timings) job.timings->total = now_seconds() - start;
-    if (job.status != DICE_OK && first_error == DICE_OK) {
-      first_error = job.status;
-      set_error(job.status, "batch job %d: %s", j, job.error);
+    completed[j] = 1;
+    if (on_done) on_done(j);
+  };
+  if (on_done) options.on_job_done = complete;
+  DICE_TRY(runner.run(requests.data(), count, options, &rr));
+  int32_t first_error = DICE_OK;
+  for (int32_t j = 0; j < count; ++j) {
+    if (!completed[j]) complete(j);
+    if (requests[j].code != DICE_OK && first_error == DICE_OK) {
+      first_error = requests[j].code;
+      set_error(first_error, "batch job %d: %s", j, requests[j].error.c_str());
     }
   }
   if (report) {

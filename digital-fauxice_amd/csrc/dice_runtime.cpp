@@ -149,6 +149,51 @@ uint64_t Runner::workspace_bytes() const {
   return total;
 }
 
+uint64_t job_reservation(int64_t height, int64_t width, uint32_t flags, bool diagnostics) {
+  const uint64_t H = (uint64_t)std::max<int64_t>(height, 1);
+  const uint64_t W = (uint64_t)std::max<int64_t>(width, 1);
+  const uint64_t pixels = H * W;
+  // DeviceBuffer::ensure: the need plus a quarter, rounded up to 1 MiB
+  auto grown = [](uint64_t need) {
+    need = std::max<uint64_t>(need, 1);
+    return (uint64_t)((need + need / 4 + 0xFFFFFull) & ~0xFFFFFull);
+  };
+  uint64_t tiled = 0;  // the buffers whose size follows the tile shape: the largest shape
+  int shapes[16][2];
+  const int count = std::min(16, tile_shapes(shapes, 16));
+  for (int k = 0; k < count; ++k) {
+    const uint64_t TY = (uint64_t)shapes[k][0], TX = (uint64_t)shapes[k][1];
+    const uint64_t tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
+    const uint64_t tiles = tiles_x * tiles_y, segments = tiles_y * TY * tiles_x;
+    tiled = std::max(tiled, grown(tiles * TY * TX * 2) + grown(tiles * 4) +
+                                2 * grown(segments * 4) + grown((segments / 1024 + 2) * 4));
+  }
+  const uint64_t stages = H + 6, su = 6 * W, sites = pixels + 1;
+  uint64_t bytes = tiled;
+  for (uint64_t need : {(H + 1) * 4, (uint64_t)64, (uint64_t)64,
+                        (uint64_t)kCountSlots * kCountStride * 8, 5 * W * 4, 5 * W * 4,
+                        5 * W * 12, su * sizeof(Site), su, ((H + 7) / 8) * (W / 8) + 8, H * 24,
+                        H * 4, (H / 8) * 16 + 16, H * 24, (H / 8) * 16 + 16, (H / 8) * 8 + 8,
+                        su * sizeof(WriterRecord), sizeof(ChainState) + kMaxBands * 4,
+                        sites * sizeof(Site), sites * sizeof(WriterRecord),
+                        (su + sites + pixels / kStateGroup + 2) * 4})
+    bytes += grown(need);
+  bytes += 6 * grown(stages * 4) + 2 * grown(stages);
+  if (diagnostics) bytes += grown(pixels * 4) + 2 * grown(pixels);
+  if (!(flags & DICE_MAIN_ON_DEVICE)) bytes += grown(pixels * 8);
+  if (!(flags & DICE_OUT_ON_DEVICE)) bytes += grown(pixels * 6);
+  return bytes;
+}
+
+int32_t Runner::release_workspaces() {
+  HIP_TRY(hipSetDevice(device_));
+  DICE_TRY(wait_stream(stream_));
+  DICE_TRY(wait_stream(fetch_stream_));
+  DICE_TRY(wait_stream(out_stream_));
+  ws_.clear();
+  return DICE_OK;
+}
+
 // Sizes every job's workspace, places its input, stage table and output,
 // and uploads the descriptors.
 int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
@@ -624,6 +669,105 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   };
   std::vector<double> chain_seconds(n, 0.0), hash_seconds(n, 0.0), copy_seconds(n, 0.0);
   std::vector<double> chain_waits(n, 0.0), band_waits(n, 0.0);
+
+  // Per job, once its threads are done: the device error flags, the final
+  // cancellation check, counts, diagnostics, timings and the output commit.
+  // A job without callbacks is finalized by the last of its own threads
+  // (so with a completion hook it completes without waiting for the rest
+  // of the batch); every other job on the calling thread after the batch.
+  std::unique_ptr<std::atomic<int>[]> live(new std::atomic<int>[n]);
+  std::vector<char> early(n, 0), finalized(n, 0);
+  std::vector<int> commit;  // calling-thread jobs whose host output is copied in parallel
+  auto finalize = [&](int j, bool on_caller) {
+    JobRequest& R = jobs[j];
+    const JobDev& J = H_jobs[j];
+    Workspace& w = *ws_[j];
+    const double t_fin = now_seconds();
+    auto body = [&]() -> int32_t {
+      if (outcome[j].code != DICE_OK) return set_error(outcome[j].code, "%s",
+                                                      outcome[j].error.c_str());
+      HIP_TRY(hipSetDevice(device_));
+      if (!on_caller) HIP_TRY(hipEventSynchronize(w.tail));
+      const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
+      DICE_TRY(flag_error(flags));
+      if (cancel[j].load() || R.cb.cancelled()) return cancelled_error();
+      const unsigned long long* slots =
+          reinterpret_cast<const unsigned long long*>(w.h_small.as<uint8_t>() + 128);
+      uint64_t written = 0, changed = 0;
+      for (int slot = 0; slot < kCountSlots; ++slot) {
+        written += slots[slot * kCountStride + 0];
+        changed += slots[slot * kCountStride + 1];
+      }
+      R.result->written_pixels = written;
+      R.result->changed_pixels = changed;
+      if (options.device_chain) {
+        const ChainState& chain = *w.h_chain.as<ChainState>();
+        for (int stage = 0; stage < 6; ++stage) {
+          R.result->startup_attempted[stage] = chain.startup_attempted[stage];
+          R.result->startup_advances[stage] = chain.startup_advances[stage];
+        }
+        R.result->startup_final_rng_state = chain.startup_final;
+        R.result->attempted_pixels = w.h_row_start.as<uint32_t>()[R.height];
+        R.result->public_rng_advances = chain.advances;
+        R.result->final_rng_state = chain.state;
+      }
+      const size_t pixels = (size_t)R.height * R.width;
+      if (R.diag) {
+        if (R.diag->score_plane)
+          HIP_TRY(hipMemcpy(R.diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
+        if (R.diag->at_floor_mask)
+          HIP_TRY(hipMemcpy(R.diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
+        if (R.diag->changed_mask)
+          HIP_TRY(hipMemcpy(R.diag->changed_mask, J.diag_changed, pixels,
+                            hipMemcpyDeviceToHost));
+      }
+      R.cb.progress(3, R.height, R.height, (int64_t)R.result->attempted_pixels,
+                    (int64_t)written);
+      if (R.timings) {
+        dice_stage_timings& T = *R.timings;
+        T.producer_device += t_a - t_start;
+        T.analysis_device += t_b - t_a;
+        T.writer_host += chain_seconds[j];
+        T.download_sites += chain_waits[j];  // chain waiting for its draw schedules
+        T.upload += band_waits[j];           // hash waiting for the chain
+        T.download_output += copy_seconds[j];
+        T.sha256_host += hash_seconds[j];
+        T.finish_device += now_seconds() - t_fin;
+        if (fine_on_) {
+          T.producer_kernels_ms += fine_ms(0, 1);
+          T.scan_kernel_ms += fine_ms(1, 2);
+          T.segments_kernel_ms += fine_ms(2, 3);
+          T.sites_kernel_ms += fine_ms(4, 5);
+          T.startup_kernels_ms += fine_ms(5, 6);
+          T.writer_prep_kernel_ms += fine_ms(6, 7);
+        } else {
+          T.producer_kernels_ms += ms_a;
+          T.sites_kernel_ms += ms_b;
+        }
+      }
+      // The caller's host buffer is written only now, after every check
+      // of its job passed (engine.py:256-260 writes the caller's output
+      // once, on success).
+      if ((R.flags & DICE_OUT_ON_DEVICE) == 0) {
+        if (on_caller)
+          commit.push_back(j);
+        else
+          memcpy(R.out, w.h_out.ptr, pixels * 6);
+      }
+      return DICE_OK;
+    };
+    if (int32_t rc = body()) {
+      R.code = rc;
+      R.error = error_buffer();
+    }
+    finalized[j] = 1;
+  };
+  auto job_exit = [&](int j) {
+    if (live[j].fetch_sub(1) == 1 && early[j]) {
+      finalize(j, false);
+      options.on_job_done(j);
+    }
+  };
   // The device writer chain (opt-in): after phase B, one wavefront per job
   // walks its chain, one launch per band for the whole batch, on the fetch
   // stream (it fetches nothing in this mode), each band followed by its
@@ -677,11 +821,20 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
                                kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, cs));
         HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
                                hipMemcpyDeviceToHost, cs));
+        HIP_TRY(hipEventRecord(w.tail, cs));
       }
       return DICE_OK;
     };
     device_chain_rc = enqueue();  // a failure fails every job; no hash waits for it
     device_chain_seconds = now_seconds() - t0;
+  }
+  for (int j = 0; j < n; ++j) {
+    live[j].store(0);
+    if (H_jobs[j].skip) continue;
+    const bool hashes = (jobs[j].flags & DICE_SKIP_SHA256) == 0 ||
+                        (jobs[j].flags & DICE_OUT_ON_DEVICE) == 0;
+    live[j].store((options.device_chain ? 0 : 1) + (hashes ? 1 : 0));
+    early[j] = options.on_job_done && !jobs[j].cb.cb && live[j].load() > 0;
   }
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
@@ -710,7 +863,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     // chain thread
     spawn([this, j, bands, jobs, H_jobs, &outcome, &chain_seconds,
                           &chain_waits, &values_queued, &chain_failed, &fetch_band,
-                          &cancel]() {
+                          &cancel, &job_exit]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const JobDev& J = H_jobs[j];
@@ -767,6 +920,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
                                kCountSlots * kCountStride * 8, hipMemcpyDeviceToHost, stream_));
         HIP_TRY(hipMemcpyAsync(w.h_small.as<uint8_t>() + 64, J.error_flags, 4,
                                hipMemcpyDeviceToHost, stream_));
+        HIP_TRY(hipEventRecord(w.tail, stream_));
         return DICE_OK;
       };
       if (int32_t rc = body()) {
@@ -783,16 +937,17 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       if (trace_enabled())
         fprintf(stderr, "dice-trace chain job=%d total=%.2fms fetch_wait=%.2fms walk=%.2fms\n", j,
                 chain_seconds[j] * 1e3, fetch_wait * 1e3, walking * 1e3);
+      job_exit(j);
     });
     }
     // hash / download thread
+    if ((jobs[j].flags & DICE_SKIP_SHA256) != 0 && (jobs[j].flags & DICE_OUT_ON_DEVICE) != 0)
+      continue;
     spawn([this, j, bands, jobs, &outcome, &hash_seconds, &copy_seconds,
-                          &band_waits, &cancel]() {
+                          &band_waits, &cancel, &job_exit]() {
       Workspace& w = *ws_[j];
       JobRequest& R = jobs[j];
       const bool want_sha = (R.flags & DICE_SKIP_SHA256) == 0;
-      const bool host_out = (R.flags & DICE_OUT_ON_DEVICE) == 0;
-      if (!want_sha && !host_out) return;
       double hashing = 0.0, copying = 0.0;
       auto body = [&]() -> int32_t {
         HIP_TRY(hipSetDevice(device_));
@@ -833,6 +988,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       if (trace_enabled())
         fprintf(stderr, "dice-trace hash job=%d band_wait=%.2fms copy_wait=%.2fms hash=%.2fms\n",
                 j, band_waits[j] * 1e3, copying * 1e3, hashing * 1e3);
+      job_exit(j);
     });
   }
   // the copy scheduler: the lowest band ready across all jobs goes next
@@ -921,95 +1077,15 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   }
   for (std::thread& t : threads) t.join();
   for (int j = 0; j < n; ++j)
-    if (!H_jobs[j].skip && outcome[j].code == DICE_OK) report_bands(j);
+    if (!H_jobs[j].skip && outcome[j].code == DICE_OK && !finalized[j]) report_bands(j);
   DICE_TRY(wait_stream(stream_));
   DICE_TRY(wait_stream(out_stream_));
-  for (int j = 0; j < n; ++j) {
-    if (H_jobs[j].skip) continue;
-  }
   DICE_TRY(wait_stream(fetch_stream_));
   const double t_host = now_seconds();
-
-  // ---- counts, checks, diagnostics, results
-  std::vector<int> commit;  // jobs whose host output is written once every check passed
-  for (int j = 0; j < n; ++j) {
-    JobRequest& R = jobs[j];
-    const JobDev& J = H_jobs[j];
-    if (J.skip) continue;
-    Workspace& w = *ws_[j];
-    if (outcome[j].code != DICE_OK) {
-      R.code = outcome[j].code;
-      R.error = outcome[j].error;
-      continue;
-    }
-    const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
-    if (int32_t rc = flag_error(flags)) {
-      R.code = rc;
-      R.error = error_buffer();
-      continue;
-    }
-    if (cancel[j].load() || R.cb.cancelled()) {
-      R.code = cancelled_error();
-      R.error = error_buffer();
-      continue;
-    }
-    const unsigned long long* slots =
-        reinterpret_cast<const unsigned long long*>(w.h_small.as<uint8_t>() + 128);
-    uint64_t written = 0, changed = 0;
-    for (int slot = 0; slot < kCountSlots; ++slot) {
-      written += slots[slot * kCountStride + 0];
-      changed += slots[slot * kCountStride + 1];
-    }
-    R.result->written_pixels = written;
-    R.result->changed_pixels = changed;
-    if (options.device_chain) {
-      const ChainState& chain = *w.h_chain.as<ChainState>();
-      for (int stage = 0; stage < 6; ++stage) {
-        R.result->startup_attempted[stage] = chain.startup_attempted[stage];
-        R.result->startup_advances[stage] = chain.startup_advances[stage];
-      }
-      R.result->startup_final_rng_state = chain.startup_final;
-      R.result->attempted_pixels = w.h_row_start.as<uint32_t>()[R.height];
-      R.result->public_rng_advances = chain.advances;
-      R.result->final_rng_state = chain.state;
-    }
-    const size_t pixels = (size_t)R.height * R.width;
-    if ((R.flags & DICE_OUT_ON_DEVICE) == 0) commit.push_back(j);
-    if (R.diag) {
-      if (R.diag->score_plane)
-        HIP_TRY(hipMemcpy(R.diag->score_plane, J.diag_score, pixels * 4, hipMemcpyDeviceToHost));
-      if (R.diag->at_floor_mask)
-        HIP_TRY(hipMemcpy(R.diag->at_floor_mask, J.diag_floor, pixels, hipMemcpyDeviceToHost));
-      if (R.diag->changed_mask)
-        HIP_TRY(hipMemcpy(R.diag->changed_mask, J.diag_changed, pixels, hipMemcpyDeviceToHost));
-    }
-    R.cb.progress(3, R.height, R.height, (int64_t)R.result->attempted_pixels, (int64_t)written);
-    if (R.timings) {
-      dice_stage_timings& T = *R.timings;
-      T.producer_device += t_a - t_start;
-      T.analysis_device += t_b - t_a;
-      T.writer_host += chain_seconds[j];
-      T.download_sites += chain_waits[j];  // chain waiting for its draw schedules
-      T.upload += band_waits[j];           // hash waiting for the chain
-      T.download_output += copy_seconds[j];
-      T.sha256_host += hash_seconds[j];
-      T.finish_device += now_seconds() - t_host;
-      if (fine_on_) {
-        T.producer_kernels_ms += fine_ms(0, 1);
-        T.scan_kernel_ms += fine_ms(1, 2);
-        T.segments_kernel_ms += fine_ms(2, 3);
-        T.sites_kernel_ms += fine_ms(4, 5);
-        T.startup_kernels_ms += fine_ms(5, 6);
-        T.writer_prep_kernel_ms += fine_ms(6, 7);
-      } else {
-        T.producer_kernels_ms += ms_a;
-        T.sites_kernel_ms += ms_b;
-      }
-    }
-  }
-  // The caller's host buffers are written only now, after every check of
-  // their job passed (engine.py:256-260 writes the caller's output once,
-  // on success): pinned staging -> caller, in parallel pieces.
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip && !finalized[j]) finalize(j, true);
+  // Host outputs of the jobs finalized here: pinned staging -> caller, in
+  // parallel pieces.
   if (!commit.empty()) {
     struct Piece {
       uint8_t* dst;
@@ -1036,6 +1112,9 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     copier();
     for (std::thread& t : copiers) t.join();
   }
+  if (options.on_job_done)
+    for (int j = 0; j < n; ++j)
+      if (!early[j] || !finalized[j]) options.on_job_done(j);
   if (trace_enabled()) {
     const double t_end = now_seconds();
     fprintf(stderr,

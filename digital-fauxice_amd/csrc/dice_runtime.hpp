@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -58,6 +59,10 @@ struct RunOptions {
   bool fine_timing = false;  // HIP events between kernel groups (direct launches only)
   bool device_chain = false; // the serial writer chain on one device wavefront per job
   int bands = 8;             // writer/hash pipeline bands per job
+  // Called once per job as soon as that job's outcome (jobs[j].code, its
+  // result and output) is final; from a worker thread for a job without
+  // callbacks, else from the calling thread before run() returns.
+  std::function<void(int)> on_job_done;
 };
 
 struct RunReport {
@@ -109,6 +114,8 @@ class Runner {
   int tile_cols() const { return tile_cols_; }
   hipStream_t stream() const { return stream_; }
   uint64_t workspace_bytes() const;
+  // Frees every job workspace (the next batch grows them again).
+  int32_t release_workspaces();
 
  private:
   friend class FrameStream;
@@ -151,7 +158,17 @@ class Runner {
 
 // Validation, prepass and one batch run of caller-described jobs (the body
 // of dice_process_batch, also used by the serving engine).
+// An upper bound of the device bytes a job can add to the workspace of the
+// Runner slot it runs in (every buffer grown with its headroom and 1 MiB
+// rounding, every pixel an attempted site, any compiled tile shape): the
+// unit of the serving engine's HBM admission.
+uint64_t job_reservation(int64_t height, int64_t width, uint32_t flags, bool diagnostics);
+
+// `on_done(j)`, when given, is called once per job as soon as jobs[j]'s
+// status, error and result are final (possibly from a worker thread, before
+// the batch ends); the job's dice_job is not touched after that.
 int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t batch_flags,
-                     dice_batch_report* report);
+                     dice_batch_report* report,
+                     const std::function<void(int)>& on_done = nullptr);
 
 }  // namespace dice

@@ -77,6 +77,7 @@ struct Workspace {
   hipEvent_t fetched[kMaxBands] = {};  // draw schedules of band b on the host (copy stream)
   hipEvent_t copied[kMaxBands] = {};   // output rows of band b on the host (output stream)
   hipEvent_t startup_fetched = nullptr;
+  hipEvent_t tail = nullptr;  // the job's last device work and result copies (runner stream)
   int bands = 1;
   bool events = false;
   // chain -> copy scheduler -> hash hand-off: `published` counts the bands
@@ -91,6 +92,7 @@ struct Workspace {
       for (int b = 0; b < kMaxBands; ++b)
         if (set[b]) (void)hipEventDestroy(set[b]);
     if (startup_fetched) (void)hipEventDestroy(startup_fetched);
+    if (tail) (void)hipEventDestroy(tail);
   }
   int32_t init_events() {
     if (events) return DICE_OK;
@@ -100,6 +102,7 @@ struct Workspace {
     for (hipEvent_t* set : {band, fetched, copied})
       for (int b = 0; b < kMaxBands; ++b) HIP_TRY(hipEventCreateWithFlags(&set[b], flags));
     HIP_TRY(hipEventCreateWithFlags(&startup_fetched, flags));
+    HIP_TRY(hipEventCreateWithFlags(&tail, flags));
     events = true;
     return DICE_OK;
   }

@@ -257,6 +257,8 @@ class DiceEngineStats(C.Structure):
         ("hbm_budget", C.c_uint64),
         ("workspace_bytes", C.c_uint64),
         ("elapsed_seconds", C.c_double),
+        ("workspace_bytes_peak", C.c_uint64),
+        ("workspace_releases", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -373,6 +375,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "engine_destroy": (None, [VOIDP]),
     "engine_submit": (C.c_int32, [VOIDP, C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
     "engine_wait": (C.c_int32, [VOIDP, C.c_uint64, C.c_double]),
+    "engine_reservation": (C.c_int32, [C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
     "engine_stats": (C.c_int32, [VOIDP, C.POINTER(DiceEngineStats)]),
     "engine_reset_stats": (C.c_int32, [VOIDP]),
     "t_unscaled_averages": (C.c_int32, [VOIDP, f32p, i64, f64p]),

@@ -105,6 +105,18 @@ class HipServingEngine:
                                                    (height, width))
         return ticket.value
 
+    def reservation(self, job: ProcessingJob) -> int:
+        """The HBM bytes ``job`` is admitted against (its workspace bound)."""
+
+        entry = DiceJob()
+        entry.height, entry.width = job.acquisition.main.pixels.shape[:2]
+        entry.flags = 0
+        value = C.c_uint64()
+        code = self._lib.dice_engine_reservation(C.byref(entry), C.byref(value))
+        if code != _abi.DICE_OK:
+            raise _exception_for(code, self._lib.dice_last_error().decode(errors="replace"))
+        return int(value.value)
+
     def wait(self, ticket: int, timeout: float | None = None):
         """The job's ``ProcessingResult`` (its output is None for device jobs)."""
 

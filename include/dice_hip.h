@@ -300,10 +300,11 @@ typedef struct dice_engine dice_engine;
 
 typedef struct dice_engine_config {
   int32_t device;
-  int32_t drivers;          /* concurrent batches, 1..8 (0: 2) */
+  int32_t drivers;          /* concurrent batches, 1..16 (0: 2) */
   int32_t max_batch;        /* jobs per batch, 1..64 (0: 8) */
   uint32_t batch_flags;     /* DICE_BATCH_GRAPH | DICE_BATCH_AUTOTUNE */
-  uint64_t hbm_budget;      /* bytes of job reservations admitted at once (0: 80% of free HBM) */
+  uint64_t hbm_budget;      /* HBM the drivers' job workspaces may hold (0: 80% of free HBM);
+                               jobs are admitted by their dice_engine_reservation */
   double linger_ms;         /* a partial batch waits this long for more jobs while another
                                batch is running (0: never waits) */
 } dice_engine_config;
@@ -315,15 +316,22 @@ typedef struct dice_engine_report {
   double mpix_completed;
   double latency_p50_ms, latency_p99_ms, latency_max_ms;  /* submit -> finished */
   double queue_p50_ms;            /* submit -> started */
-  uint64_t admitted_bytes, admitted_bytes_peak, hbm_budget;
-  uint64_t workspace_bytes;       /* HBM the drivers' workspaces hold */
+  uint64_t admitted_bytes, admitted_bytes_peak, hbm_budget;  /* the most the drivers'
+                                     workspaces can hold (now, peak), and its limit */
+  uint64_t workspace_bytes;       /* HBM the idle drivers' workspaces hold */
   double elapsed_seconds;         /* first submit -> last finish */
+  uint64_t workspace_bytes_peak;  /* the largest workspace_bytes seen after a batch */
+  uint64_t workspace_releases;    /* times an idle driver freed its workspaces for the queue */
 } dice_engine_report;
 
 int32_t dice_engine_create(const dice_engine_config* config, dice_engine** out);
 void dice_engine_destroy(dice_engine* engine);
 /* DICE_OVER_BUDGET when the job alone cannot fit the budget. */
 int32_t dice_engine_submit(dice_engine* engine, dice_job* job, uint64_t* ticket);
+/* The bytes of HBM a job is admitted against: an upper bound of what it can
+ * add to a driver's workspaces (every pixel an attempted site, buffer
+ * headroom and rounding included). */
+int32_t dice_engine_reservation(const dice_job* job, uint64_t* bytes);
 /* Returns the job's status (DICE_PENDING when timeout_seconds >= 0 runs out;
  * a negative timeout waits until the job is done).  A finished ticket is
  * forgotten once waited for. */

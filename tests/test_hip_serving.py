@@ -98,9 +98,13 @@ def test_engine_with_device_chain(frames, oracle_shas):
 
 
 def test_budget_admits_one_reservation_at_a_time(frames, oracle_shas):
-    largest = max(m.shape[0] * m.shape[1] for m, _ in frames)
-    # about one largest job's reservation: admission must queue the rest
-    budget = largest * 120 + (2 << 20)
+    """A budget of about one largest job's reservation: admission queues the
+    rest, idle drivers free their workspaces for the queue's head, and the
+    workspaces the engine holds never exceed the budget."""
+
+    with HipServingEngine(drivers=2, max_batch=8) as probe:
+        reservations = [probe.reservation(make_job(m, p, "size")) for m, p in frames]
+    budget = max(reservations) + min(reservations) // 2
     with HipServingEngine(drivers=2, max_batch=8, hbm_budget=budget) as engine:
         tickets = [(i, engine.submit(make_job(m, p, f"budget-{i}")))
                    for i, (m, p) in enumerate(frames)]
@@ -108,6 +112,54 @@ def test_budget_admits_one_reservation_at_a_time(frames, oracle_shas):
             assert engine.wait(t, timeout=120).replay.output_sha256 == oracle_shas[i]
         stats = engine.stats()
     assert 0 < stats["admitted_bytes_peak"] <= budget
+    # the engine's own figures: what idle drivers hold, and its largest value
+    # after any batch (a ticket completes before its batch's bookkeeping)
+    assert stats["workspace_bytes"] <= budget
+    assert 0 < stats["workspace_bytes_peak"] <= budget
+    assert stats["workspace_releases"] >= 1  # the second driver had to give its memory up
+
+
+def test_jobs_complete_on_their_own_within_a_batch(frames, oracle_shas):
+    """A small crop batched with a full frame completes before the frame's
+    hash is done: tickets complete per job, not per batch."""
+
+    from dice_amd import synth
+
+    big = synth.frame(synth.config2_full_frame(seed=31))
+    big_prepass = synth.prepass_for(big)
+    main, prepass = frames[6]  # 64 x 72
+    finished = {}
+    with HipServingEngine(drivers=2, max_batch=8, linger_ms=300.0) as engine:
+        warm = engine.submit(make_job(main, prepass, "warm"))
+        engine.wait(warm, timeout=120)
+        # one driver runs a blocker; the frame and the crop then queue up and
+        # go as one batch (the idle driver lingers for more jobs while a batch
+        # runs, and the first driver finds both when it is free)
+        blocker = engine.submit(make_job(big, big_prepass, "blocker"))
+        deadline = time.perf_counter() + 60
+        while engine.stats()["running"] == 0 and time.perf_counter() < deadline:
+            time.sleep(0.0005)
+        tickets = {"frame": engine.submit(make_job(big, big_prepass, "frame")),
+                   "crop": engine.submit(make_job(main, prepass, "crop"))}
+        engine.wait(blocker, timeout=120)
+        waiters = []
+        for name, ticket in tickets.items():
+            def wait(name=name, ticket=ticket):
+                result = engine.wait(ticket, timeout=120)
+                finished[name] = (time.perf_counter(), result)
+            waiters.append(threading.Thread(target=wait))
+        for w in waiters:
+            w.start()
+        for w in waiters:
+            w.join()
+        # the batch's own bookkeeping follows its last ticket
+        deadline = time.perf_counter() + 30
+        while engine.stats()["batches"] < 3 and time.perf_counter() < deadline:
+            time.sleep(0.001)
+        stats = engine.stats()
+    assert finished["crop"][1].replay.output_sha256 == oracle_shas[6]
+    assert finished["crop"][0] < finished["frame"][0]
+    assert stats["batches"] == 3 and stats["mean_batch"] > 1.0  # frame and crop together
 
 
 def test_job_larger_than_the_budget_is_refused(frames):
