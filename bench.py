@@ -419,6 +419,8 @@ def main() -> int:
             "unit": "Mpix/s",
             "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
             "tile": [crop_report.tile_rows, crop_report.tile_cols],
+            "sites_variant": crop_report.sites_variant,
+            "tune_source": crop_report.tune_source,
             "graph_replayed": bool(crop_report.graph_replayed),
         }
         log(f"crops: {crops}")
@@ -460,7 +462,8 @@ def main() -> int:
             plain_prints.append(fingerprint(res))
         plain_mix_elapsed = time.perf_counter() - a0
         with HipServingEngine(drivers=args.serving_drivers, max_batch=args.serving_batch,
-                              graph=False, device_chain=args.device_chain) as served:
+                              graph=False, autotune=True,
+                              device_chain=args.device_chain) as served:
             def closed_loop():
                 a = time.perf_counter()
                 ts = []
@@ -783,6 +786,8 @@ def main() -> int:
         "batch_p50_ms": round(float(np.percentile(batch_lat, 50)), 3),
         "batch_p99_ms": round(float(np.percentile(batch_lat, 99)), 3),
         "tile": [batch_report.tile_rows, batch_report.tile_cols],
+        "sites_variant": batch_report.sites_variant,
+        "tune_source": batch_report.tune_source,
         "graph_replayed": bool(batch_report.graph_replayed),
     }
     line = {

@@ -391,6 +391,10 @@ int32_t process_jobs(Runner& runner, dice_job* jobs, int32_t count, uint32_t bat
     report->phase_b_ms = rr.phase_b_ms;
     report->workspace_bytes = rr.workspace_bytes;
     report->wall_seconds = now_seconds() - start;
+    report->sites_variant = rr.sites_variant;
+    report->tune_source = rr.tune_source;
+    report->tune_candidates = rr.tune_candidates;
+    report->tune_rejected = rr.tune_rejected;
   }
   return first_error;
 }
@@ -704,6 +708,11 @@ int32_t dice_process_job(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t heigh
   if (rc == DICE_OK) cb.progress(4, 1, 1, 0, 0);
   if (timings) timings->total = now_seconds() - start;
   return rc;
+}
+
+int32_t dice_tuning_cache(const char* path) {
+  set_tuning_cache(path);
+  return DICE_OK;
 }
 
 int32_t dice_process_job_noalloc(dice_ctx* ctx, const uint16_t* main_rgbi, int64_t height,

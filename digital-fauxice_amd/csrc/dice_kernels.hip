@@ -888,8 +888,11 @@ constexpr int sites_occupancy() {
                                                          : 1;
 }
 
-template <int TY, int TX>
-__global__ __launch_bounds__(kThreads, (sites_occupancy<TY, TX>())) void k_sites(JobSet set) {
+// `Blocks` is the workgroups per CU the register allocation targets: the
+// LDS occupancy of the tile shape (variant 0), or one fewer (variant 1: a
+// register-rich build, for the tuner to weigh).
+template <int TY, int TX, int Blocks>
+__global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
   using Sh = SitesShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
@@ -1954,8 +1957,13 @@ void launch_scan_shape(const JobSet& set, hipStream_t stream) {
   launch_persistent(k_scan<TY, TX>, set, kThreads, stream);
 }
 template <int TY, int TX>
-void launch_sites_shape(const JobSet& set, hipStream_t stream) {
-  launch_persistent(k_sites<TY, TX>, set, kThreads, stream);
+void launch_sites_shape(const JobSet& set, int variant, hipStream_t stream) {
+  constexpr int kBlocks = sites_occupancy<TY, TX>();
+  constexpr int kFewer = kBlocks > 1 ? kBlocks - 1 : 1;
+  if (variant == 1)
+    launch_persistent(k_sites<TY, TX, kFewer>, set, kThreads, stream);
+  else
+    launch_persistent(k_sites<TY, TX, kBlocks>, set, kThreads, stream);
 }
 }  // namespace
 
@@ -2006,7 +2014,8 @@ hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int 
 }
 
 hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint32_t* queue,
-                        hipStream_t stream) {
+                        hipStream_t stream, int variant) {
+  if (variant < 0 || variant >= kSitesVariants) return hipErrorInvalidValue;
   int ty = 0, tx = 0;
   bool any = false;
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
@@ -2014,14 +2023,42 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint3
   JobSet set =
       make_set(d_jobs, h_jobs, n, [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
   set.queue = queue;
-#define DICE_MATCH(a, b)                  \
-  if (ty == a && tx == b) {               \
-    launch_sites_shape<a, b>(set, stream); \
-    return hipGetLastError();             \
+#define DICE_MATCH(a, b)                           \
+  if (ty == a && tx == b) {                        \
+    launch_sites_shape<a, b>(set, variant, stream); \
+    return hipGetLastError();                      \
   }
   DICE_TILE_SHAPES(DICE_MATCH)
 #undef DICE_MATCH
   return hipErrorInvalidValue;
+}
+
+namespace {
+// A position-sensitive, order-independent 64-bit digest of n 16-bit words:
+// the wrapping sum of a 64-bit mix of (index, word).
+__global__ __launch_bounds__(kThreads) void k_digest(const uint16_t* __restrict__ words, size_t n,
+                                                     unsigned long long* out) {
+  unsigned long long sum = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = ((unsigned long long)i << 16 | words[i]) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    sum += z ^ (z >> 31);
+  }
+  for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, sum);
+}
+}  // namespace
+
+hipError_t launch_digest(const void* data, size_t bytes, unsigned long long* out,
+                         hipStream_t stream) {
+  const size_t n = bytes / 2;
+  if (n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_digest, dim3(blocks), dim3(kThreads), 0, stream,
+                     static_cast<const uint16_t*>(data), n, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,

@@ -40,9 +40,15 @@ hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStr
 hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                 hipStream_t stream);
 // `queue`: two zeroed uint32 the launch uses as its ticket counter (and
-// leaves zeroed); one pair per stream.
+// leaves zeroed); one pair per stream.  `variant` selects the register
+// target of k_sites (0: the LDS occupancy of the tile shape, 1: one
+// workgroup per CU fewer); every variant computes the same sites.
+constexpr int kSitesVariants = 2;
 hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint32_t* queue,
-                        hipStream_t stream);
+                        hipStream_t stream, int variant = 0);
+// Adds a 64-bit digest of `bytes` (even) of device memory to *out (device).
+hipError_t launch_digest(const void* data, size_t bytes, unsigned long long* out,
+                         hipStream_t stream);
 hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                           hipStream_t stream);
 hipError_t launch_writer_prep(const JobDev* d_jobs, int n, hipStream_t stream);

@@ -18,6 +18,9 @@
 #include <mutex>
 #include <thread>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "dice_kernels.hpp"
 #include "dice_workspace.hpp"
 
@@ -140,6 +143,11 @@ int32_t Runner::init() {
   HIP_TRY(hipMemset(queue_.ptr, 0, 256));
   HIP_TRY(h_jobs_.ensure(sizeof(JobDev) * kMaxBatch));
   HIP_TRY(h_io_.ensure(kArenaIoBytes));
+  HIP_TRY(digest_.ensure(8 * kMaxBatch));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device_));
+  arch_ = prop.gcnArchName;
+  arch_ = arch_.substr(0, arch_.find(':'));
   return DICE_OK;
 }
 
@@ -377,7 +385,7 @@ int32_t Runner::launch_b(int n) {
   const JobDev* d = d_jobs_.as<JobDev>();
   const JobDev* h = h_jobs_.as<JobDev>();
   DICE_TRY(mark(4));
-  HIP_TRY(launch_sites(d, h, n, queue_.as<uint32_t>(), stream_));
+  HIP_TRY(launch_sites(d, h, n, queue_.as<uint32_t>(), stream_, sites_variant_));
   DICE_TRY(mark(5));
   HIP_TRY(launch_startup(d, h, n, stream_));
   DICE_TRY(mark(6));
@@ -396,7 +404,7 @@ std::string Runner::geometry_key(int n, char phase) const {
   const JobDev* h = h_jobs_.as<JobDev>();
   std::string key(1, phase);
   char part[96];
-  snprintf(part, sizeof part, "|%d|%dx%d", n, tile_rows_, tile_cols_);
+  snprintf(part, sizeof part, "|%d|%dx%d/%d", n, tile_rows_, tile_cols_, sites_variant_);
   key += part;
   for (int j = 0; j < n; ++j) {
     snprintf(part, sizeof part, "|%dx%d:%d%d%d%d", h[j].height, h[j].width, h[j].run_producer,
@@ -1126,6 +1134,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   if (report) {
     report->tile_rows = tile_rows_;
     report->tile_cols = tile_cols_;
+    report->sites_variant = sites_variant_;
     report->graph_replayed = replayed_a && replayed_b;
     report->phase_a_ms = ms_a;
     report->phase_b_ms = ms_b;
@@ -1344,77 +1353,256 @@ int32_t Runner::run_in_arena(JobRequest& R, uint8_t* arena, size_t arena_bytes) 
   return DICE_OK;
 }
 
+namespace {
+
+struct TuneEntry {
+  int rows = 16, cols = 64, variant = 0;
+  double ms = 0.0;
+  bool from_file = false;
+};
+
+struct TuningTable {
+  std::mutex mutex;
+  std::string path;
+  bool path_set = false, loaded = false;
+  std::map<std::string, TuneEntry> entries;  // "arch|bucket"
+};
+
+TuningTable& tuning() {
+  static TuningTable table;
+  return table;
+}
+
+std::string tuning_path_locked(TuningTable& t) {
+  if (!t.path_set) {
+    t.path_set = true;
+    if (const char* env = getenv("DICE_TUNING_CACHE")) {
+      t.path = env;
+    } else if (const char* home = getenv("HOME")) {
+      if (*home) t.path = std::string(home) + "/.cache/dice_hip/tuning.tsv";
+    }
+  }
+  return t.path;
+}
+
+// One winner per line: arch, bucket, rows, cols, sites variant, ms.
+void load_tuning_locked(TuningTable& t) {
+  if (t.loaded) return;
+  t.loaded = true;
+  const std::string path = tuning_path_locked(t);
+  if (path.empty()) return;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char arch[64], bucket[96];
+  TuneEntry e;
+  while (fscanf(f, "%63s %95s %d %d %d %lf", arch, bucket, &e.rows, &e.cols, &e.variant,
+                &e.ms) == 6) {
+    if (!tile_shape_supported(e.rows, e.cols) || e.variant < 0 || e.variant >= kSitesVariants)
+      continue;
+    e.from_file = true;
+    t.entries[std::string(arch) + "|" + bucket] = e;
+  }
+  fclose(f);
+}
+
+// Rewrites the cache file with every winner held (a temporary file renamed
+// over it); a cache that cannot be written leaves the winners in memory.
+void save_tuning_locked(TuningTable& t) {
+  const std::string path = tuning_path_locked(t);
+  if (path.empty()) return;
+  for (size_t at = path.find('/', 1); at != std::string::npos; at = path.find('/', at + 1))
+    (void)mkdir(path.substr(0, at).c_str(), 0755);
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  for (const auto& kv : t.entries) {
+    const size_t bar = kv.first.find('|');
+    fprintf(f, "%s\t%s\t%d\t%d\t%d\t%.4f\n", kv.first.substr(0, bar).c_str(),
+            kv.first.substr(bar + 1).c_str(), kv.second.rows, kv.second.cols,
+            kv.second.variant, kv.second.ms);
+  }
+  const bool ok = fclose(f) == 0;
+  if (!ok || rename(tmp.c_str(), path.c_str()) != 0) (void)remove(tmp.c_str());
+}
+
+int log2_floor(uint64_t v) {
+  int b = 0;
+  while (v >> (b + 1)) ++b;
+  return b;
+}
+
+}  // namespace
+
+void set_tuning_cache(const char* path) {
+  TuningTable& t = tuning();
+  std::lock_guard<std::mutex> lock(t.mutex);
+  t.path_set = path != nullptr;
+  t.path = path ? path : "";
+  t.entries.clear();
+  t.loaded = false;
+}
+
 int32_t Runner::autotune(JobRequest* jobs, int n, RunReport* report) {
   if (n < 1 || n > kMaxBatch) return set_error(DICE_INVALID, "a batch holds 1..%d jobs", kMaxBatch);
   HIP_TRY(hipSetDevice(device_));
-  std::string key = "T";
+  uint64_t total = 0, largest = 0;
+  int live = 0;
   for (int j = 0; j < n; ++j) {
-    char part[64];
-    snprintf(part, sizeof part, "|%lldx%lld", (long long)jobs[j].height, (long long)jobs[j].width);
-    key += part;
+    if (jobs[j].code != DICE_OK) continue;
+    const uint64_t pixels = (uint64_t)jobs[j].height * (uint64_t)jobs[j].width;
+    total += pixels;
+    largest = std::max(largest, pixels);
+    live += 1;
   }
-  auto it = tuned_.find(key);
-  if (it == tuned_.end()) {
-    int shapes[16][2];
-    const int count = std::min(16, tile_shapes(shapes, 16));
-    double best = 1e30;
-    std::pair<int, int> choice(tile_rows_, tile_cols_);
-    // trial copies: results, diagnostics and outputs of the timing runs are
-    // not the caller's
-    std::vector<JobRequest> trial(jobs, jobs + n);
-    std::vector<dice_replay_result> scratch(n);
-    for (int j = 0; j < n; ++j) {
-      trial[j].result = &scratch[j];
-      trial[j].diag = nullptr;
-      trial[j].timings = nullptr;
-      trial[j].cb = Callbacks{};
+  if (live == 0) return DICE_OK;
+  char bucket[96];
+  snprintf(bucket, sizeof bucket, "n%d-t%d-m%d", 1 << log2_floor((uint64_t)(2 * live - 1)),
+           log2_floor(total), log2_floor(largest));
+  const std::string key = arch_ + "|" + bucket;
+  TuningTable& table = tuning();
+  {
+    std::lock_guard<std::mutex> lock(table.mutex);
+    load_tuning_locked(table);
+    auto it = table.entries.find(key);
+    if (it != table.entries.end()) {
+      set_tile(it->second.rows, it->second.cols, it->second.variant);
+      if (report) {
+        report->tile_rows = tile_rows_;
+        report->tile_cols = tile_cols_;
+        report->sites_variant = sites_variant_;
+        report->tune_source = it->second.from_file ? 3 : 2;
+      }
+      return DICE_OK;
     }
-    for (int s = 0; s < count; ++s) {
-      set_tile(shapes[s][0], shapes[s][1]);
-      double shape_ms = 1e30;
-      for (int rep = 0; rep < 2; ++rep) {
-        fine_on_ = false;
-        DICE_TRY(prepare(trial.data(), n, nullptr));
-        HIP_TRY(hipEventRecord(ev_[4], stream_));
-        DICE_TRY(launch_a(n));
-        JobDev* H_jobs = h_jobs_.as<JobDev>();
-        for (int j = 0; j < n; ++j)
-          if (!H_jobs[j].skip)
-            HIP_TRY(hipMemcpyAsync(ws_[j]->h_small.ptr, ws_[j]->counters.ptr, 64,
-                                   hipMemcpyDeviceToHost, stream_));
-        HIP_TRY(hipEventRecord(ev_[5], stream_));
-        DICE_TRY(wait_stream(stream_));
+  }
+  // Every configuration, the default (16 x 64, variant 0) first: its
+  // results are the reference the others must reproduce.
+  struct Candidate {
+    int rows, cols, variant;
+  };
+  std::vector<Candidate> candidates;
+  candidates.push_back({16, 64, 0});
+  int shapes[16][2];
+  const int count = std::min(16, tile_shapes(shapes, 16));
+  for (int s = 0; s < count; ++s)
+    for (int v = 0; v < kSitesVariants; ++v)
+      if (!(shapes[s][0] == 16 && shapes[s][1] == 64 && v == 0))
+        candidates.push_back({shapes[s][0], shapes[s][1], v});
+  // trial copies: results, diagnostics and outputs of the timing runs are
+  // not the caller's
+  std::vector<JobRequest> trial(jobs, jobs + n);
+  std::vector<dice_replay_result> scratch(n);
+  for (int j = 0; j < n; ++j) {
+    trial[j].result = &scratch[j];
+    trial[j].diag = nullptr;
+    trial[j].timings = nullptr;
+    trial[j].cb = Callbacks{};
+  }
+  std::vector<unsigned long long> reference(n, 0), digest(n, 0);
+  std::vector<uint32_t> reference_sites(n, 0);
+  double default_ms = 0.0, best_ms = 1e30;
+  Candidate best = candidates[0];
+  int rejected = 0;
+  for (size_t c = 0; c < candidates.size(); ++c) {
+    set_tile(candidates[c].rows, candidates[c].cols, candidates[c].variant);
+    double shape_ms = 1e30;
+    bool agrees = true;
+    for (int rep = 0; rep < 3 && agrees; ++rep) {
+      fine_on_ = false;
+      DICE_TRY(prepare(trial.data(), n, nullptr));
+      HIP_TRY(hipEventRecord(ev_[4], stream_));
+      DICE_TRY(launch_a(n));
+      JobDev* H_jobs = h_jobs_.as<JobDev>();
+      for (int j = 0; j < n; ++j)
+        if (!H_jobs[j].skip)
+          HIP_TRY(hipMemcpyAsync(ws_[j]->h_small.ptr, ws_[j]->counters.ptr, 64,
+                                 hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipEventRecord(ev_[5], stream_));
+      DICE_TRY(wait_stream(stream_));
+      for (int j = 0; j < n; ++j) {
+        if (H_jobs[j].skip) continue;
+        Workspace& w = *ws_[j];
+        const size_t sites = w.h_small.as<uint32_t>()[0];
+        HIP_TRY(w.sites.ensure((sites + 1) * sizeof(Site), stream_));
+        HIP_TRY(w.records.ensure((sites + 1) * sizeof(WriterRecord), stream_));
+        H_jobs[j].sites = w.sites.as<Site>();
+        H_jobs[j].writer_records = w.records.as<WriterRecord>();
+      }
+      HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n,
+                             hipMemcpyHostToDevice, stream_));
+      HIP_TRY(hipEventRecord(ev_[6], stream_));
+      DICE_TRY(launch_b(n));
+      HIP_TRY(hipEventRecord(ev_[7], stream_));
+      if (rep == 0) {  // this configuration's results, digested per job
+        unsigned long long* d_digest = digest_.as<unsigned long long>();
+        HIP_TRY(launch_fill_u8(reinterpret_cast<uint8_t*>(d_digest), 8 * (size_t)n, 0, stream_));
+        for (int j = 0; j < n; ++j) {
+          const JobDev& J = H_jobs[j];
+          if (J.skip) continue;
+          const size_t sites = ws_[j]->h_small.as<uint32_t>()[0];
+          HIP_TRY(launch_digest(J.sites, sites * sizeof(Site), d_digest + j, stream_));
+          HIP_TRY(launch_digest(J.writer_records, sites * sizeof(WriterRecord), d_digest + j,
+                                stream_));
+          HIP_TRY(launch_digest(J.su_writer_records, 6 * (size_t)J.width * sizeof(WriterRecord),
+                                d_digest + j, stream_));
+          HIP_TRY(launch_digest(J.row_start, ((size_t)J.height + 1) * 4, d_digest + j, stream_));
+          HIP_TRY(launch_digest(J.out, (size_t)J.height * J.width * 6, d_digest + j, stream_));
+        }
+        HIP_TRY(hipMemcpyAsync(digest.data(), d_digest, 8 * (size_t)n, hipMemcpyDeviceToHost,
+                               stream_));
+      }
+      DICE_TRY(wait_stream(stream_));
+      if (rep == 0) {
         for (int j = 0; j < n; ++j) {
           if (H_jobs[j].skip) continue;
-          Workspace& w = *ws_[j];
-          const size_t c = w.h_small.as<uint32_t>()[0];
-          HIP_TRY(w.sites.ensure((c + 1) * sizeof(Site), stream_));
-          HIP_TRY(w.records.ensure((c + 1) * sizeof(WriterRecord), stream_));
-          H_jobs[j].sites = w.sites.as<Site>();
-          H_jobs[j].writer_records = w.records.as<WriterRecord>();
+          const uint32_t sites = ws_[j]->h_small.as<uint32_t>()[0];
+          if (c == 0) {
+            reference[j] = digest[j];
+            reference_sites[j] = sites;
+          } else if (digest[j] != reference[j] || sites != reference_sites[j]) {
+            agrees = false;
+          }
         }
-        HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n,
-                               hipMemcpyHostToDevice, stream_));
-        HIP_TRY(hipEventRecord(ev_[6], stream_));
-        DICE_TRY(launch_b(n));
-        HIP_TRY(hipEventRecord(ev_[7], stream_));
-        DICE_TRY(wait_stream(stream_));
-        float a = 0.0f, b = 0.0f;
-        (void)hipEventElapsedTime(&a, ev_[4], ev_[5]);
-        (void)hipEventElapsedTime(&b, ev_[6], ev_[7]);
-        shape_ms = std::min(shape_ms, (double)a + (double)b);
+        if (!agrees) break;
       }
-      if (shape_ms < best) {
-        best = shape_ms;
-        choice = {shapes[s][0], shapes[s][1]};
-      }
+      float a = 0.0f, b = 0.0f;
+      (void)hipEventElapsedTime(&a, ev_[4], ev_[5]);
+      (void)hipEventElapsedTime(&b, ev_[6], ev_[7]);
+      shape_ms = std::min(shape_ms, (double)a + (double)b);
     }
-    it = tuned_.emplace(key, choice).first;
+    if (!agrees) {
+      rejected += 1;
+      continue;
+    }
+    if (c == 0) default_ms = shape_ms;
+    if (shape_ms < best_ms) {
+      best_ms = shape_ms;
+      best = candidates[c];
+    }
   }
-  set_tile(it->second.first, it->second.second);
+  // keep the default unless a configuration is clearly faster
+  if (best_ms > 0.98 * default_ms) {
+    best = candidates[0];
+    best_ms = default_ms;
+  }
+  set_tile(best.rows, best.cols, best.variant);
+  {
+    std::lock_guard<std::mutex> lock(table.mutex);
+    TuneEntry e;
+    e.rows = best.rows;
+    e.cols = best.cols;
+    e.variant = best.variant;
+    e.ms = best_ms;
+    table.entries[key] = e;
+    save_tuning_locked(table);
+  }
   if (report) {
     report->tile_rows = tile_rows_;
     report->tile_cols = tile_cols_;
+    report->sites_variant = sites_variant_;
+    report->tune_source = 1;
+    report->tune_candidates = (int)candidates.size();
+    report->tune_rejected = rejected;
   }
   return DICE_OK;
 }

@@ -67,6 +67,12 @@ struct RunOptions {
 
 struct RunReport {
   int tile_rows = 0, tile_cols = 0;
+  int sites_variant = 0;
+  // how the launch configuration was chosen: 0 not tuned, 1 timed now,
+  // 2 a winner this process already holds for the batch's bucket, 3 a
+  // winner loaded from the tuning cache file
+  int tune_source = 0;
+  int tune_candidates = 0, tune_rejected = 0;
   bool graph_replayed = false;
   double phase_a_ms = 0.0, phase_b_ms = 0.0;  // device time (HIP events)
   uint64_t workspace_bytes = 0;
@@ -89,8 +95,15 @@ class Runner {
   // (each job's own status is in jobs[i].code / .error), or the code of a
   // failure that prevented the batch from running at all.
   int32_t run(JobRequest* jobs, int n, const RunOptions& options, RunReport* report);
-  // Times phases A+B of this batch for every supported tile shape and
-  // keeps the fastest (results of the timing runs are discarded).
+  // The launch configuration (tile shape, k_sites variant) for this batch's
+  // bucket (the device architecture, the job count and the total and
+  // largest frame sizes, each to a power of two): a winner already held or
+  // persisted in the tuning cache, or else every configuration timed on
+  // this batch (phases A+B) with its sites, draw schedules, row starts and
+  // output digested and compared with the default configuration's -- a
+  // configuration whose results differ is rejected -- and the fastest kept
+  // when it beats the default by more than 2 %.  The timing runs' results
+  // are discarded.
   int32_t autotune(JobRequest* jobs, int n, RunReport* report);
   // The producer record schedule of one frame (arrays only; the caller
   // packs the payloads).
@@ -106,9 +119,10 @@ class Runner {
   // DICE_OVER_BUDGET before the device phase that would need the space.
   int32_t run_in_arena(JobRequest& R, uint8_t* arena, size_t arena_bytes);
   static constexpr int64_t kArenaMaxRows = 16384;
-  void set_tile(int rows, int cols) {
+  void set_tile(int rows, int cols, int sites_variant = 0) {
     tile_rows_ = rows;
     tile_cols_ = cols;
+    sites_variant_ = sites_variant;
   }
   int tile_rows() const { return tile_rows_; }
   int tile_cols() const { return tile_cols_; }
@@ -146,13 +160,15 @@ class Runner {
   bool fine_on_ = false;
   hipEvent_t sync_event_ = nullptr;
   int tile_rows_ = 16, tile_cols_ = 64;
+  int sites_variant_ = 0;
+  std::string arch_;  // gfx architecture of the device, e.g. "gfx950"
   std::vector<std::unique_ptr<Workspace>> ws_;
   DeviceBuffer d_jobs_;
   DeviceBuffer queue_;  // ticket counter of the dynamically scheduled launches (zero at rest)
   PinnedBuffer h_jobs_;
   PinnedBuffer h_io_;  // run_in_arena's host side: row starts, chain state, counts, flags
   Graph graph_a_, graph_b_;
-  std::map<std::string, std::pair<int, int>> tuned_;
+  DeviceBuffer digest_;  // the tuner's result digests, one u64 per job
   DeviceBuffer sched_[12];
 };
 
@@ -163,6 +179,11 @@ class Runner {
 // rounding, every pixel an attempted site, any compiled tile shape): the
 // unit of the serving engine's HBM admission.
 uint64_t job_reservation(int64_t height, int64_t width, uint32_t flags, bool diagnostics);
+
+// The tuner's winners, process-wide, persisted in `path` (empty: the
+// default, $DICE_TUNING_CACHE or ~/.cache/dice_hip/tuning.tsv); setting it
+// forgets the winners held in memory, so the next lookup reads the file.
+void set_tuning_cache(const char* path);
 
 // `on_done(j)`, when given, is called once per job as soon as jobs[j]'s
 // status, error and result are final (possibly from a worker thread, before

@@ -175,7 +175,7 @@ int32_t FrameStream::advance() {
   J.su_site_state = w.states.as<uint32_t>();
   J.site_state = J.su_site_state + su;
   DICE_TRY(upload_job());
-  HIP_TRY(launch_sites(d, &J, 1, runner_.queue_.as<uint32_t>(), s));
+  HIP_TRY(launch_sites(d, &J, 1, runner_.queue_.as<uint32_t>(), s, runner_.sites_variant_));
   if (win.startup) HIP_TRY(launch_startup(d, &J, 1, s));
   HIP_TRY(launch_writer_prep(d, 1, s));
   WriterRecord* records = w.h_records.as<WriterRecord>();

@@ -224,6 +224,10 @@ class DiceBatchReport(C.Structure):
         ("phase_b_ms", C.c_double),
         ("workspace_bytes", C.c_uint64),
         ("wall_seconds", C.c_double),
+        ("sites_variant", C.c_int32),
+        ("tune_source", C.c_int32),
+        ("tune_candidates", C.c_int32),
+        ("tune_rejected", C.c_int32),
     ]
 
 
@@ -375,6 +379,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "engine_destroy": (None, [VOIDP]),
     "engine_submit": (C.c_int32, [VOIDP, C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
     "engine_wait": (C.c_int32, [VOIDP, C.c_uint64, C.c_double]),
+    "tuning_cache": (C.c_int32, [C.c_char_p]),
     "engine_reservation": (C.c_int32, [C.POINTER(DiceJob), C.POINTER(C.c_uint64)]),
     "engine_stats": (C.c_int32, [VOIDP, C.POINTER(DiceEngineStats)]),
     "engine_reset_stats": (C.c_int32, [VOIDP]),

@@ -259,7 +259,20 @@ typedef struct dice_batch_report {
   double phase_a_ms, phase_b_ms;  /* device time of the two batched phases */
   uint64_t workspace_bytes;       /* HBM held by the context's job workspaces */
   double wall_seconds;
+  int32_t sites_variant;          /* register variant of the site kernel used */
+  int32_t tune_source;            /* DICE_BATCH_AUTOTUNE: 1 timed on this batch, 2 a winner
+                                     held for its bucket, 3 read from the tuning cache */
+  int32_t tune_candidates;        /* configurations timed (tune_source 1) */
+  int32_t tune_rejected;          /* of those, configurations whose results differed */
 } dice_batch_report;
+
+/* Where the tuner's winners persist (no reference counterpart: the CUDA
+ * backend uses fixed launch shapes, cuda_backend/engine.py:298-423).  A
+ * winner is kept per (gfx architecture, bucket: job count and the total and
+ * largest frame sizes, each to a power of two).  NULL selects the default,
+ * $DICE_TUNING_CACHE or ~/.cache/dice_hip/tuning.tsv.  Forgets the winners
+ * held in memory, so the next tuned batch reads the file again. */
+int32_t dice_tuning_cache(const char* path);
 
 int32_t dice_abi_version(void);
 const char* dice_last_error(void);

@@ -3,7 +3,9 @@
 from __future__ import annotations
 
 import json
+import os
 import sys
+import tempfile
 from pathlib import Path
 
 import pytest
@@ -17,6 +19,10 @@ GOLDEN = ROOT / "tests" / "golden"
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+    # the tuner's winners persist in a file: keep the test session's apart
+    if "DICE_TUNING_CACHE" not in os.environ:
+        os.environ["DICE_TUNING_CACHE"] = str(
+            Path(tempfile.mkdtemp(prefix="dice-tuning-")) / "tuning.tsv")
 
 
 @pytest.fixture(scope="session")

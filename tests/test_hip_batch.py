@@ -136,6 +136,61 @@ def test_batch_autotune_keeps_results(frames, oracle_fingerprints):
         assert _core(replay_fingerprint(outcome.results[i].replay)) == oracle_fingerprints[i]
 
 
+def test_autotune_winner_is_reused_by_bucket_and_persisted(tmp_path):
+    """The tuner keys its winner by (architecture, bucket), not by exact
+    geometry: a second batch of different crops in the same bucket reuses
+    it, and after the in-memory winners are forgotten it is read back from
+    the cache file.  Every configuration's results were checked against the
+    default's (none rejected), and every run equals the oracle."""
+
+    from dice_amd import synth
+
+    lib = engine.library()
+    cache = tmp_path / "tuning.tsv"
+    assert lib.dice_tuning_cache(str(cache).encode()) == 0
+    try:
+        def crops(shapes, seed):
+            out = []
+            for i, (h, w) in enumerate(shapes):
+                main = synth.frame(synth.scaled_spec(h, w, seed=seed + i))
+                out.append((main, synth.prepass_for(main)))
+            return out
+
+        def check(batch, outcome):
+            outcome.raise_first()
+            params = stable_profile_params()
+            for (main, prepass), result in zip(batch, outcome.results):
+                cpu = oracle_lib.process_job(main, prepass, params,
+                                             DEFAULT_PROFILE.prepass_parameters())
+                assert result.replay.output_sha256 == cpu.output_sha256
+
+        first = crops([(700, 900), (800, 1000)], 7100)
+        tuned = process_batch_hip([make_job(m, p, "tune-a") for m, p in first], autotune=True)
+        check(first, tuned)
+        report = tuned.report
+        assert report["tune_source"] == 1
+        assert report["tune_candidates"] == 12 and report["tune_rejected"] == 0
+        winner = (report["tile_rows"], report["tile_cols"], report["sites_variant"])
+        # different geometries, same bucket: n2, total and largest in the same octaves
+        second = crops([(650, 950), (820, 980)], 7200)
+        hit = process_batch_hip([make_job(m, p, "tune-b") for m, p in second], autotune=True)
+        check(second, hit)
+        assert hit.report["tune_source"] == 2
+        assert (hit.report["tile_rows"], hit.report["tile_cols"],
+                hit.report["sites_variant"]) == winner
+        lines = cache.read_text().splitlines()
+        assert any(line.split("\t")[1] == "n2-t20-m19" for line in lines), lines
+        assert lib.dice_tuning_cache(str(cache).encode()) == 0  # forget, read back
+        again = process_batch_hip([make_job(m, p, "tune-c") for m, p in second],
+                                  autotune=True)
+        check(second, again)
+        assert again.report["tune_source"] == 3
+        assert (again.report["tile_rows"], again.report["tile_cols"],
+                again.report["sites_variant"]) == winner
+    finally:
+        lib.dice_tuning_cache(None)
+
+
 def test_failing_job_fails_alone(frames, oracle_fingerprints):
     """A job rejected by validation inside the batch does not disturb the others."""
 
