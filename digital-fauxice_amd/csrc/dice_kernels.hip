@@ -54,7 +54,8 @@ __device__ __forceinline__ int xcd_contiguous_index(int block, int blocks) {
   return start + slot;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over a block of `Threads`.
+template <int Threads = kThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value,
                                                          uint32_t* wave_sums,
                                                          uint32_t* total) {
@@ -69,7 +70,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value,
   __syncthreads();
   uint32_t offset = 0, sum = 0;
 #pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) {
+  for (int w = 0; w < Threads / 64; ++w) {
     uint32_t s = wave_sums[w];
     offset += w < wave ? s : 0u;
     sum += s;
@@ -846,7 +847,7 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
   });
 }
 
-template <int TY, int TX>
+template <int TY, int TX, int Threads>
 struct SitesShared {
   static constexpr int AR = TY + 10, AC = TX + 12;  // aux, edge-clamped, +-5/+-6
   static constexpr int HR = TY + 10, HC = TX + 10;  // score / waux history, +-5
@@ -854,7 +855,7 @@ struct SitesShared {
   // feature-record positions: the tile and a one-pixel ring (the cross
   // neighbours of its sites)
   static constexpr int PR = TY + 2, PC = TX + 2, NP = PR * PC;
-  static constexpr int kPerThread = (NP + kThreads - 1) / kThreads;
+  static constexpr int kPerThread = (NP + Threads - 1) / Threads;
   float aux[AR][AC];
   float score[HR][HC];
   float waux[HR][HC];
@@ -869,7 +870,7 @@ struct SitesShared {
   uint8_t writer_floor[TY];
   uint32_t row_start[TY];
   uint32_t row_offset[TY];
-  uint32_t wave_sums[kThreads / 64];
+  uint32_t wave_sums[Threads / 64];
 };
 
 // Reconstruction sites of one tile.  A feature record (streaming.py:236-308)
@@ -879,21 +880,32 @@ struct SitesShared {
 // neighbour mode, its four neighbours -- gets its averages once, in a table
 // of raw ratios and validity bits, and each site applies its own fallback
 // when it reads them.  Adjacent sites share most of their records.
-// Workgroups per CU the LDS of a tile shape allows (160 KB per CU), as
-// the occupancy target of its register allocation.
-template <int TY, int TX>
+// Workgroups per CU the LDS of a tile shape allows (160 KB per CU).
+template <int TY, int TX, int Threads>
 constexpr int sites_occupancy() {
-  return sizeof(SitesShared<TY, TX>) * 3 <= 160 * 1024   ? 3
-         : sizeof(SitesShared<TY, TX>) * 2 <= 160 * 1024 ? 2
-                                                         : 1;
+  return sizeof(SitesShared<TY, TX, Threads>) * 3 <= 160 * 1024   ? 3
+         : sizeof(SitesShared<TY, TX, Threads>) * 2 <= 160 * 1024 ? 2
+                                                                  : 1;
 }
 
-// `Blocks` is the workgroups per CU the register allocation targets: the
-// LDS occupancy of the tile shape (variant 0), or one fewer (variant 1: a
-// register-rich build, for the tuner to weigh).
-template <int TY, int TX, int Blocks>
-__global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
-  using Sh = SitesShared<TY, TX>;
+// The wide workgroup of a tile shape: as many threads as give the CU 16
+// waves (4 per SIMD) at the workgroups its LDS allows -- 1024 threads where
+// one tile fills the LDS, else 512 (two workgroups; where the LDS would
+// allow three, the 128-register budget of 4 waves per SIMD admits two).
+template <int TY, int TX>
+constexpr int sites_wide_threads() {
+  return sites_occupancy<TY, TX, 512>() == 1 ? 1024 : 512;
+}
+
+// `Threads` per workgroup and `Waves` per SIMD, the occupancy the register
+// allocation targets.  Variant 0 is the wide workgroup at 4 waves per SIMD:
+// every phase of a tile takes fewer rounds of the workgroup, and the other
+// tile on the CU runs while one waits.  Variant 1 is 256 threads at the
+// LDS occupancy of the shape (up to 3 waves per SIMD, more registers per
+// wave), for the tuner to weigh.
+template <int TY, int TX, int Threads, int Waves>
+__global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
+  using Sh = SitesShared<TY, TX, Threads>;
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
   constexpr int kWords = (NP + 31) / 32;
@@ -906,12 +918,16 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
     const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
     const int H = J.height, W = J.width;
     const int y0 = tile_y * TY, x0 = tile_x * TX;
-    const int tid = threadIdx.x;
+    // threadIdx.x made opaque once per tile, so that the LDS addresses
+    // derived from it are recomputed in each tile instead of being held
+    // across the persistent loop (at 128 registers they were spilled)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const float floor = P.score_floor;
     const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
 
-    for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
-    for (int i = tid; i < kWords; i += kThreads) S.needed_bits[i] = 0u;
+    for (int r = tid; r < AR; r += Threads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
+    for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
     // Per tile row (one lane each, in the first wave): its first entry in
     // the site list (an exclusive scan of the row counts), its first place
     // in the site array, and its writer stage.
@@ -940,7 +956,7 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
     // The record positions the sites need (record index = (ty+1)*PC + tx+1).
     // Neighbours outside the frame's columns are all-zero records, not
     // averages (streaming.py:259-308).
-    for (uint32_t q = tid; q < total; q += kThreads) {
+    for (uint32_t q = tid; q < total; q += Threads) {
       const int li = list[q];
       const int ty = li / TX, tx = li % TX;
       const int x = x0 + tx;
@@ -957,7 +973,7 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
 
     // Working lanes and aux on the clamped halo; working RGB staged for the
     // weighted-RGB history.
-    for (int i = tid; i < AR * AC; i += kThreads) {
+    for (int i = tid; i < AR * AC; i += Threads) {
       const int r = i / AC, c = i % AC;
       const int ly = y0 - 5 + r, lx = x0 - 6 + c;
       uint32_t lanes[4];
@@ -980,7 +996,7 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
     // Score / weighted-aux history with the asymmetric guards
     // (streaming.py:201-233): zero outside the row and above row -1, the
     // floor pseudo-row at -1, the last row repeated below.
-    for (int i = tid; i < HR * HC; i += kThreads) {
+    for (int i = tid; i < HR * HC; i += Threads) {
       const int r = i / HC, c = i % HC;
       const int ly = y0 - 5 + r, lx = x0 - 5 + c;
       float s, wa;
@@ -1003,7 +1019,7 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
       S.waux[r][c] = wa;
     }
     __syncthreads();
-    for (int i = tid; i < RR * RC; i += kThreads) {
+    for (int i = tid; i < RR * RC; i += Threads) {
       const int rr = i / RC, rc = i % RC;
       const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
       if (lx < 0 || lx >= W || ly < -1) {
@@ -1030,7 +1046,7 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
         const int i = first + k;
         if (i < NP && ((S.needed_bits[i >> 5] >> (i & 31)) & 1u)) ++count;
       }
-      uint32_t at = block_exclusive_scan(count, S.wave_sums, &needed);
+      uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &needed);
 #pragma unroll
       for (int k = 0; k < Sh::kPerThread; ++k) {
         const int i = first + k;
@@ -1042,13 +1058,18 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
     // The averages of every needed position (streaming.py:236-256; the A
     // lanes divide by the unrounded binary64 W, reconstruction.py:365-371).
     // Position (pr, pc) has its patch at history rows pr..pr+8, columns
-    // pc..pc+8.
-    for (uint32_t k = tid; k < needed; k += kThreads) {
-      const int i = S.needed_list[k];
-      const int pr = i / PC, pc = i % PC;
-      double unrounded[3], numerator[3];
-      unscaled_averages([&](int dy, int dx) { return S.score[pr + dy][pc + dx]; }, unrounded);
-      unscaled_averages([&](int dy, int dx) { return S.waux[pr + dy][pc + dx]; }, numerator);
+    // pc..pc+8, read from one base address at fixed offsets.
+    auto averages = [&](const float* plane, int pr, int pc, double q[3]) {
+      Column col[9];
+#pragma unroll
+      for (int x = 0; x < 9; ++x) {
+        const float* c = plane + pr * HC + pc + x;
+        col[x] = column_scratch(c[0], c[HC], c[2 * HC], c[3 * HC], c[4 * HC], c[5 * HC],
+                                c[6 * HC], c[7 * HC], c[8 * HC]);
+      }
+      combine_columns(col, q);
+    };
+    auto write_record = [&](int i, const double unrounded[3], const double numerator[3]) {
       uint8_t valid = 0;
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
@@ -1058,13 +1079,36 @@ __global__ __launch_bounds__(kThreads, Blocks) void k_sites(JobSet set) {
         valid |= positive ? (uint8_t)(1u << l) : (uint8_t)0;
       }
       S.record_valid[i] = valid;
+    };
+    if (2 * needed <= (uint32_t)Threads) {
+      // Two lanes per position when the positions fit one round: the even
+      // lane averages the score plane, the odd lane the weighted-aux plane,
+      // and the pair swaps its sums.
+      const uint32_t k = tid >> 1;
+      const bool odd = tid & 1;
+      const int i = k < needed ? S.needed_list[k] : 0;
+      const float* plane = odd ? &S.waux[0][0] : &S.score[0][0];
+      double mine[3] = {0.0, 0.0, 0.0};
+      if (k < needed) averages(plane, i / PC, i % PC, mine);
+      double other[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) other[l] = __shfl_xor(mine[l], 1, 64);
+      if (k < needed && !odd) write_record(i, mine, other);
+    } else {
+      for (uint32_t k = tid; k < needed; k += Threads) {
+        const int i = S.needed_list[k];
+        double unrounded[3], numerator[3];
+        averages(&S.score[0][0], i / PC, i % PC, unrounded);
+        averages(&S.waux[0][0], i / PC, i % PC, numerator);
+        write_record(i, unrounded, numerator);
+      }
     }
     __syncthreads();
 
     // One site per thread.  Eligibility already implies the driver gate
     // passes (reconstruction.py:890-904 re-tests the same row gate and the
     // same point score), so every listed centre is attempted.
-    for (uint32_t q = tid; q < total; q += kThreads) {
+    for (uint32_t q = tid; q < total; q += Threads) {
       const int li = list[q];
       const int ty = li / TX, tx = li % TX;
       const int y = y0 + ty, x = x0 + tx;
@@ -1958,12 +2002,14 @@ void launch_scan_shape(const JobSet& set, hipStream_t stream) {
 }
 template <int TY, int TX>
 void launch_sites_shape(const JobSet& set, int variant, hipStream_t stream) {
-  constexpr int kBlocks = sites_occupancy<TY, TX>();
-  constexpr int kFewer = kBlocks > 1 ? kBlocks - 1 : 1;
+  constexpr int kWide = sites_wide_threads<TY, TX>();
+  static_assert(sites_occupancy<TY, TX, kWide>() * kWide >= 1024,
+                "the wide workgroup must reach 16 waves per CU");
+  constexpr int kBlocks = sites_occupancy<TY, TX, kThreads>();
   if (variant == 1)
-    launch_persistent(k_sites<TY, TX, kFewer>, set, kThreads, stream);
+    launch_persistent(k_sites<TY, TX, kThreads, kBlocks>, set, kThreads, stream);
   else
-    launch_persistent(k_sites<TY, TX, kBlocks>, set, kThreads, stream);
+    launch_persistent(k_sites<TY, TX, kWide, 4>, set, kWide, stream);
 }
 }  // namespace
 

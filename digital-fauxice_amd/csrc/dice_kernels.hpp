@@ -40,9 +40,10 @@ hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStr
 hipError_t launch_scan_segments(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                 hipStream_t stream);
 // `queue`: two zeroed uint32 the launch uses as its ticket counter (and
-// leaves zeroed); one pair per stream.  `variant` selects the register
-// target of k_sites (0: the LDS occupancy of the tile shape, 1: one
-// workgroup per CU fewer); every variant computes the same sites.
+// leaves zeroed); one pair per stream.  `variant` selects the workgroup
+// of k_sites (0: 512 or 1024 threads at 4 waves per SIMD; 1: 256 threads
+// at the LDS occupancy of the tile shape); every variant computes the same
+// sites.
 constexpr int kSitesVariants = 2;
 hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint32_t* queue,
                         hipStream_t stream, int variant = 0);
