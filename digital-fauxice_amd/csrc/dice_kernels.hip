@@ -858,6 +858,9 @@ struct SitesShared {
   static constexpr int kPerThread = (NP + Threads - 1) / Threads;
   float aux[AR][AC];
   float score[HR][HC];
+  // the two planes sit 16 banks apart (of 32), so that the score and
+  // weighted-aux lanes of the feature records read disjoint banks
+  float pad_planes[((16 - (HR * HC) % 32) + 32) % 32];
   float waux[HR][HC];
   float wrgb[3][RR][RC];
   float record_w[3][NP];      // f32 of the unrounded score averages
@@ -897,6 +900,23 @@ constexpr int sites_wide_threads() {
   return sites_occupancy<TY, TX, 512>() == 1 ? 1024 : 512;
 }
 
+// Tiles per work unit of k_sites: a workgroup takes a vertical strip of
+// this many tiles, and each tile after the first keeps the bottom rows of
+// the halo the tile above staged (10 of the 26 aux and history rows, 8 of
+// the 24 weighted-RGB rows at 16-row tiles) instead of loading and
+// recomputing them.  Tiles shorter than the 10-row halo overlap do not
+// strip.  Strips cover the first three quarters of a job's tile rows; the
+// rest go one tile per unit, so that the units handed out last are short.
+__host__ __device__ constexpr int sites_strip(int tile_rows) { return tile_rows >= 10 ? 4 : 1; }
+__host__ __device__ inline int sites_strips(int tile_rows, int trow_n) {
+  const int k = sites_strip(tile_rows);
+  return k > 1 ? (trow_n * 3 / 4) / k : 0;
+}
+__host__ __device__ inline int sites_units(int tile_rows, int trow_n, int tiles_x) {
+  const int strips = sites_strips(tile_rows, trow_n);
+  return (strips + trow_n - strips * sites_strip(tile_rows)) * tiles_x;
+}
+
 // `Threads` per workgroup and `Waves` per SIMD, the occupancy the register
 // allocation targets.  Variant 0 is the wide workgroup at 4 waves per SIMD:
 // every phase of a tile takes fewer rounds of the workgroup, and the other
@@ -910,12 +930,11 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
   constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
   constexpr int kWords = (NP + 31) / 32;
   __shared__ Sh S;
-  for_each_unit_dynamic(set, [&](const JobDev& J, const int unit) {
+  // One tile with sites; `staged`: rows of its halo are those the tile
+  // above left in LDS.
+  auto sites_tile = [&](const JobDev& J, const int t, const int tile_y, const int tile_x,
+                        const uint32_t total, const bool staged) {
     const DevParams& P = J.P;
-    const int t = J.win.trow_lo * J.tiles_x + unit;
-    const uint32_t total = J.tile_count[t];
-    if (total == 0) return;
-    const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
     const int H = J.height, W = J.width;
     const int y0 = tile_y * TY, x0 = tile_x * TX;
     // threadIdx.x made opaque once per tile, so that the LDS addresses
@@ -951,6 +970,22 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       }
       if (i < TY) S.row_start[i] = inclusive - count;
     }
+    // A tile below a staged one takes its top halo rows from the bottom rows
+    // the tile above left (rows [TY, R) -> [0, R - TY); the two ranges are
+    // disjoint, so no barrier separates the reads from the writes).
+    static_assert(sites_strip(TY) == 1 || (AR - TY <= TY && HR - TY <= TY && RR - TY <= TY),
+                  "striped tiles must be at least as tall as their halo overlap");
+    if (sites_strip(TY) > 1 && staged) {
+      auto shift_up = [&](float* plane, int rows, int cols) {
+        for (int i = tid; i < (rows - TY) * cols; i += Threads) plane[i] = plane[i + TY * cols];
+      };
+      shift_up(&S.aux[0][0], AR, AC);
+      shift_up(&S.score[0][0], HR, HC);
+      shift_up(&S.waux[0][0], HR, HC);
+      shift_up(&S.wrgb[0][0][0], RR, RC);
+      shift_up(&S.wrgb[1][0][0], RR, RC);
+      shift_up(&S.wrgb[2][0][0], RR, RC);
+    }
     __syncthreads();
 
     // The record positions the sites need (record index = (ty+1)*PC + tx+1).
@@ -972,8 +1007,10 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
 
     // Working lanes and aux on the clamped halo; working RGB staged for the
-    // weighted-RGB history.
-    for (int i = tid; i < AR * AC; i += Threads) {
+    // weighted-RGB history.  Below a staged tile, from aux row AR - TY - 1
+    // on: the first row whose working RGB the weighted-RGB rows still need
+    // (its aux, rewritten, is the value already there).
+    for (int i = tid + (staged ? (AR - TY - 1) * AC : 0); i < AR * AC; i += Threads) {
       const int r = i / AC, c = i % AC;
       const int ly = y0 - 5 + r, lx = x0 - 6 + c;
       uint32_t lanes[4];
@@ -996,7 +1033,7 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     // Score / weighted-aux history with the asymmetric guards
     // (streaming.py:201-233): zero outside the row and above row -1, the
     // floor pseudo-row at -1, the last row repeated below.
-    for (int i = tid; i < HR * HC; i += Threads) {
+    for (int i = tid + (staged ? (HR - TY) * HC : 0); i < HR * HC; i += Threads) {
       const int r = i / HC, c = i % HC;
       const int ly = y0 - 5 + r, lx = x0 - 5 + c;
       float s, wa;
@@ -1019,7 +1056,7 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       S.waux[r][c] = wa;
     }
     __syncthreads();
-    for (int i = tid; i < RR * RC; i += Threads) {
+    for (int i = tid + (staged ? (RR - TY) * RC : 0); i < RR * RC; i += Threads) {
       const int rr = i / RC, rc = i % RC;
       const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
       if (lx < 0 || lx >= W || ly < -1) {
@@ -1164,6 +1201,34 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       site.orig[2] = original[2];
       site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
       J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
+    }
+  };
+  constexpr int kStrip = sites_strip(TY);
+  for_each_unit_dynamic(set, [&](const JobDev& J, const int unit) {
+    const int strips = sites_strips(TY, J.win.trow_n);
+    const int strip_units = strips * J.tiles_x;
+    int tile_x, trow_first, trow_end;
+    if (unit < strip_units) {
+      tile_x = unit % J.tiles_x;
+      trow_first = J.win.trow_lo + (unit / J.tiles_x) * kStrip;
+      trow_end = trow_first + kStrip;
+    } else {
+      const int u = unit - strip_units;
+      tile_x = u % J.tiles_x;
+      trow_first = J.win.trow_lo + strips * kStrip + u / J.tiles_x;
+      trow_end = trow_first + 1;
+    }
+    bool staged = false;  // the LDS holds the halo of the tile above
+    for (int tile_y = trow_first; tile_y < trow_end; ++tile_y) {
+      const int t = tile_y * J.tiles_x + tile_x;
+      const uint32_t total = J.tile_count[t];
+      if (total == 0) {
+        staged = false;
+        continue;
+      }
+      sites_tile(J, t, tile_y, tile_x, total, staged);
+      staged = kStrip > 1;
+      __syncthreads();  // the next tile reuses the LDS
     }
   });
 }
@@ -2066,8 +2131,10 @@ hipError_t launch_sites(const JobDev* d_jobs, const JobDev* h_jobs, int n, uint3
   bool any = false;
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
   if (!any) return hipSuccess;
-  JobSet set =
-      make_set(d_jobs, h_jobs, n, [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
+  // one unit per strip of sites_strip(ty) tiles, then one per tile
+  JobSet set = make_set(d_jobs, h_jobs, n, [ty](const JobDev& J) {
+    return sites_units(ty, J.win.trow_n, J.tiles_x);
+  });
   set.queue = queue;
 #define DICE_MATCH(a, b)                           \
   if (ty == a && tx == b) {                        \
