@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 #include "dice_device.hpp"
@@ -707,24 +708,41 @@ __device__ __forceinline__ RowParams load_row_params(const JobDev& J, int sy) {
   return p;
 }
 
+// k_scan runs 512-thread workgroups: each stages the 32 KB no-op table once
+// for 8 waves.  The register target is the occupancy the LDS allows (four
+// workgroups, 8 waves per SIMD at 64 registers, at the 16x64 tile).
+constexpr int kScanThreads = 512;
 template <int TY, int TX>
 struct ScanShared {
   static constexpr int AR = TY + 8, AC = TX + 8;  // aux, edge-clamped, +-4
   uint32_t noop_delta[8192];  // the no-op emit table as 4-bit deltas (32 KB)
   float aux[AR][AC];
+  // aux < decision threshold: row-major over the halo (bit r * AC + c), and
+  // one word per halo column (bit r; 32 bits while the halo has 32 rows)
+  static_assert(AR <= 64, "one 64-bit word per halo column");
+  using ColumnWord = typename std::conditional<(AR <= 32), uint32_t, unsigned long long>::type;
+  unsigned long long under_flat[(AR * AC + 63) / 64];
+  ColumnWord under_column[AC];
   RowParams rows[AR];
   uint8_t writer_floor[TY];
   int32_t writer_gate[TY];
   uint32_t row_count[TY];
-  // eligible centres per wave in each slab of kThreads centres
-  uint32_t slab_wave[TY * TX / kThreads][kThreads / 64];
+  // eligible centres per wave in each slab of kScanThreads centres
+  uint32_t slab_wave[TY * TX / kScanThreads][kScanThreads / 64];
 };
 
 template <int TY, int TX>
-__global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
+constexpr int scan_waves() {
+  const int blocks = (int)((160 * 1024) / sizeof(ScanShared<TY, TX>));
+  const int waves = blocks * (kScanThreads / 64) / 4;
+  return waves < 8 ? waves : 8;
+}
+
+template <int TY, int TX>
+__global__ __launch_bounds__(kScanThreads, (scan_waves<TY, TX>())) void k_scan(JobSet set) {
   using Sh = ScanShared<TY, TX>;
   constexpr int AR = Sh::AR, AC = Sh::AC;
-  static_assert((TY * TX) % kThreads == 0, "tile must be a multiple of the block");
+  static_assert((TY * TX) % kScanThreads == 0, "tile must be a multiple of the block");
   __shared__ Sh S;
   bool table_staged = false;  // uniform: every job of a device shares the tables
   for_each_unit(set, [&](const JobDev& J, const int unit) {
@@ -738,23 +756,24 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
 
     if (!table_staged) {  // once per workgroup; the barrier below orders it
       const uint4* table = reinterpret_cast<const uint4*>(J.noop_delta);
-      for (int i = tid; i < 8192 / 4; i += kThreads)
+      for (int i = tid; i < 8192 / 4; i += kScanThreads)
         reinterpret_cast<uint4*>(S.noop_delta)[i] = table[i];
       table_staged = true;
     }
-    for (int r = tid; r < AR; r += kThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
-    for (int i = tid; i < TY; i += kThreads) {
+    for (int r = tid; r < AR; r += kScanThreads) S.rows[r] = load_row_params(J, clampi(y0 - 4 + r, 0, H - 1));
+    for (int i = tid; i < TY; i += kScanThreads) {
       const int y = min(y0 + i, H - 1);
       S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
       S.writer_gate[i] = J.stages.row_gate[y + 6];
       S.row_count[i] = 0;
     }
+    for (int i = tid; i < AC; i += kScanThreads) S.under_column[i] = 0;
     __syncthreads();
 
     // x3a.py:193-221 on the clamped halo; no-op emit (output.py:76-94) of the
     // centres.
     uint32_t aux_error = 0;
-    for (int i = tid; i < AR * AC; i += kThreads) {
+    for (int i = tid; i < AR * AC; i += kScanThreads) {
       const int r = i / AC, c = i % AC;
       const int ly = y0 - 4 + r, lx = x0 - 4 + c;
       uint32_t lanes[4];
@@ -766,6 +785,12 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
       const float a = rp.is_one ? rp.replacement : auxiliary(visible, infrared, rp.alpha, rp.offset);
       if (!__builtin_isfinite(a)) aux_error = kErrNonfiniteAux;
       S.aux[r][c] = a;
+      // the decision bars' samples as bits: a wave holds 64 consecutive
+      // cells (i of its lane 0 is a multiple of 64)
+      const bool under = a < P.decision_threshold;
+      const unsigned long long under_wave = __ballot(under);
+      if ((tid & 63) == 0) S.under_flat[i >> 6] = under_wave;
+      if (under) atomicOr(&S.under_column[c], (typename Sh::ColumnWord)1 << r);
       const int ty = r - 4, tx = c - 4;
       if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
         // emit(LUT[raw]) is a function of the raw sample alone
@@ -783,14 +808,14 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
     __syncthreads();
 
     // Centre score (x3a.py:245-286), decision bars (streaming.py:157-198) and
-    // eligibility (streaming.py:461-466).  Centre k * kThreads + tid: the 64
+    // eligibility (streaming.py:461-466).  Centre k * kScanThreads + tid: the 64
     // lanes of a wave read 64 consecutive LDS words (no bank conflicts) and
     // store consecutive diagnostics.
-    constexpr int kPer = TY * TX / kThreads;
+    constexpr int kPer = TY * TX / kScanThreads;
     uint32_t eligible_bits = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int li = k * kThreads + tid;
+      const int li = k * kScanThreads + tid;
       const int ty = li / TX, tx = li % TX;
       const int y = y0 + ty, x = x0 + tx;
       if (y >= H || x >= W) continue;
@@ -806,8 +831,20 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
       }
       if (S.writer_gate[ty] != 0) continue;
       if (S.writer_floor[ty] && sc >= 1.0f) continue;
-      auto aux_at = [&](int dy, int dx) { return S.aux[ar + dy][ac + dx]; };
-      if (decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit)) continue;
+      // decision_fallback's four nine-sample bars, counted from the bits
+      const int R = P.radius;
+      auto row_bar = [&](int r) {  // columns ac-4 .. ac+4 of halo row r
+        const int f = r * AC + ac - 4, w = f >> 6, o = f & 63;
+        unsigned long long bits = S.under_flat[w] >> o;
+        if (o > 55) bits |= S.under_flat[w + 1] << (64 - o);
+        return __popcll(bits & 0x1FFull);
+      };
+      auto column_bar = [&](int c) {  // rows ar-4 .. ar+4 of halo column c
+        return __popcll(((unsigned long long)S.under_column[c] >> (ar - 4)) & 0x1FFull);
+      };
+      if (column_bar(ac - R) > P.count_limit || row_bar(ar - R) > P.count_limit ||
+          column_bar(ac + R) > P.count_limit || row_bar(ar + R) > P.count_limit)
+        continue;
       eligible_bits |= 1u << k;
     }
     // The queue in row-major order: slab by slab, and within a slab wave by
@@ -828,20 +865,20 @@ __global__ __launch_bounds__(kThreads) void k_scan(JobSet set) {
     for (int k = 0; k < kPer; ++k) {
       uint32_t before = 0, slab = 0;
 #pragma unroll
-      for (int w = 0; w < kThreads / 64; ++w) {
+      for (int w = 0; w < kScanThreads / 64; ++w) {
         const uint32_t c = S.slab_wave[k][w];
         before += w < wave ? c : 0u;
         slab += c;
       }
       if ((eligible_bits >> k) & 1u) {
-        const int li = k * kThreads + tid;
+        const int li = k * kScanThreads + tid;
         list[total + before + rank[k]] = (uint16_t)li;
         atomicAdd(&S.row_count[li / TX], 1u);
       }
       total += slab;
     }
     __syncthreads();
-    for (int i = tid; i < TY; i += kThreads)
+    for (int i = tid; i < TY; i += kScanThreads)
       J.seg_count[(size_t)(y0 + i) * J.tiles_x + tile_x] = S.row_count[i];
     if (tid == 0) J.tile_count[t] = total;
   });
@@ -1032,8 +1069,11 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
 
     // Score / weighted-aux history with the asymmetric guards
     // (streaming.py:201-233): zero outside the row and above row -1, the
-    // floor pseudo-row at -1, the last row repeated below.
-    for (int i = tid + (staged ? (HR - TY) * HC : 0); i < HR * HC; i += Threads) {
+    // floor pseudo-row at -1, the last row repeated below; each cell also
+    // weights its pixel's working RGB.  Below a staged tile, from history
+    // row HR - TY - 1 on: that row's scores weight the first restaged
+    // weighted-RGB row (recomputed, they equal the values already there).
+    for (int i = tid + (staged ? (HR - TY - 1) * HC : 0); i < HR * HC; i += Threads) {
       const int r = i / HC, c = i % HC;
       const int ly = y0 - 5 + r, lx = x0 - 5 + c;
       float s, wa;
@@ -1054,23 +1094,21 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       }
       S.score[r][c] = s;
       S.waux[r][c] = wa;
-    }
-    __syncthreads();
-    for (int i = tid + (staged ? (RR - TY) * RC : 0); i < RR * RC; i += Threads) {
-      const int rr = i / RC, rc = i % RC;
-      const int ly = y0 - 4 + rr, lx = x0 - 4 + rc;
-      if (lx < 0 || lx >= W || ly < -1) {
-        S.wrgb[0][rr][rc] = 0.0f;
-        S.wrgb[1][rr][rc] = 0.0f;
-        S.wrgb[2][rr][rc] = 0.0f;
-      } else {
-        const float s = S.score[rr + 1][rc + 1];
-        S.wrgb[0][rr][rc] = s * S.wrgb[0][rr][rc];
-        S.wrgb[1][rr][rc] = s * S.wrgb[1][rr][rc];
-        S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
+      // the weighted-RGB cell of the same pixel (history row r is weighted-
+      // RGB row r - 1, column c is column c - 1), zero outside the frame
+      const int rr = r - 1, rc = c - 1;
+      if (rr >= 0 && rr < RR && rc >= 0 && rc < RC) {
+        if (lx < 0 || lx >= W || ly < -1) {
+          S.wrgb[0][rr][rc] = 0.0f;
+          S.wrgb[1][rr][rc] = 0.0f;
+          S.wrgb[2][rr][rc] = 0.0f;
+        } else {
+          S.wrgb[0][rr][rc] = s * S.wrgb[0][rr][rc];
+          S.wrgb[1][rr][rc] = s * S.wrgb[1][rr][rc];
+          S.wrgb[2][rr][rc] = s * S.wrgb[2][rr][rc];
+        }
       }
     }
-
     __syncthreads();
 
     // Compact the needed positions (each thread owns a run of them).
@@ -2063,7 +2101,7 @@ bool tile_shape_supported(int ty, int tx) {
 namespace {
 template <int TY, int TX>
 void launch_scan_shape(const JobSet& set, hipStream_t stream) {
-  launch_persistent(k_scan<TY, TX>, set, kThreads, stream);
+  launch_persistent(k_scan<TY, TX>, set, kScanThreads, stream);
 }
 template <int TY, int TX>
 void launch_sites_shape(const JobSet& set, int variant, hipStream_t stream) {
