@@ -731,6 +731,17 @@ struct ScanShared {
   uint32_t slab_wave[TY * TX / kScanThreads][kScanThreads / 64];
 };
 
+// Tiles per work unit of k_scan: a vertical strip of kScanStrip tiles, each
+// after the first keeping the 8 halo rows the tile above staged.  Strips
+// cover the first three quarters of a job's tile rows; the rest go one tile
+// per unit, so that the units dealt last are short.
+constexpr int kScanStrip = 4;
+__host__ __device__ inline int scan_strips(int trow_n) { return (trow_n * 3 / 4) / kScanStrip; }
+__host__ __device__ inline int scan_units(int trow_n, int tiles_x) {
+  const int strips = scan_strips(trow_n);
+  return (strips + trow_n - strips * kScanStrip) * tiles_x;
+}
+
 template <int TY, int TX>
 constexpr int scan_waves() {
   const int blocks = (int)((160 * 1024) / sizeof(ScanShared<TY, TX>));
@@ -745,10 +756,11 @@ __global__ __launch_bounds__(kScanThreads, (scan_waves<TY, TX>())) void k_scan(J
   static_assert((TY * TX) % kScanThreads == 0, "tile must be a multiple of the block");
   __shared__ Sh S;
   bool table_staged = false;  // uniform: every job of a device shares the tables
-  for_each_unit(set, [&](const JobDev& J, const int unit) {
+  // One tile; `staged`: the top AR - TY halo rows are those the tile above
+  // left in LDS.
+  auto scan_tile = [&](const JobDev& J, const int tile_y, const int tile_x, const bool staged) {
     const DevParams& P = J.P;
-    const int t = J.win.trow_lo * J.tiles_x + unit;
-    const int tile_y = t / J.tiles_x, tile_x = t % J.tiles_x;
+    const int t = tile_y * J.tiles_x + tile_x;
     const int H = J.height, W = J.width;
     const int y0 = tile_y * TY, x0 = tile_x * TX;
     const int tid = threadIdx.x;
@@ -767,13 +779,43 @@ __global__ __launch_bounds__(kScanThreads, (scan_waves<TY, TX>())) void k_scan(J
       S.writer_gate[i] = J.stages.row_gate[y + 6];
       S.row_count[i] = 0;
     }
-    for (int i = tid; i < AC; i += kScanThreads) S.under_column[i] = 0;
+    // Below a staged tile, halo rows [TY, AR) become rows [0, AR - TY) (the
+    // ranges are disjoint: AR - TY = 8 <= TY), in the aux plane and in both
+    // bit layouts (TY * AC is a multiple of 64).
+    static_assert(AR - TY <= TY && (TY * AC) % 64 == 0, "halo rows shift by whole words");
+    for (int i = tid; i < AC; i += kScanThreads)
+      S.under_column[i] = staged ? S.under_column[i] >> TY : 0;
+    if (staged) {
+      for (int i = tid; i < (AR - TY) * AC; i += kScanThreads)
+        (&S.aux[0][0])[i] = (&S.aux[0][0])[i + TY * AC];
+      for (int i = tid; i < (AR - TY) * AC / 64; i += kScanThreads)
+        S.under_flat[i] = S.under_flat[i + TY * AC / 64];
+    }
     __syncthreads();
 
     // x3a.py:193-221 on the clamped halo; no-op emit (output.py:76-94) of the
     // centres.
     uint32_t aux_error = 0;
-    for (int i = tid; i < AR * AC; i += kScanThreads) {
+    auto noop = [&](uint32_t raw) {
+      const uint32_t delta = (S.noop_delta[raw >> 3] >> ((raw & 7u) * 4u)) & 15u;
+      return (uint16_t)(raw + delta - (uint32_t)kNoopDeltaBias);
+    };
+    // Below a staged tile only halo rows [AR - TY, AR) are loaded; the tile
+    // rows above them (halo rows 4 .. AR - TY - 1) still need their output.
+    if (staged) {
+      for (int i = tid; i < (AR - TY - 4) * TX; i += kScanThreads) {
+        const int ly = y0 + i / TX, lx = x0 + i % TX;
+        if (ly < H && lx < W) {
+          uint32_t lanes[4];
+          load_pixel(J.rgbi, (size_t)ly * W + lx, lanes);
+          uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
+          o[0] = noop(lanes[0]);
+          o[1] = noop(lanes[1]);
+          o[2] = noop(lanes[2]);
+        }
+      }
+    }
+    for (int i = tid + (staged ? (AR - TY) * AC : 0); i < AR * AC; i += kScanThreads) {
       const int r = i / AC, c = i % AC;
       const int ly = y0 - 4 + r, lx = x0 - 4 + c;
       uint32_t lanes[4];
@@ -794,10 +836,6 @@ __global__ __launch_bounds__(kScanThreads, (scan_waves<TY, TX>())) void k_scan(J
       const int ty = r - 4, tx = c - 4;
       if (ty >= 0 && ty < TY && tx >= 0 && tx < TX && ly < H && lx < W) {
         // emit(LUT[raw]) is a function of the raw sample alone
-        auto noop = [&](uint32_t raw) {
-          const uint32_t delta = (S.noop_delta[raw >> 3] >> ((raw & 7u) * 4u)) & 15u;
-          return (uint16_t)(raw + delta - (uint32_t)kNoopDeltaBias);
-        };
         uint16_t* o = J.out + ((size_t)ly * W + lx) * 3;
         o[0] = noop(lanes[0]);
         o[1] = noop(lanes[1]);
@@ -881,6 +919,25 @@ __global__ __launch_bounds__(kScanThreads, (scan_waves<TY, TX>())) void k_scan(J
     for (int i = tid; i < TY; i += kScanThreads)
       J.seg_count[(size_t)(y0 + i) * J.tiles_x + tile_x] = S.row_count[i];
     if (tid == 0) J.tile_count[t] = total;
+  };
+  for_each_unit(set, [&](const JobDev& J, const int unit) {
+    const int strips = scan_strips(J.win.trow_n);
+    const int strip_units = strips * J.tiles_x;
+    int tile_x, trow_first, trow_end;
+    if (unit < strip_units) {
+      tile_x = unit % J.tiles_x;
+      trow_first = J.win.trow_lo + (unit / J.tiles_x) * kScanStrip;
+      trow_end = trow_first + kScanStrip;
+    } else {
+      const int u = unit - strip_units;
+      tile_x = u % J.tiles_x;
+      trow_first = J.win.trow_lo + strips * kScanStrip + u / J.tiles_x;
+      trow_end = trow_first + 1;
+    }
+    for (int tile_y = trow_first; tile_y < trow_end; ++tile_y) {
+      scan_tile(J, tile_y, tile_x, tile_y > trow_first);
+      if (tile_y + 1 < trow_end) __syncthreads();  // the next tile reuses the LDS
+    }
   });
 }
 
@@ -2140,8 +2197,10 @@ hipError_t launch_scan(const JobDev* d_jobs, const JobDev* h_jobs, int n, hipStr
   bool any = false;
   if (!batch_tile(h_jobs, n, &ty, &tx, &any)) return hipErrorInvalidValue;
   if (!any) return hipSuccess;
-  const JobSet set = make_set(d_jobs, h_jobs, n,
-                              [](const JobDev& J) { return J.win.trow_n * J.tiles_x; });
+  // one unit per strip of kScanStrip tiles, then one per tile
+  const JobSet set = make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
+    return scan_units(J.win.trow_n, J.tiles_x);
+  });
 #define DICE_MATCH(a, b)                 \
   if (ty == a && tx == b) {              \
     launch_scan_shape<a, b>(set, stream); \
