@@ -259,7 +259,7 @@ typedef struct dice_batch_report {
   double phase_a_ms, phase_b_ms;  /* device time of the two batched phases */
   uint64_t workspace_bytes;       /* HBM held by the context's job workspaces */
   double wall_seconds;
-  int32_t sites_variant;          /* register variant of the site kernel used */
+  int32_t sites_variant;          /* site-kernel workgroup variant used (0 wide, 1 256 threads) */
   int32_t tune_source;            /* DICE_BATCH_AUTOTUNE: 1 timed on this batch, 2 a winner
                                      held for its bucket, 3 read from the tuning cache */
   int32_t tune_candidates;        /* configurations timed (tune_source 1) */
