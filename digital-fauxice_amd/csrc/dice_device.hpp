@@ -96,6 +96,54 @@ DICE_HD uint32_t prefix_count(const F& holds) {
   return lo;
 }
 
+// prefix_count started from an estimate of the count: it gallops from the
+// estimate to a bracket and bisects the bracket.  For a monotone predicate
+// the result is prefix_count's; an estimate within k states of the count
+// takes about 2 log2(k) + 4 evaluations instead of 26.  A non-finite
+// estimate bisects the whole range.
+template <class F>
+DICE_HD uint32_t prefix_count_near(const F& holds, double estimate) {
+  if (!(estimate == estimate) || estimate - estimate != 0.0) return prefix_count(holds);
+  if (!holds(0u)) return 0;
+  if (holds(kLcgMask)) return 1u << 24;
+  // the count c lies in [1, 2^24 - 1]: holds(u) <=> u < c
+  const double top = (double)(kLcgMask - 1);
+  const uint32_t g = estimate <= 1.0 ? 1u : (estimate >= top ? kLcgMask - 1 : (uint32_t)estimate);
+  uint32_t lo, hi;
+  if (holds(g)) {  // c > g
+    lo = g + 1;
+    hi = kLcgMask;
+    for (uint32_t step = 1;; step <<= 1) {
+      const uint32_t p = g + step;
+      if (p >= kLcgMask) break;
+      if (!holds(p)) {
+        hi = p;
+        break;
+      }
+      lo = p + 1;
+    }
+  } else {  // c <= g
+    lo = 1;
+    hi = g;
+    for (uint32_t step = 1; step < g; step <<= 1) {
+      const uint32_t p = g - step;
+      if (holds(p)) {
+        lo = p + 1;
+        break;
+      }
+      hi = p;
+    }
+  }
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (holds(mid))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // The integer draw schedule of one site (reconstruction.py:916-982,
 // dither.py:59-104).  Every operation between the LCG state and the two
 // comparisons the writer makes -- acceptance low < cand32 + delta < high
@@ -134,12 +182,21 @@ DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const Write
       };
       const double cand = site.cand[c];
       const double source = (double)site.orig[c];
+      // where x + delta(u) crosses t in real arithmetic: the estimate each
+      // threshold search starts from
+      auto crossing = [&](double x, double t) {
+        const double state = ((t - x) / (envelope * span) + 0.5) / kNikonNormalization - 1.0;
+        return mirrored ? (double)kLcgMask - state : state;
+      };
       // acceptance low < cd + delta < high holds for u in [lo, hi); when
       // accepted, the floor test cand + delta <= orig holds for u < fe; a
       // rejected draw (delta = 0) keeps the value when cand <= orig
-      const uint32_t lo = prefix_count([&](uint32_t u) { return cd + delta(u) <= K.low; });
-      const uint32_t hi = prefix_count([&](uint32_t u) { return cd + delta(u) < K.high; });
-      const uint32_t fe = prefix_count([&](uint32_t u) { return cand + delta(u) <= source; });
+      const uint32_t lo = prefix_count_near([&](uint32_t u) { return cd + delta(u) <= K.low; },
+                                            crossing(cd, K.low));
+      const uint32_t hi = prefix_count_near([&](uint32_t u) { return cd + delta(u) < K.high; },
+                                            crossing(cd, K.high));
+      const uint32_t fe = prefix_count_near([&](uint32_t u) { return cand + delta(u) <= source; },
+                                            crossing(cand, source));
       const bool keep_when_rejected = cand + 0.0 <= source;
       if (!keep_when_rejected) {
         // keep <=> lo <= u < min(hi, fe)
