@@ -1456,33 +1456,44 @@ __global__ __launch_bounds__(kThreads) void k_startup_rows(JobSet set) {
 }
 
 namespace {
-// _startup_histories (startup.py:74-128) as point accessors.
+// The startup histories one workgroup's sites read, staged in LDS: real rows
+// 0..4 (the pseudo-row -1 derives from row 0), columns x0 - 6 .. x0 + 261
+// clamped into the frame.
+constexpr int kStartupCols = kThreads + 12;
+struct StartupShared {
+  float score[5][kStartupCols];
+  float aux[5][kStartupCols];
+  float work[3][5][kStartupCols];
+};
+
+// _startup_histories (startup.py:74-128) as point accessors over the staged
+// columns (x0 - 6 + c for column c).
 struct StartupView {
-  const JobDev& J;
-  int W;
+  const StartupShared& S;
+  int W, x0;
   float floor;
+  __device__ int col(int x) const { return x - x0 + 6; }
   __device__ float score(int y, int x) const {
     if (x < 0 || x >= W || y < -1) return 0.0f;
     if (y == -1) return floor;
-    return J.su_score[y * W + x];
+    return S.score[y][col(x)];
   }
   __device__ float waux(int y, int x) const {
     if (x < 0 || x >= W || y < -1) return 0.0f;
-    if (y == -1) return J.su_aux[x] * floor;
-    return J.su_score[y * W + x] * J.su_aux[y * W + x];
+    if (y == -1) return S.aux[0][col(x)] * floor;
+    return S.score[y][col(x)] * S.aux[y][col(x)];
   }
   __device__ float wrgb(int y, int x, int k) const {
     if (x < 0 || x >= W || y < -1) return 0.0f;
-    if (y == -1) return J.su_work[x * 3 + k] * floor;
-    return J.su_score[y * W + x] * J.su_work[(y * W + x) * 3 + k];
+    if (y == -1) return S.work[k][0][col(x)] * floor;
+    return S.score[y][col(x)] * S.work[k][y][col(x)];
   }
   // Raw decision history: first real row for y < 0; left guard replicates;
-  // the right guard is replicated for real rows and zero for y < 0.
+  // the right guard is replicated for real rows and zero for y < 0 (the
+  // staged columns are clamped into the frame).
   __device__ float raw(int y, int x) const {
-    const float* source = J.su_aux + (y < 0 ? 0 : y) * W;
-    if (x < 0) return source[0];
-    if (x >= W) return y >= 0 ? source[W - 1] : 0.0f;
-    return source[x];
+    if (x >= W && y < 0) return 0.0f;
+    return S.aux[y < 0 ? 0 : y][col(x)];
   }
   // startup.py:131-183: rows at or below the first hidden centre and
   // out-of-row columns hold allocator zero.
@@ -1500,19 +1511,35 @@ struct StartupView {
 };
 }  // namespace
 
+// The hidden-startup sites: one workgroup per (stage, run of kThreads
+// columns), its columns' startup histories staged in LDS first.
 __global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
+  __shared__ StartupShared S;
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
     const int W = J.width;
-    const int i = unit * blockDim.x + threadIdx.x;
-    if (i >= 6 * W) return;
-    const int stage = i / W + 1, x = i % W;
+    const int chunks = (W + kThreads - 1) / kThreads;
+    const int stage = unit / chunks + 1;
+    const int x0 = (unit % chunks) * kThreads;
+    const int x = x0 + threadIdx.x;
+    for (int i = threadIdx.x; i < 5 * kStartupCols; i += kThreads) {
+      const int y = i / kStartupCols, c = i % kStartupCols;
+      const size_t at = (size_t)y * W + clampi(x0 - 6 + c, 0, W - 1);
+      S.score[y][c] = J.su_score[at];
+      S.aux[y][c] = J.su_aux[at];
+      S.work[0][y][c] = J.su_work[at * 3 + 0];
+      S.work[1][y][c] = J.su_work[at * 3 + 1];
+      S.work[2][y][c] = J.su_work[at * 3 + 2];
+    }
+    __syncthreads();
+    if (x >= W) return;
+    const int i = (stage - 1) * W + x;
     const int center_y = stage - 7;
     const bool cross = P.cross_neighbor != 0;
     const int writer_width = (cross && stage <= 2) ? W - 4 : W;
     J.su_attempted[i] = 0;
     if (x >= writer_width) return;
-    StartupView V{J, W, P.score_floor};
+    StartupView V{S, W, x0, P.score_floor};
     const StageDev& st = J.stages;
     const bool floor_enabled = P.gate_primary || st.gate_secondary[stage - 1];
     const int32_t row_gate = st.row_gate[stage - 1];
@@ -1540,8 +1567,8 @@ __global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
     };
     float candidates[3][3];
     rgb_candidates(wrgb_at, weights, candidates);
-    const float original[3] = {J.su_work[x * 3 + 0], J.su_work[x * 3 + 1],
-                               J.su_work[x * 3 + 2]};
+    const float original[3] = {S.work[0][0][V.col(x)], S.work[1][0][V.col(x)],
+                               S.work[2][0][V.col(x)]};
     Site site;
     combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
                       features, record_count, combiner_reference, P.combine, site.cand);
@@ -2279,7 +2306,7 @@ hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                            : 0u;
                     }), kThreads, stream);
   launch_persistent(k_startup_sites, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
-                      return J.win.startup ? blocks_for(6L * J.width, kThreads) : 0u;
+                      return J.win.startup ? 6u * blocks_for(J.width, kThreads) : 0u;
                     }), kThreads, stream);
   return hipGetLastError();
 }
