@@ -1456,14 +1456,21 @@ __global__ __launch_bounds__(kThreads) void k_startup_rows(JobSet set) {
 }
 
 namespace {
-// The startup histories one workgroup's sites read, staged in LDS: real rows
-// 0..4 (the pseudo-row -1 derives from row 0), columns x0 - 6 .. x0 + 261
-// clamped into the frame.
-constexpr int kStartupCols = kThreads + 12;
+// k_startup_sites: 16 sites per workgroup, 16 lanes per site.  Lanes 0-4
+// average the score windows of the site's five feature records, lanes 5-9
+// their weighted-aux windows, lanes 10-12 the three weighted-RGB windows,
+// lane 13 counts the decision bars; lane 0 then forms the site from them.
+constexpr int kStartupSites = kThreads / 16;
+// The startup histories of the workgroup's columns, staged in LDS: real
+// rows 0..4 (the pseudo-row -1 derives from row 0), columns x0 - 6 ..
+// x0 + kStartupSites + 5 clamped into the frame.
+constexpr int kStartupCols = kStartupSites + 12;
 struct StartupShared {
   float score[5][kStartupCols];
   float aux[5][kStartupCols];
   float work[3][5][kStartupCols];
+  double sums[kStartupSites][13][3];  // the lanes' averages
+  uint8_t fallback[kStartupSites];    // the decision bars' verdict
 };
 
 // _startup_histories (startup.py:74-128) as point accessors over the staged
@@ -1497,31 +1504,23 @@ struct StartupView {
   }
   // startup.py:131-183: rows at or below the first hidden centre and
   // out-of-row columns hold allocator zero.
-  __device__ void record(int y, int x, float fallback, float w[4], float f[4]) const {
-    if (y <= -6 || x < 0 || x >= W) {
-      w[0] = w[1] = w[2] = w[3] = 0.0f;
-      f[0] = f[1] = f[2] = f[3] = 0.0f;
-      return;
-    }
-    auto score_at = [&](int dy, int dx) { return score(y - 4 + dy, x - 4 + dx); };
-    auto waux_at = [&](int dy, int dx) { return waux(y - 4 + dy, x - 4 + dx); };
-    const float point_score = score(y, x);
-    feature_record(score_at, waux_at, raw(y, x), point_score, fallback, w, f);
-  }
+  __device__ bool has_record(int y, int x) const { return y > -6 && x >= 0 && x < W; }
 };
 }  // namespace
 
-// The hidden-startup sites: one workgroup per (stage, run of kThreads
+// The hidden-startup sites: one workgroup per (stage, run of kStartupSites
 // columns), its columns' startup histories staged in LDS first.
 __global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
   __shared__ StartupShared S;
   for_each_unit(set, [&](const JobDev& J, const int unit) {
     const DevParams& P = J.P;
     const int W = J.width;
-    const int chunks = (W + kThreads - 1) / kThreads;
+    const int chunks = (W + kStartupSites - 1) / kStartupSites;
     const int stage = unit / chunks + 1;
-    const int x0 = (unit % chunks) * kThreads;
-    const int x = x0 + threadIdx.x;
+    const int x0 = (unit % chunks) * kStartupSites;
+    const int site = threadIdx.x >> 4, job = threadIdx.x & 15;
+    const int x = x0 + site;
+    const int center_y = stage - 7;
     for (int i = threadIdx.x; i < 5 * kStartupCols; i += kThreads) {
       const int y = i / kStartupCols, c = i % kStartupCols;
       const size_t at = (size_t)y * W + clampi(x0 - 6 + c, 0, W - 1);
@@ -1532,14 +1531,42 @@ __global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
       S.work[2][y][c] = J.su_work[at * 3 + 2];
     }
     __syncthreads();
-    if (x >= W) return;
+    StartupView V{S, W, x0, P.score_floor};
+    // the five records: centre, left, right, up, down
+    const int ry[5] = {center_y, center_y, center_y, center_y - 1, center_y + 1};
+    const int rx[5] = {x, x - 1, x + 1, x, x};
+    if (x < W && job < 13) {
+      double q[3] = {0.0, 0.0, 0.0};
+      if (job < 10) {
+        const int r = job < 5 ? job : job - 5;
+        const int y = ry[r], xr = rx[r];
+        if (V.has_record(y, xr)) {
+          if (job < 5)
+            unscaled_averages([&](int dy, int dx) { return V.score(y - 4 + dy, xr - 4 + dx); }, q);
+          else
+            unscaled_averages([&](int dy, int dx) { return V.waux(y - 4 + dy, xr - 4 + dx); }, q);
+        }
+      } else {
+        const int k = job - 10;
+        auto lane = [&](int dy, int dx) { return V.wrgb(center_y - 4 + dy, x - 4 + dx, k); };
+        unscaled_averages(lane, q);
+        q[2] = rgb_q16(lane);
+      }
+      S.sums[site][job][0] = q[0];
+      S.sums[site][job][1] = q[1];
+      S.sums[site][job][2] = q[2];
+    } else if (x < W && job == 13) {
+      auto aux_at = [&](int dy, int dx) { return V.raw(center_y + dy, x + dx); };
+      S.fallback[site] =
+          decision_fallback(aux_at, P.decision_threshold, P.radius, P.count_limit) ? 1 : 0;
+    }
+    __syncthreads();
+    if (x >= W || job != 0) return;
     const int i = (stage - 1) * W + x;
-    const int center_y = stage - 7;
     const bool cross = P.cross_neighbor != 0;
     const int writer_width = (cross && stage <= 2) ? W - 4 : W;
     J.su_attempted[i] = 0;
     if (x >= writer_width) return;
-    StartupView V{S, W, x0, P.score_floor};
     const StageDev& st = J.stages;
     const bool floor_enabled = P.gate_primary || st.gate_secondary[stage - 1];
     const int32_t row_gate = st.row_gate[stage - 1];
@@ -1547,36 +1574,60 @@ __global__ __launch_bounds__(kThreads) void k_startup_sites(JobSet set) {
     const float fallback_center = st.coarse[max(1, center_y + 6) - 1];
     const float fallback_up = st.coarse[max(1, center_y + 5) - 1];
     const float fallback_down = st.coarse[max(1, center_y + 7) - 1];
-    auto aux_at = [&](int dy, int dx) { return V.raw(center_y + dy, x + dx); };
-    const bool fallback = decision_fallback(aux_at, P.decision_threshold, P.radius,
-                                            P.count_limit);
+    const float record_fallback[5] = {fallback_center, fallback_center, fallback_center,
+                                      fallback_up, fallback_down};
+    // feature_record (streaming.py:236-256) from the lanes' sums
     float weights[4], features[5][4];
-    V.record(center_y, x, fallback_center, weights, features[0]);
-    if (fallback || row_gate != 0 || (floor_enabled && weights[3] >= 1.0f)) return;
+    auto record = [&](int r, float w[4], float f[4]) {
+      const int y = ry[r], xr = rx[r];
+      if (!V.has_record(y, xr)) {
+        w[0] = w[1] = w[2] = w[3] = 0.0f;
+        f[0] = f[1] = f[2] = f[3] = 0.0f;
+        return;
+      }
+      const double* unrounded = S.sums[site][r];
+      const double* numerator = S.sums[site][r + 5];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        w[l] = (float)unrounded[l];
+        f[l] = unrounded[l] > 0.0 ? (float)(numerator[l] / unrounded[l]) : record_fallback[r];
+      }
+      w[3] = V.score(y, xr);
+      f[3] = w[3] > 0.0f ? V.raw(y, xr) : record_fallback[r];
+    };
+    record(0, weights, features[0]);
+    if (S.fallback[site] || row_gate != 0 || (floor_enabled && weights[3] >= 1.0f)) return;
     int record_count = 1;
     if (cross) {
       record_count = 5;
       float unused[4];
-      V.record(center_y, x - 1, fallback_center, unused, features[1]);
-      V.record(center_y, x + 1, fallback_center, unused, features[2]);
-      V.record(center_y - 1, x, fallback_up, unused, features[3]);
-      V.record(center_y + 1, x, fallback_down, unused, features[4]);
+      for (int r = 1; r < 5; ++r) record(r, unused, features[r]);
     }
-    auto wrgb_at = [&](int dy, int dx, int k) {
-      return V.wrgb(center_y - 4 + dy, x - 4 + dx, k);
-    };
+    // rgb_candidates (reconstruction.py:425-462) from the lanes' sums
     float candidates[3][3];
-    rgb_candidates(wrgb_at, weights, candidates);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double* q = S.sums[site][10 + k];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        if (weights[s] > 0.0f) {
+          double reciprocal = 1.0 / (double)weights[s];
+          candidates[s][k] = (float)(q[s] * reciprocal);
+        } else {
+          candidates[s][k] = (float)q[s];
+        }
+      }
+    }
     const float original[3] = {S.work[0][0][V.col(x)], S.work[1][0][V.col(x)],
                                S.work[2][0][V.col(x)]};
-    Site site;
+    Site out;
     combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
-                      features, record_count, combiner_reference, P.combine, site.cand);
-    site.orig[0] = original[0];
-    site.orig[1] = original[1];
-    site.orig[2] = original[2];
-    site.pixel = (uint32_t)x | (floor_enabled ? kSiteFloorBit : 0u);
-    J.su_sites[i] = site;
+                      features, record_count, combiner_reference, P.combine, out.cand);
+    out.orig[0] = original[0];
+    out.orig[1] = original[1];
+    out.orig[2] = original[2];
+    out.pixel = (uint32_t)x | (floor_enabled ? kSiteFloorBit : 0u);
+    J.su_sites[i] = out;
     J.su_attempted[i] = 1;
   });
 }
@@ -2306,7 +2357,7 @@ hipError_t launch_startup(const JobDev* d_jobs, const JobDev* h_jobs, int n,
                                            : 0u;
                     }), kThreads, stream);
   launch_persistent(k_startup_sites, make_set(d_jobs, h_jobs, n, [](const JobDev& J) {
-                      return J.win.startup ? 6u * blocks_for(J.width, kThreads) : 0u;
+                      return J.win.startup ? 6u * blocks_for(J.width, kStartupSites) : 0u;
                     }), kThreads, stream);
   return hipGetLastError();
 }
