@@ -19,6 +19,9 @@ Legs:
   pipelined  the headline `value`: K x B x rounds frames through the serving
            engine (batches of B, `drivers` batches in flight, so the device
            phases of one batch overlap the host work of the others);
+  pipelined_without_sha256  the same jobs with the output SHA-256 skipped
+           (outputs stay in HBM): what the device path and the host writer
+           walk sustain without the hashing; informational only;
   crops    16 crops of uneven size (sides 512-3000 px, aspect 0.5-2) as one
            batch (same machinery): Mpix/s and batch latency.
   routing  the hybrid's at-floor routing of frame 0's at-floor mask on the
@@ -101,6 +104,8 @@ def parse_args() -> argparse.Namespace:
                         help="concurrent batches in the pipelined and serving legs")
     parser.add_argument("--pipeline-rounds", type=int, default=8,
                         help="the pipelined leg runs steps x batch x this many frames")
+    parser.add_argument("--no-hashless", action="store_true",
+                        help="skip the pipelined leg without the output SHA-256")
     parser.add_argument("--no-crops", action="store_true", help="skip the uneven-crops leg")
     parser.add_argument("--no-end-to-end", action="store_true",
                         help="skip the host-buffer (PCIe-inclusive) leg")
@@ -392,10 +397,49 @@ def main() -> int:
     # host cores the pipelined leg kept busy (user + system time of every
     # thread of this process over the leg's wall time)
     pipe_cores = ((cpu_end.user - cpu_start.user) + (cpu_end.system - cpu_start.system)) / pipe_elapsed
-    del pipe_outs
     value = jobs_total * frame_mpix / pipe_elapsed
     log(f"pipelined batches: {jobs_total} frames in {pipe_elapsed:.3f} s "
         f"({value:.1f} Mpix/s, mean batch {pipe_stats['mean_batch']:.2f})")
+
+    # ---- the same pipeline without the output SHA-256 (the outputs stay in
+    # HBM): what the device path and the host writer walk sustain without
+    # the host hashing 137 MB per frame.  Informational, never the headline;
+    # every job's counters must equal the hashed leg's.
+    hashless = None
+    if not args.no_hashless:
+        with HipServingEngine(drivers=args.drivers, max_batch=args.batch, graph=True,
+                              autotune=True) as served:
+            for t in [served.submit_device(full.d_main[k % args.batch].address, height, width,
+                                           full.prepasses[k % args.batch],
+                                           pipe_outs[k % len(pipe_outs)].address, sha256=False)
+                      for k in range(min(jobs_total, 2 * args.drivers * args.batch))]:
+                served.wait(t)
+            synchronize()
+            served.reset_stats()
+            cpu_start = os.times()
+            start = time.perf_counter()
+            tickets = [served.submit_device(full.d_main[k % args.batch].address, height, width,
+                                            full.prepasses[k % args.batch],
+                                            pipe_outs[k % len(pipe_outs)].address, sha256=False)
+                       for k in range(jobs_total)]
+            counters = []
+            for t in tickets:
+                r = served.wait(t).replay
+                counters.append((r.attempted_pixels, r.written_pixels, r.public_rng_advances,
+                                 r.final_rng_state, r.changed_pixels))
+            elapsed = time.perf_counter() - start
+            cpu_end = os.times()
+        hashless = {
+            "note": "the pipelined leg's jobs without the output SHA-256 (outputs stay in HBM); "
+                    "never the headline value",
+            "jobs": jobs_total,
+            "value": round(jobs_total * frame_mpix / elapsed, 2), "unit": "Mpix/s",
+            "host_cores_busy": round(((cpu_end.user - cpu_start.user)
+                                      + (cpu_end.system - cpu_start.system)) / elapsed, 2),
+            "counters_equal_hashed": counters == [p[1:] for p in pipe_prints],
+        }
+        log(f"pipelined without the SHA-256: {hashless}")
+    del pipe_outs
 
     # ---- crops leg: 16 uneven crops as one batch
     crops = None
@@ -694,7 +738,8 @@ def main() -> int:
         bit_exact = (all(p == expected[0] for p in plain_results)
                      and all(prints[:checked] == expected for prints in batch_results)
                      and all(p == expected[k % args.batch] for k, p in enumerate(pipe_prints)
-                             if k % args.batch < checked))
+                             if k % args.batch < checked)
+                     and (hashless is None or hashless["counters_equal_hashed"]))
         if serving is not None:
             # every served job against the oracle on its pool frame
             oracle_prints = {}
@@ -827,6 +872,7 @@ def main() -> int:
             "host_cores_busy": round(pipe_cores, 2),
             "host_cores": len(os.sched_getaffinity(0)),
         },
+        "pipelined_without_sha256": hashless,
         "batched_single_call": batched_single_call,
         "plain": {
             "value": round(plain_value, 2),
