@@ -182,3 +182,33 @@ def test_wait_reports_pending_then_result(frames, oracle_shas):
             return
         assert engine.wait(t, timeout=120).replay.output_sha256 == oracle_shas[5]
     assert _abi.DICE_PENDING == 8
+
+
+def test_device_resident_jobs_without_the_hash(frames):
+    """Jobs whose frame and output stay in HBM, with the SHA-256 skipped (the
+    bench's hashless pipelined leg): the hash is empty, every counter equals
+    the oracle's, and the output read back from the device equals the
+    oracle's output array."""
+
+    from dice_amd.hip_backend.device import DeviceArray
+
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    picks = frames[:4]
+    mains = [DeviceArray.from_host(np.ascontiguousarray(m)) for m, _ in picks]
+    outs = [DeviceArray(m.shape[0] * m.shape[1] * 6) for m, _ in picks]
+    with HipServingEngine(drivers=2, max_batch=2) as engine:
+        tickets = [engine.submit_device(mains[i].address, m.shape[0], m.shape[1], p,
+                                        outs[i].address, sha256=False)
+                   for i, (m, p) in enumerate(picks)]
+        results = [engine.wait(t, timeout=120) for t in tickets]
+    for i, ((main, prepass), result) in enumerate(zip(picks, results)):
+        cpu = oracle_lib.process_job(main, prepass, params, prepass_params)
+        replay = result.replay
+        assert replay.output_sha256 in ("", None), i
+        assert (replay.attempted_pixels, replay.written_pixels, replay.public_rng_advances,
+                replay.final_rng_state, replay.changed_pixels) == (
+            cpu.attempted_pixels, cpu.written_pixels, cpu.public_rng_advances,
+            cpu.final_rng_state, cpu.changed_pixels), i
+        output = outs[i].download(np.empty((main.shape[0], main.shape[1], 3), np.uint16))
+        assert np.array_equal(output, cpu.output), i
