@@ -37,6 +37,16 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
 }
 
 // Loads the four uint16 lanes of one pixel with one 64-bit load.
+// The value of the other lane of this lane's pair (lanes 2m, 2m+1): a DPP
+// quad permutation [1, 0, 3, 2] on each half, no LDS round trip.  Both
+// lanes of the pair must be active.
+__device__ __forceinline__ double swap_pair(double v) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)bits, (int)(uint32_t)bits, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(bits >> 32), (int)(uint32_t)(bits >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
 __device__ __forceinline__ void load_pixel(const uint16_t* rgbi, size_t index,
                                            uint32_t lanes[4]) {
   uint2 q = *reinterpret_cast<const uint2*>(rgbi + index * 4);
@@ -949,7 +959,9 @@ struct SitesShared {
   // feature-record positions: the tile and a one-pixel ring (the cross
   // neighbours of its sites)
   static constexpr int PR = TY + 2, PC = TX + 2, NP = PR * PC;
-  static constexpr int kPerThread = (NP + Threads - 1) / Threads;
+  // record runs: kRun adjacent table columns of one table row (5: every run
+  // pair of a 16x64 tile fits one round of 512 lanes)
+  static constexpr int kRun = 5, kRowRuns = (PC + kRun - 1) / kRun, kRuns = PR * kRowRuns;
   float aux[AR][AC];
   float score[HR][HC];
   // the two planes sit 16 banks apart (of 32), so that the score and
@@ -961,7 +973,7 @@ struct SitesShared {
   float record_ratio[3][NP];  // f32(waux average / unrounded score average)
   uint8_t record_valid[NP];   // bit l: score average l > 0 (else the fallback)
   uint32_t needed_bits[(NP + 31) / 32];
-  uint16_t needed_list[NP];
+  uint16_t run_list[kRuns];
   RowParams rows[AR];
   float writer_coarse[TY];
   uint8_t writer_floor[TY];
@@ -1168,21 +1180,35 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
     __syncthreads();
 
-    // Compact the needed positions (each thread owns a run of them).
-    uint32_t needed = 0;
+    // The needed positions in runs: each (table row, aligned run of kRun
+    // table columns) with a needed position is one work item, listed in
+    // order by a block scan.
+    constexpr int kRun = Sh::kRun, kRowRuns = Sh::kRowRuns, kRuns = Sh::kRuns;
+    constexpr int kRunsPerThread = (kRuns + Threads - 1) / Threads;
+    // the needed bits of run (row r, run b), bit j = table column b * kRun + j
+    auto run_bits = [&](int r, int b) -> uint32_t {
+      const int f = r * PC + b * kRun;
+      const int w = f >> 5, o = f & 31;
+      uint32_t bits = S.needed_bits[w] >> o;
+      if (o > 32 - kRun && w + 1 < kWords) bits |= S.needed_bits[w + 1] << (32 - o);
+      const int n = min(kRun, PC - b * kRun);
+      return bits & ((1u << n) - 1u);
+    };
+    uint32_t runs = 0;
     {
-      const int first = tid * Sh::kPerThread;
+      const int first = tid * kRunsPerThread;
       uint32_t count = 0;
 #pragma unroll
-      for (int k = 0; k < Sh::kPerThread; ++k) {
-        const int i = first + k;
-        if (i < NP && ((S.needed_bits[i >> 5] >> (i & 31)) & 1u)) ++count;
+      for (int k = 0; k < kRunsPerThread; ++k) {
+        const int e = first + k;
+        if (e < kRuns && run_bits(e / kRowRuns, e % kRowRuns) != 0u) ++count;
       }
-      uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &needed);
+      uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &runs);
 #pragma unroll
-      for (int k = 0; k < Sh::kPerThread; ++k) {
-        const int i = first + k;
-        if (i < NP && ((S.needed_bits[i >> 5] >> (i & 31)) & 1u)) S.needed_list[at++] = (uint16_t)i;
+      for (int k = 0; k < kRunsPerThread; ++k) {
+        const int e = first + k;
+        if (e < kRuns && run_bits(e / kRowRuns, e % kRowRuns) != 0u)
+          S.run_list[at++] = (uint16_t)e;
       }
     }
     __syncthreads();
@@ -1190,49 +1216,66 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     // The averages of every needed position (streaming.py:236-256; the A
     // lanes divide by the unrounded binary64 W, reconstruction.py:365-371).
     // Position (pr, pc) has its patch at history rows pr..pr+8, columns
-    // pc..pc+8, read from one base address at fixed offsets.
-    auto averages = [&](const float* plane, int pr, int pc, double q[3]) {
+    // pc..pc+8.  A column scratch (reconstruction.py:187-237) is a function
+    // of its nine samples alone, so the positions of a run share theirs:
+    // the lane walks the run from its first needed position to its last
+    // with nine column scratches in registers, each step adding one column,
+    // and combines the nine for every needed position (reconstruction.py:
+    // 240-268).  Two lanes per run: the even lane averages the score plane,
+    // the odd lane the weighted-aux plane, and the pair swaps its sums; the
+    // even lane stores the f32 weights and the first ratio, the odd lane the
+    // other two.
+    auto column_at = [&](const float* c) {
+      return column_scratch(c[0], c[HC], c[2 * HC], c[3 * HC], c[4 * HC], c[5 * HC], c[6 * HC],
+                            c[7 * HC], c[8 * HC]);
+    };
+    for (uint32_t k = tid; k < 2 * runs; k += Threads) {
+      const bool odd = k & 1u;
+      const int e = S.run_list[k >> 1];
+      const int r = e / kRowRuns, b = e % kRowRuns;
+      uint32_t bits = run_bits(r, b);
+      const int lead = __builtin_ctz(bits);
+      bits >>= lead;
+      const int span = 32 - __builtin_clz(bits);  // 1 .. kRun
+      const int i0 = r * PC + b * kRun + lead;      // record index of the first
+      const float* base = (odd ? &S.waux[0][0] : &S.score[0][0]) + r * HC + b * kRun + lead;
       Column col[9];
 #pragma unroll
-      for (int x = 0; x < 9; ++x) {
-        const float* c = plane + pr * HC + pc + x;
-        col[x] = column_scratch(c[0], c[HC], c[2 * HC], c[3 * HC], c[4 * HC], c[5 * HC],
-                                c[6 * HC], c[7 * HC], c[8 * HC]);
-      }
-      combine_columns(col, q);
-    };
-    auto write_record = [&](int i, const double unrounded[3], const double numerator[3]) {
-      uint8_t valid = 0;
+      for (int x = 0; x < 9; ++x) col[x] = column_at(base + x);
 #pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        S.record_w[l][i] = (float)unrounded[l];
-        const bool positive = unrounded[l] > 0.0;
-        S.record_ratio[l][i] = positive ? (float)(numerator[l] / unrounded[l]) : 0.0f;
-        valid |= positive ? (uint8_t)(1u << l) : (uint8_t)0;
-      }
-      S.record_valid[i] = valid;
-    };
-    if (2 * needed <= (uint32_t)Threads) {
-      // Two lanes per position when the positions fit one round: the even
-      // lane averages the score plane, the odd lane the weighted-aux plane,
-      // and the pair swaps its sums.
-      const uint32_t k = tid >> 1;
-      const bool odd = tid & 1;
-      const int i = k < needed ? S.needed_list[k] : 0;
-      const float* plane = odd ? &S.waux[0][0] : &S.score[0][0];
-      double mine[3] = {0.0, 0.0, 0.0};
-      if (k < needed) averages(plane, i / PC, i % PC, mine);
-      double other[3];
+      for (int j = 0; j < kRun; ++j) {
+        if (j >= span) break;  // the pair walks the same run
+        if (j > 0) {
 #pragma unroll
-      for (int l = 0; l < 3; ++l) other[l] = __shfl_xor(mine[l], 1, 64);
-      if (k < needed && !odd) write_record(i, mine, other);
-    } else {
-      for (uint32_t k = tid; k < needed; k += Threads) {
-        const int i = S.needed_list[k];
-        double unrounded[3], numerator[3];
-        averages(&S.score[0][0], i / PC, i % PC, unrounded);
-        averages(&S.waux[0][0], i / PC, i % PC, numerator);
-        write_record(i, unrounded, numerator);
+          for (int x = 0; x < 8; ++x) col[x] = col[x + 1];
+          col[8] = column_at(base + j + 8);
+        }
+        double mine[3], other[3];
+        combine_columns(col, mine);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) other[l] = swap_pair(mine[l]);
+        if ((bits >> j) & 1u) {
+          const double* unrounded = odd ? other : mine;
+          const double* numerator = odd ? mine : other;
+          const int i = i0 + j;
+          auto ratio = [&](int l) {
+            S.record_ratio[l][i] =
+                unrounded[l] > 0.0 ? (float)(numerator[l] / unrounded[l]) : 0.0f;
+          };
+          if (!odd) {
+            uint8_t valid = 0;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              S.record_w[l][i] = (float)unrounded[l];
+              valid |= unrounded[l] > 0.0 ? (uint8_t)(1u << l) : (uint8_t)0;
+            }
+            S.record_valid[i] = valid;
+            ratio(0);
+          } else {
+            ratio(1);
+            ratio(2);
+          }
+        }
       }
     }
     __syncthreads();
