@@ -973,7 +973,8 @@ struct SitesShared {
   float record_ratio[3][NP];  // f32(waux average / unrounded score average)
   uint8_t record_valid[NP];   // bit l: score average l > 0 (else the fallback)
   uint32_t needed_bits[(NP + 31) / 32];
-  uint16_t run_list[kRuns];
+  // needed positions (at most Threads / 2 of them) or record runs
+  uint16_t work_list[kRuns > Threads / 2 ? kRuns : Threads / 2];
   RowParams rows[AR];
   float writer_coarse[TY];
   uint8_t writer_floor[TY];
@@ -1180,9 +1181,8 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
     __syncthreads();
 
-    // The needed positions in runs: each (table row, aligned run of kRun
-    // table columns) with a needed position is one work item, listed in
-    // order by a block scan.
+    // The needed positions by runs: a run is an aligned group of kRun
+    // adjacent table columns of one table row.
     constexpr int kRun = Sh::kRun, kRowRuns = Sh::kRowRuns, kRuns = Sh::kRuns;
     constexpr int kRunsPerThread = (kRuns + Threads - 1) / Threads;
     // the needed bits of run (row r, run b), bit j = table column b * kRun + j
@@ -1194,21 +1194,44 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       const int n = min(kRun, PC - b * kRun);
       return bits & ((1u << n) - 1u);
     };
-    uint32_t runs = 0;
+    // One block scan counts both: runs in the low 16 bits, positions in the
+    // high 16 bits.  Where the positions fit one lane pair each in a single
+    // round, the list holds positions (a run's lanes would walk longer);
+    // otherwise it holds runs.
+    uint32_t runs = 0, needed = 0;
     {
       const int first = tid * kRunsPerThread;
       uint32_t count = 0;
 #pragma unroll
       for (int k = 0; k < kRunsPerThread; ++k) {
         const int e = first + k;
-        if (e < kRuns && run_bits(e / kRowRuns, e % kRowRuns) != 0u) ++count;
+        if (e < kRuns) {
+          const uint32_t bits = run_bits(e / kRowRuns, e % kRowRuns);
+          if (bits != 0u) count += 1u + ((uint32_t)__builtin_popcount(bits) << 16);
+        }
       }
-      uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &runs);
+      uint32_t total;
+      const uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &total);
+      runs = total & 0xFFFFu;
+      needed = total >> 16;
+      const bool by_position = 2 * needed <= (uint32_t)Threads;
+      uint32_t slot = by_position ? at >> 16 : at & 0xFFFFu;
 #pragma unroll
       for (int k = 0; k < kRunsPerThread; ++k) {
         const int e = first + k;
-        if (e < kRuns && run_bits(e / kRowRuns, e % kRowRuns) != 0u)
-          S.run_list[at++] = (uint16_t)e;
+        if (e >= kRuns) continue;
+        const int r = e / kRowRuns, b = e % kRowRuns;
+        uint32_t bits = run_bits(r, b);
+        if (bits == 0u) continue;
+        if (!by_position) {
+          S.work_list[slot++] = (uint16_t)e;
+          continue;
+        }
+        while (bits != 0u) {
+          const int j = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          S.work_list[slot++] = (uint16_t)(r * PC + b * kRun + j);
+        }
       }
     }
     __syncthreads();
@@ -1216,65 +1239,83 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     // The averages of every needed position (streaming.py:236-256; the A
     // lanes divide by the unrounded binary64 W, reconstruction.py:365-371).
     // Position (pr, pc) has its patch at history rows pr..pr+8, columns
-    // pc..pc+8.  A column scratch (reconstruction.py:187-237) is a function
-    // of its nine samples alone, so the positions of a run share theirs:
-    // the lane walks the run from its first needed position to its last
-    // with nine column scratches in registers, each step adding one column,
-    // and combines the nine for every needed position (reconstruction.py:
-    // 240-268).  Two lanes per run: the even lane averages the score plane,
-    // the odd lane the weighted-aux plane, and the pair swaps its sums; the
-    // even lane stores the f32 weights and the first ratio, the odd lane the
-    // other two.
+    // pc..pc+8.  Two lanes per position or run: the even lane averages the
+    // score plane, the odd lane the weighted-aux plane, and the pair swaps
+    // its sums; the even lane stores the f32 weights and the first ratio,
+    // the odd lane the other two.
     auto column_at = [&](const float* c) {
       return column_scratch(c[0], c[HC], c[2 * HC], c[3 * HC], c[4 * HC], c[5 * HC], c[6 * HC],
                             c[7 * HC], c[8 * HC]);
     };
-    for (uint32_t k = tid; k < 2 * runs; k += Threads) {
-      const bool odd = k & 1u;
-      const int e = S.run_list[k >> 1];
-      const int r = e / kRowRuns, b = e % kRowRuns;
-      uint32_t bits = run_bits(r, b);
-      const int lead = __builtin_ctz(bits);
-      bits >>= lead;
-      const int span = 32 - __builtin_clz(bits);  // 1 .. kRun
-      const int i0 = r * PC + b * kRun + lead;      // record index of the first
-      const float* base = (odd ? &S.waux[0][0] : &S.score[0][0]) + r * HC + b * kRun + lead;
-      Column col[9];
+    auto store_record = [&](int i, bool odd, const double mine[3], const double other[3]) {
+      const double* unrounded = odd ? other : mine;
+      const double* numerator = odd ? mine : other;
+      auto ratio = [&](int l) {
+        S.record_ratio[l][i] = unrounded[l] > 0.0 ? (float)(numerator[l] / unrounded[l]) : 0.0f;
+      };
+      if (!odd) {
+        uint8_t valid = 0;
 #pragma unroll
-      for (int x = 0; x < 9; ++x) col[x] = column_at(base + x);
-#pragma unroll
-      for (int j = 0; j < kRun; ++j) {
-        if (j >= span) break;  // the pair walks the same run
-        if (j > 0) {
-#pragma unroll
-          for (int x = 0; x < 8; ++x) col[x] = col[x + 1];
-          col[8] = column_at(base + j + 8);
+        for (int l = 0; l < 3; ++l) {
+          S.record_w[l][i] = (float)unrounded[l];
+          valid |= unrounded[l] > 0.0 ? (uint8_t)(1u << l) : (uint8_t)0;
         }
+        S.record_valid[i] = valid;
+        ratio(0);
+      } else {
+        ratio(1);
+        ratio(2);
+      }
+    };
+    if (2 * needed <= (uint32_t)Threads) {
+      // one lane pair per position, in one round
+      const uint32_t k = tid >> 1;
+      const bool odd = tid & 1;
+      if (k < needed) {
+        const int i = S.work_list[k];
+        const float* base = (odd ? &S.waux[0][0] : &S.score[0][0]) + (i / PC) * HC + i % PC;
+        Column col[9];
+#pragma unroll
+        for (int x = 0; x < 9; ++x) col[x] = column_at(base + x);
         double mine[3], other[3];
         combine_columns(col, mine);
 #pragma unroll
         for (int l = 0; l < 3; ++l) other[l] = swap_pair(mine[l]);
-        if ((bits >> j) & 1u) {
-          const double* unrounded = odd ? other : mine;
-          const double* numerator = odd ? mine : other;
-          const int i = i0 + j;
-          auto ratio = [&](int l) {
-            S.record_ratio[l][i] =
-                unrounded[l] > 0.0 ? (float)(numerator[l] / unrounded[l]) : 0.0f;
-          };
-          if (!odd) {
-            uint8_t valid = 0;
+        store_record(i, odd, mine, other);
+      }
+    } else {
+      // One lane pair per run: a column scratch (reconstruction.py:187-237)
+      // is a function of its nine samples alone, so the positions of a run
+      // share theirs.  The pair walks the run from its first needed position
+      // to its last with nine column scratches in registers, each step
+      // adding one column, and combines the nine for every needed position
+      // (reconstruction.py:240-268).
+      for (uint32_t k = tid; k < 2 * runs; k += Threads) {
+        const bool odd = k & 1u;
+        const int e = S.work_list[k >> 1];
+        const int r = e / kRowRuns, b = e % kRowRuns;
+        uint32_t bits = run_bits(r, b);
+        const int lead = __builtin_ctz(bits);
+        bits >>= lead;
+        const int span = 32 - __builtin_clz(bits);  // 1 .. kRun
+        const int i0 = r * PC + b * kRun + lead;      // record index of the first
+        const float* base = (odd ? &S.waux[0][0] : &S.score[0][0]) + r * HC + b * kRun + lead;
+        Column col[9];
 #pragma unroll
-            for (int l = 0; l < 3; ++l) {
-              S.record_w[l][i] = (float)unrounded[l];
-              valid |= unrounded[l] > 0.0 ? (uint8_t)(1u << l) : (uint8_t)0;
-            }
-            S.record_valid[i] = valid;
-            ratio(0);
-          } else {
-            ratio(1);
-            ratio(2);
+        for (int x = 0; x < 9; ++x) col[x] = column_at(base + x);
+#pragma unroll
+        for (int j = 0; j < kRun; ++j) {
+          if (j >= span) break;  // the pair walks the same run
+          if (j > 0) {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) col[x] = col[x + 1];
+            col[8] = column_at(base + j + 8);
           }
+          double mine[3], other[3];
+          combine_columns(col, mine);
+#pragma unroll
+          for (int l = 0; l < 3; ++l) other[l] = swap_pair(mine[l]);
+          if ((bits >> j) & 1u) store_record(i0 + j, odd, mine, other);
         }
       }
     }
