@@ -1052,30 +1052,63 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     const float floor = P.score_floor;
     const uint16_t* list = J.site_lists + (size_t)t * (TY * TX);
 
-    for (int r = tid; r < AR; r += Threads) S.rows[r] = load_row_params(J, clampi(y0 - 5 + r, 0, H - 1));
-    for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
-    // Per tile row (one lane each, in the first wave): its first entry in
-    // the site list (an exclusive scan of the row counts), its first place
-    // in the site array, and its writer stage.
+    // The tile's tables in registers first: its halo rows' parameters (one
+    // row per thread) and, per tile row (one lane each, in the first wave),
+    // the row's site count, its first place in the site array and its
+    // writer stage.
+    static_assert(AR <= Threads, "one halo row per thread");
     static_assert(TY <= 64, "one lane per tile row");
+    RowParams row_params{};
+    if (tid < AR) row_params = load_row_params(J, clampi(y0 - 5 + tid, 0, H - 1));
+    uint32_t head_count = 0, head_offset = 0;
+    float head_coarse = 0.0f;
+    bool head_floor = false;
+    if (tid < TY) {
+      const size_t seg = (size_t)(y0 + tid) * J.tiles_x + tile_x;
+      const int y = min(y0 + tid, H - 1);
+      head_count = J.seg_count[seg];
+      head_offset = J.seg_offset[seg];
+      head_coarse = J.stages.coarse[y + 6];
+      head_floor = P.gate_primary || J.stages.gate_secondary[y + 6];
+    }
+    // Then the raw halo pixels this thread stages: requested after the
+    // tables, so that waiting for the tables does not wait for them, and
+    // in flight through the tables' stores, the strip shift and the site
+    // marks.  Below a staged tile, from aux row AR - TY - 1 on: the first
+    // row whose working RGB the weighted-RGB rows still need (its aux,
+    // rewritten, is the value already there).
+    constexpr int kStageRounds = (AR * AC + Threads - 1) / Threads;
+    const int stage_from = staged ? (AR - TY - 1) * AC : 0;
+    uint2 halo_raw[kStageRounds];
+#pragma unroll
+    for (int u = 0; u < kStageRounds; ++u) {
+      const int i = stage_from + tid + u * Threads;
+      halo_raw[u] = make_uint2(0u, 0u);
+      if (i < AR * AC) {
+        const int r = i / AC, c = i % AC;
+        const size_t pixel = (size_t)clampi(y0 - 5 + r, 0, H - 1) * W + clampi(x0 - 6 + c, 0, W - 1);
+        halo_raw[u] = reinterpret_cast<const uint2*>(J.rgbi)[pixel];
+      }
+    }
+
+    if (tid < AR) S.rows[tid] = row_params;
+    for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
+    // the rows' first entries in the site list: an exclusive scan of the
+    // row counts
     if (tid < 64) {
       const int i = tid;
-      uint32_t count = 0;
-      if (i < TY) {
-        const size_t seg = (size_t)(y0 + i) * J.tiles_x + tile_x;
-        const int y = min(y0 + i, H - 1);
-        count = J.seg_count[seg];
-        S.row_offset[i] = J.seg_offset[seg];
-        S.writer_coarse[i] = J.stages.coarse[y + 6];
-        S.writer_floor[i] = (P.gate_primary || J.stages.gate_secondary[y + 6]) ? 1 : 0;
-      }
-      uint32_t inclusive = count;
+      uint32_t inclusive = head_count;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const uint32_t t = __shfl_up(inclusive, d, 64);
         if (i >= d) inclusive += t;
       }
-      if (i < TY) S.row_start[i] = inclusive - count;
+      if (i < TY) {
+        S.row_start[i] = inclusive - head_count;
+        S.row_offset[i] = head_offset;
+        S.writer_coarse[i] = head_coarse;
+        S.writer_floor[i] = head_floor ? 1 : 0;
+      }
     }
     // A tile below a staged one takes its top halo rows from the bottom rows
     // the tile above left (rows [TY, R) -> [0, R - TY); the two ranges are
@@ -1114,16 +1147,24 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
 
     // Working lanes and aux on the clamped halo; working RGB staged for the
-    // weighted-RGB history.  Below a staged tile, from aux row AR - TY - 1
-    // on: the first row whose working RGB the weighted-RGB rows still need
-    // (its aux, rewritten, is the value already there).
-    for (int i = tid + (staged ? (AR - TY - 1) * AC : 0); i < AR * AC; i += Threads) {
+    // weighted-RGB history.  Every round's table gathers are requested
+    // before any is used.
+    float halo_work[kStageRounds][4];
+#pragma unroll
+    for (int u = 0; u < kStageRounds; ++u) {
+      const uint2 q = halo_raw[u];
+      halo_work[u][0] = J.lut[q.x & 0xFFFFu];
+      halo_work[u][1] = J.lut[q.x >> 16];
+      halo_work[u][2] = J.lut[q.y & 0xFFFFu];
+      halo_work[u][3] = J.lut[q.y >> 16];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageRounds; ++u) {
+      const int i = stage_from + tid + u * Threads;
+      if (i >= AR * AC) continue;
       const int r = i / AC, c = i % AC;
-      const int ly = y0 - 5 + r, lx = x0 - 6 + c;
-      uint32_t lanes[4];
-      load_pixel(J.rgbi, (size_t)clampi(ly, 0, H - 1) * W + clampi(lx, 0, W - 1), lanes);
-      const float work0 = J.lut[lanes[0]], work1 = J.lut[lanes[1]];
-      const float work2 = J.lut[lanes[2]], work3 = J.lut[lanes[3]];
+      const float work0 = halo_work[u][0], work1 = halo_work[u][1];
+      const float work2 = halo_work[u][2], work3 = halo_work[u][3];
       const float visible =
           P.visible_channel == 0 ? work0 : (P.visible_channel == 1 ? work1 : work2);
       const RowParams& rp = S.rows[r];
