@@ -975,6 +975,7 @@ struct SitesShared {
   uint32_t needed_bits[(NP + 31) / 32];
   // needed positions (at most Threads / 2 of them) or record runs
   uint16_t work_list[kRuns > Threads / 2 ? kRuns : Threads / 2];
+  uint32_t run_totals;  // runs (low 16 bits) and needed positions (high 16 bits)
   RowParams rows[AR];
   float writer_coarse[TY];
   uint8_t writer_floor[TY];
@@ -1071,10 +1072,18 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       head_coarse = J.stages.coarse[y + 6];
       head_floor = P.gate_primary || J.stages.gate_secondary[y + 6];
     }
+    // The tile's site list entries this thread marks.
+    constexpr int kListRounds = (TY * TX + Threads - 1) / Threads;
+    uint32_t listed[kListRounds];
+#pragma unroll
+    for (int u = 0; u < kListRounds; ++u) {
+      const uint32_t q = (uint32_t)tid + (uint32_t)u * Threads;
+      listed[u] = q < total ? (uint32_t)list[q] : 0u;
+    }
     // Then the raw halo pixels this thread stages: requested after the
-    // tables, so that waiting for the tables does not wait for them, and
-    // in flight through the tables' stores, the strip shift and the site
-    // marks.  Below a staged tile, from aux row AR - TY - 1 on: the first
+    // tables and the list, so that waiting for those does not wait for
+    // them, and in flight through the tables' stores, the strip shift and
+    // the site marks.  Below a staged tile, from aux row AR - TY - 1 on: the first
     // row whose working RGB the weighted-RGB rows still need (its aux,
     // rewritten, is the value already there).
     constexpr int kStageRounds = (AR * AC + Threads - 1) / Threads;
@@ -1092,7 +1101,6 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
 
     if (tid < AR) S.rows[tid] = row_params;
-    for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
     // the rows' first entries in the site list: an exclusive scan of the
     // row counts
     if (tid < 64) {
@@ -1108,6 +1116,27 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
         S.row_offset[i] = head_offset;
         S.writer_coarse[i] = head_coarse;
         S.writer_floor[i] = head_floor ? 1 : 0;
+      }
+    }
+    // The record positions the sites need (record index = (ty+1)*PC + tx+1),
+    // into bits the previous tile's site phase cleared.  Neighbours outside
+    // the frame's columns are all-zero records, not averages
+    // (streaming.py:259-308).
+#pragma unroll
+    for (int u = 0; u < kListRounds; ++u) {
+      const uint32_t q = (uint32_t)tid + (uint32_t)u * Threads;
+      if (q >= total) continue;
+      const int li = (int)listed[u];
+      const int ty = li / TX, tx = li % TX;
+      const int x = x0 + tx;
+      const int centre = (ty + 1) * PC + tx + 1;
+      auto mark = [&](int i) { atomicOr(&S.needed_bits[i >> 5], 1u << (i & 31)); };
+      mark(centre);
+      if (P.cross_neighbor) {
+        if (x > 0) mark(centre - 1);
+        if (x < W - 1) mark(centre + 1);
+        mark(centre - PC);
+        mark(centre + PC);
       }
     }
     // A tile below a staged one takes its top halo rows from the bottom rows
@@ -1127,24 +1156,6 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       shift_up(&S.wrgb[2][0][0], RR, RC);
     }
     __syncthreads();
-
-    // The record positions the sites need (record index = (ty+1)*PC + tx+1).
-    // Neighbours outside the frame's columns are all-zero records, not
-    // averages (streaming.py:259-308).
-    for (uint32_t q = tid; q < total; q += Threads) {
-      const int li = list[q];
-      const int ty = li / TX, tx = li % TX;
-      const int x = x0 + tx;
-      const int centre = (ty + 1) * PC + tx + 1;
-      auto mark = [&](int i) { atomicOr(&S.needed_bits[i >> 5], 1u << (i & 31)); };
-      mark(centre);
-      if (P.cross_neighbor) {
-        if (x > 0) mark(centre - 1);
-        if (x < W - 1) mark(centre + 1);
-        mark(centre - PC);
-        mark(centre + PC);
-      }
-    }
 
     // Working lanes and aux on the clamped halo; working RGB staged for the
     // weighted-RGB history.  Every round's table gathers are requested
@@ -1220,12 +1231,9 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
         }
       }
     }
-    __syncthreads();
-
     // The needed positions by runs: a run is an aligned group of kRun
     // adjacent table columns of one table row.
     constexpr int kRun = Sh::kRun, kRowRuns = Sh::kRowRuns, kRuns = Sh::kRuns;
-    constexpr int kRunsPerThread = (kRuns + Threads - 1) / Threads;
     // the needed bits of run (row r, run b), bit j = table column b * kRun + j
     auto run_bits = [&](int r, int b) -> uint32_t {
       const int f = r * PC + b * kRun;
@@ -1235,30 +1243,35 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
       const int n = min(kRun, PC - b * kRun);
       return bits & ((1u << n) - 1u);
     };
-    // One block scan counts both: runs in the low 16 bits, positions in the
-    // high 16 bits.  Where the positions fit one lane pair each in a single
-    // round, the list holds positions (a run's lanes would walk longer);
-    // otherwise it holds runs.
-    uint32_t runs = 0, needed = 0;
-    {
-      const int first = tid * kRunsPerThread;
+    // The first wave lists them, after its share of the pass above, with
+    // one wave-level scan that counts both: runs in the low 16 bits,
+    // positions in the high 16 bits.  Where the positions fit one lane pair
+    // each in a single round, the list holds positions (a run's lanes would
+    // walk longer); otherwise it holds runs.
+    if (tid < 64) {
+      constexpr int kPerLane = (kRuns + 63) / 64;
+      const int first = tid * kPerLane;
       uint32_t count = 0;
 #pragma unroll
-      for (int k = 0; k < kRunsPerThread; ++k) {
+      for (int k = 0; k < kPerLane; ++k) {
         const int e = first + k;
         if (e < kRuns) {
           const uint32_t bits = run_bits(e / kRowRuns, e % kRowRuns);
           if (bits != 0u) count += 1u + ((uint32_t)__builtin_popcount(bits) << 16);
         }
       }
-      uint32_t total;
-      const uint32_t at = block_exclusive_scan<Threads>(count, S.wave_sums, &total);
-      runs = total & 0xFFFFu;
-      needed = total >> 16;
-      const bool by_position = 2 * needed <= (uint32_t)Threads;
+      uint32_t inclusive = count;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inclusive, d, 64);
+        if (tid >= d) inclusive += t;
+      }
+      const uint32_t totals = __shfl(inclusive, 63, 64);
+      const uint32_t at = inclusive - count;
+      const bool by_position = 2 * (totals >> 16) <= (uint32_t)Threads;
       uint32_t slot = by_position ? at >> 16 : at & 0xFFFFu;
 #pragma unroll
-      for (int k = 0; k < kRunsPerThread; ++k) {
+      for (int k = 0; k < kPerLane; ++k) {
         const int e = first + k;
         if (e >= kRuns) continue;
         const int r = e / kRowRuns, b = e % kRowRuns;
@@ -1274,8 +1287,10 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
           S.work_list[slot++] = (uint16_t)(r * PC + b * kRun + j);
         }
       }
+      if (tid == 0) S.run_totals = totals;
     }
     __syncthreads();
+    const uint32_t runs = S.run_totals & 0xFFFFu, needed = S.run_totals >> 16;
 
     // The averages of every needed position (streaming.py:236-256; the A
     // lanes divide by the unrounded binary64 W, reconstruction.py:365-371).
@@ -1362,6 +1377,10 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
     __syncthreads();
 
+    // The needed bits are read for the last time above: clear them for the
+    // next tile's marks (a barrier follows every tile).
+    for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
+
     // One site per thread.  Eligibility already implies the driver gate
     // passes (reconstruction.py:890-904 re-tests the same row gate and the
     // same point score), so every listed centre is attempted.
@@ -1424,6 +1443,8 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     }
   };
   constexpr int kStrip = sites_strip(TY);
+  // the needed bits start clear; each tile's site phase clears them again
+  for (int i = threadIdx.x; i < kWords; i += Threads) S.needed_bits[i] = 0u;
   for_each_unit_dynamic(set, [&](const JobDev& J, const int unit) {
     const int strips = sites_strips(TY, J.win.trow_n);
     const int strip_units = strips * J.tiles_x;
