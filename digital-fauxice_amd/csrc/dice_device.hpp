@@ -183,9 +183,11 @@ DICE_HD WriterRecord writer_record(const Site& site, bool attempted, const Write
       const double cand = site.cand[c];
       const double source = (double)site.orig[c];
       // where x + delta(u) crosses t in real arithmetic: the estimate each
-      // threshold search starts from
+      // threshold search starts from (only the search's path depends on
+      // it, so one reciprocal serves the three)
+      const double per_delta = 1.0 / (envelope * span);
       auto crossing = [&](double x, double t) {
-        const double state = ((t - x) / (envelope * span) + 0.5) / kNikonNormalization - 1.0;
+        const double state = ((t - x) * per_delta + 0.5) * (1.0 / kNikonNormalization) - 1.0;
         return mirrored ? (double)kLcgMask - state : state;
       };
       // acceptance low < cd + delta < high holds for u in [lo, hi); when
