@@ -101,7 +101,9 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--device-chain", action="store_true",
                         help="batched legs run each job's writer chain on one device wavefront")
     parser.add_argument("--drivers", type=int, default=2,
-                        help="concurrent batches in the pipelined and serving legs")
+                        help="concurrent batches in the pipelined leg")
+    parser.add_argument("--pipeline-batch", type=int, default=None,
+                        help="largest batch the pipelined leg's engine forms (default: --batch)")
     parser.add_argument("--pipeline-rounds", type=int, default=8,
                         help="the pipelined leg runs steps x batch x this many frames")
     parser.add_argument("--no-hashless", action="store_true",
@@ -364,18 +366,19 @@ def main() -> int:
     from dice_amd.hip_backend.serving import HipServingEngine
 
     jobs_total = args.steps * args.batch * max(1, args.pipeline_rounds)
+    pipe_batch = args.pipeline_batch or args.batch
     # output buffers are recycled: a batch takes jobs in submission order and at
     # most `drivers` batches run at once, so a buffer is free again long before
     # its next job starts
     pipe_outs = [DeviceArray(height * width * 6)
-                 for _ in range(min(jobs_total, max(64, 4 * args.drivers * args.batch)))]
-    with HipServingEngine(drivers=args.drivers, max_batch=args.batch, graph=True,
+                 for _ in range(min(jobs_total, max(64, 4 * args.drivers * pipe_batch)))]
+    with HipServingEngine(drivers=args.drivers, max_batch=pipe_batch, graph=True,
                           device_chain=args.device_chain,
                           autotune=True) as served:
         warm = [served.submit_device(full.d_main[k % args.batch].address, height, width,
                                      full.prepasses[k % args.batch],
                                      pipe_outs[k % len(pipe_outs)].address)
-                for k in range(min(jobs_total, 2 * args.drivers * args.batch))]
+                for k in range(min(jobs_total, 2 * args.drivers * pipe_batch))]
         for t in warm:
             served.wait(t)
         synchronize()
@@ -407,12 +410,12 @@ def main() -> int:
     # every job's counters must equal the hashed leg's.
     hashless = None
     if not args.no_hashless:
-        with HipServingEngine(drivers=args.drivers, max_batch=args.batch, graph=True,
+        with HipServingEngine(drivers=args.drivers, max_batch=pipe_batch, graph=True,
                               autotune=True) as served:
             for t in [served.submit_device(full.d_main[k % args.batch].address, height, width,
                                            full.prepasses[k % args.batch],
                                            pipe_outs[k % len(pipe_outs)].address, sha256=False)
-                      for k in range(min(jobs_total, 2 * args.drivers * args.batch))]:
+                      for k in range(min(jobs_total, 2 * args.drivers * pipe_batch))]:
                 served.wait(t)
             synchronize()
             served.reset_stats()
@@ -852,11 +855,12 @@ def main() -> int:
             "workload": (f"{jobs_total} full 5782x3946 RGBI16 frames (steps {args.steps} x batch "
                          f"{args.batch} x {max(1, args.pipeline_rounds)} rounds; {args.batch} "
                          "distinct frames with 400 disks + 40 scratches each), resident in HBM, "
-                         f"through the batching engine: batches of {args.batch}, {args.drivers} "
+                         f"through the batching engine: batches of up to {pipe_batch}, {args.drivers} "
                          "concurrent drivers, persistent launches over each batch, graph replay, "
                          "autotuned tiles; SHA-256 of every output"),
             "frames_timed": jobs_total,
             "drivers": args.drivers,
+            "pipeline_batch": pipe_batch,
             "writer_chain": "device wavefront" if args.device_chain else "host core",
             "global_batch": args.batch,
             "height": height,
