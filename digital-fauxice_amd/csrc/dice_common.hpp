@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/dice_hip.h"
@@ -40,6 +41,19 @@ inline int32_t set_error(int32_t code, const char* fmt, ...) {
     int32_t _rc = (expr);               \
     if (_rc != DICE_OK) return _rc;     \
   } while (0)
+
+// Waits for an event without holding a core.  hipEventSynchronize and
+// hipStreamSynchronize keep the calling thread running for the whole wait,
+// blocking-sync events included (measured on MI355X under ROCm 7: a 200 ms
+// wait costs 200 ms of CPU), which takes cores from the writer chains and
+// the hashes; polling with short sleeps costs about 1 % of the wait.
+inline hipError_t wait_event(hipEvent_t event) {
+  for (int polls = 0;; ++polls) {
+    const hipError_t e = hipEventQuery(event);
+    if (e != hipErrorNotReady) return e;
+    std::this_thread::sleep_for(std::chrono::microseconds(polls < 32 ? 20 : 100));
+  }
+}
 
 inline double now_seconds() {
   return std::chrono::duration<double>(

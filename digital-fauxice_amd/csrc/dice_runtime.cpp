@@ -358,7 +358,7 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
 // Waits for everything queued on `stream` without spinning a core.
 int32_t Runner::wait_stream(hipStream_t stream) {
   HIP_TRY(hipEventRecord(sync_event_, stream));
-  HIP_TRY(hipEventSynchronize(sync_event_));
+  HIP_TRY(wait_event(sync_event_));
   return DICE_OK;
 }
 
@@ -695,7 +695,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       if (outcome[j].code != DICE_OK) return set_error(outcome[j].code, "%s",
                                                       outcome[j].error.c_str());
       HIP_TRY(hipSetDevice(device_));
-      if (!on_caller) HIP_TRY(hipEventSynchronize(w.tail));
+      if (!on_caller) HIP_TRY(wait_event(w.tail));
       const uint32_t flags = *reinterpret_cast<uint32_t*>(w.h_small.as<uint8_t>() + 64);
       DICE_TRY(flag_error(flags));
       if (cancel[j].load() || R.cb.cancelled()) return cancelled_error();
@@ -888,7 +888,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         dice_replay_result* res = R.result;
         uint32_t state = R.rng_state;
         const double f0 = now_seconds();
-        HIP_TRY(hipEventSynchronize(w.startup_fetched));
+        HIP_TRY(wait_event(w.startup_fetched));
         fetch_wait += now_seconds() - f0;
         state = walk_startup(records, W, state, states, res);
         HIP_TRY(hipMemcpyAsync(J.su_site_state, states, su * 4, hipMemcpyHostToDevice, stream_));
@@ -902,7 +902,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
           const uint32_t s1 = row_start[band_row(H, bands, b + 1)];
           if (b + kFetchAhead < bands) DICE_TRY(fetch_band(J, w, b + kFetchAhead));
           const double f0 = now_seconds();
-          HIP_TRY(hipEventSynchronize(w.fetched[b]));
+          HIP_TRY(wait_event(w.fetched[b]));
           fetch_wait += now_seconds() - f0;
           const double k0 = now_seconds();
           state = chain_records_grouped(main, s0, s1 - s0, state, group_states, advances);
@@ -972,7 +972,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
           }
           const double c0 = now_seconds();
           band_waits[j] += c0 - p0;
-          HIP_TRY(hipEventSynchronize(w.copied[b]));
+          HIP_TRY(wait_event(w.copied[b]));
           const double c1 = now_seconds();
           if (trace_enabled())
             fprintf(stderr, "dice-trace band job=%d b=%d published=%.2f copied=%.2f\n", j, b,

@@ -96,7 +96,8 @@ struct Workspace {
   }
   int32_t init_events() {
     if (events) return DICE_OK;
-    // worker threads wait on these: block instead of spinning, so waiting
+    // worker threads wait on these through wait_event (polling with sleeps:
+    // hipEventSynchronize spins even on blocking-sync events), so waiting
     // threads leave the cores to the chains and hashes
     const unsigned flags = hipEventDisableTiming | hipEventBlockingSync;
     for (hipEvent_t* set : {band, fetched, copied})
