@@ -46,12 +46,19 @@ inline int32_t set_error(int32_t code, const char* fmt, ...) {
 // hipStreamSynchronize keep the calling thread running for the whole wait,
 // blocking-sync events included (measured on MI355X under ROCm 7: a 200 ms
 // wait costs 200 ms of CPU), which takes cores from the writer chains and
-// the hashes; polling with short sleeps costs about 1 % of the wait.
+// the hashes.  Here the first 200 us poll and yield (a sleep lasts at least
+// one timer tick, too long for the streaming path's sub-millisecond block
+// waits), then the thread sleeps 50 us between polls, about 1 % of a long
+// wait in CPU.
 inline hipError_t wait_event(hipEvent_t event) {
-  for (int polls = 0;; ++polls) {
+  const auto start = std::chrono::steady_clock::now();
+  for (;;) {
     const hipError_t e = hipEventQuery(event);
     if (e != hipErrorNotReady) return e;
-    std::this_thread::sleep_for(std::chrono::microseconds(polls < 32 ? 20 : 100));
+    if (std::chrono::steady_clock::now() - start < std::chrono::microseconds(200))
+      std::this_thread::yield();
+    else
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
 }
 
