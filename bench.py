@@ -39,7 +39,8 @@ Legs:
            against the oracle in the CPU leg.
   serving  Poisson arrivals (70 % 1024-2048 px crops, 30 % full frames) at
            80 % of the batched capacity into the serving engine (continuous
-           batching, HBM-budget admission, two drivers): p50/p99 latency and
+           batching, HBM-budget admission, eight drivers with batches of up
+           to two): p50/p99 latency and
            Mpix/s, against the same jobs processed one at a time.
 
 Every timed job's output SHA-256 and counters are compared with the compiled
@@ -100,7 +101,7 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
     parser.add_argument("--device-chain", action="store_true",
                         help="batched legs run each job's writer chain on one device wavefront")
-    parser.add_argument("--drivers", type=int, default=2,
+    parser.add_argument("--drivers", type=int, default=3,
                         help="concurrent batches in the pipelined leg")
     parser.add_argument("--pipeline-batch", type=int, default=None,
                         help="largest batch the pipelined leg's engine forms (default: --batch)")
