@@ -39,7 +39,7 @@ class _Pending:
 class HipServingEngine:
     """Submit / wait front end of the serving engine."""
 
-    def __init__(self, *, device: int = 0, drivers: int = 2, max_batch: int = 8,
+    def __init__(self, *, device: int = 0, drivers: int = 3, max_batch: int = 8,
                  hbm_budget: int = 0, graph: bool = True, autotune: bool = False,
                  linger_ms: float = 2.0, device_chain: bool = False) -> None:
         self._lib = library()

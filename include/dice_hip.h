@@ -313,7 +313,7 @@ typedef struct dice_engine dice_engine;
 
 typedef struct dice_engine_config {
   int32_t device;
-  int32_t drivers;          /* concurrent batches, 1..16 (0: 2) */
+  int32_t drivers;          /* concurrent batches, 1..16 (0: 3) */
   int32_t max_batch;        /* jobs per batch, 1..64 (0: 8) */
   uint32_t batch_flags;     /* DICE_BATCH_GRAPH | DICE_BATCH_AUTOTUNE */
   uint64_t hbm_budget;      /* HBM the drivers' job workspaces may hold (0: 80% of free HBM);
