@@ -1008,6 +1008,89 @@ constexpr int sites_wide_threads() {
   return sites_occupancy<TY, TX, 512>() == 1 ? 1024 : 512;
 }
 
+// The reconstruction sites of a tile with at most Threads / 3 of them, three
+// lanes per site: lane k takes channel k.  The candidates
+// (reconstruction.py:425-462) and the combiner (:672-887) are per channel
+// once the site's weights and feature records are known, so each lane runs
+// a third of the site's chain of binary64 operations and stores its
+// channel's fields of the site.  Lanes [0, total) take channel 0 of the
+// listed sites in order, the next `total` channel 1, the last channel 2, so
+// a wave reads one plane of the weighted RGB at a time.  Eligibility
+// already implies the driver gate passes (reconstruction.py:890-904 re-tests
+// the same row gate and the same point score), so every listed centre is
+// attempted.  The same records and averages as the one-lane loop of
+// k_sites, whose comments say where each comes from.
+template <int TY, int TX, int Threads>
+__device__ __forceinline__ void sites_channel(SitesShared<TY, TX, Threads>& S, const JobDev& J,
+                                              const uint16_t* list, int y0, int x0,
+                                              uint32_t total, uint32_t u) {
+  using Sh = SitesShared<TY, TX, Threads>;
+  constexpr int PC = Sh::PC;
+  const DevParams& P = J.P;
+  const int W = J.width;
+  const uint64_t* rgbi = reinterpret_cast<const uint64_t*>(J.rgbi);
+  const float* lut = J.lut;
+  const uint32_t k = u / total, q = u - k * total;
+  const int li = list[q];
+  const int ty = li / TX, tx = li % TX;
+  const int y = y0 + ty, x = x0 + tx;
+  const float fallback = S.writer_coarse[ty];
+  const uint64_t pixel = rgbi[(size_t)y * W + x];  // R, G, B, IR (16 bits each)
+  const float original = lut[(uint32_t)(pixel >> (16 * k)) & 0xFFFFu];
+  const int cr = ty + 5, cc = tx + 5, ar = ty + 5, ac = tx + 6;
+  const int centre = (ty + 1) * PC + tx + 1;
+  auto feature = [&](int i, float point_aux, float point_score, float f[4]) {
+    const uint8_t valid = S.record_valid[i];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) f[l] = (valid >> l) & 1u ? S.record_ratio[l][i] : fallback;
+    f[3] = point_score > 0.0f ? point_aux : fallback;
+  };
+  float weights[4], features[5][4];
+  weights[0] = S.record_w[0][centre];
+  weights[1] = S.record_w[1][centre];
+  weights[2] = S.record_w[2][centre];
+  weights[3] = S.score[cr][cc];
+  feature(centre, S.aux[ar][ac], S.score[cr][cc], features[0]);
+  int record_count = 1;
+  if (P.cross_neighbor) {
+    record_count = 5;
+    if (x > 0) {
+      feature(centre - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], features[1]);
+    } else {
+      features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+    }
+    if (x < W - 1) {
+      feature(centre + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], features[2]);
+    } else {
+      features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
+    }
+    feature(centre - PC, S.aux[ar - 1][ac], S.score[cr - 1][cc], features[3]);
+    feature(centre + PC, S.aux[ar + 1][ac], S.score[cr + 1][cc], features[4]);
+  }
+  auto lane = [&](int dy, int dx) { return S.wrgb[k][ty + dy][tx + dx]; };
+  double averages[3];
+  unscaled_averages(lane, averages);
+  averages[2] = rgb_q16(lane);
+  float candidates[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (weights[c] > 0.0f) {
+      double reciprocal = 1.0 / (double)weights[c];
+      candidates[c] = (float)(averages[c] * reciprocal);
+    } else {
+      candidates[c] = (float)averages[c];
+    }
+  }
+  CombineShared shared;
+  combine_shared(weights, features, record_count, fallback, P.combine, shared);
+  Site* site = J.sites + S.row_offset[ty] + (q - S.row_start[ty]);
+  site->cand[k] = combine_channel((int)k, candidates[0], candidates[1], candidates[2], original,
+                                  shared, P.combine);
+  site->orig[k] = original;
+  if (k == 0)
+    site->pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
+}
+
 // Tiles per work unit of k_sites: a workgroup takes a vertical strip of
 // this many tiles, and each tile after the first keeps the bottom rows of
 // the halo the tile above staged (10 of the 26 aux and history rows, 8 of
@@ -1037,6 +1120,9 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
   constexpr int AR = Sh::AR, AC = Sh::AC, HR = Sh::HR, HC = Sh::HC;
   constexpr int RR = Sh::RR, RC = Sh::RC, PC = Sh::PC, NP = Sh::NP;
   constexpr int kWords = (NP + 31) / 32;
+  // tiles of at most kChannelRounds * Threads / 3 sites take three lanes
+  // per site
+  constexpr uint32_t kChannelRounds = 1;
   __shared__ Sh S;
   // One tile with sites; `staged`: rows of its halo are those the tile
   // above left in LDS.
@@ -1381,65 +1467,76 @@ __global__ __launch_bounds__(Threads, Waves) void k_sites(JobSet set) {
     // next tile's marks (a barrier follows every tile).
     for (int i = tid; i < kWords; i += Threads) S.needed_bits[i] = 0u;
 
-    // One site per thread.  Eligibility already implies the driver gate
-    // passes (reconstruction.py:890-904 re-tests the same row gate and the
-    // same point score), so every listed centre is attempted.
-    for (uint32_t q = tid; q < total; q += Threads) {
-      const int li = list[q];
-      const int ty = li / TX, tx = li % TX;
-      const int y = y0 + ty, x = x0 + tx;
-      const int cr = ty + 5, cc = tx + 5;  // history index of the centre
-      const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
-      const int centre = (ty + 1) * PC + tx + 1;
-      const float fallback = S.writer_coarse[ty];
-      // the site's own working RGB, loaded first so that its latency
-      // overlaps the averaging below
-      uint32_t lanes[4];
-      load_pixel(J.rgbi, (size_t)y * W + x, lanes);
-      const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
-      // record at table index i with point values (aux, score) of its pixel
-      auto feature = [&](int i, float point_aux, float point_score, float f[4]) {
-        const uint8_t valid = S.record_valid[i];
+    if (3 * total <= (uint32_t)(kChannelRounds * Threads)) {
+      // Few sites: three lanes per site, lane k taking channel k
+      // (sites_channel), in at most kChannelRounds rounds of the workgroup.
+      // The lane index made opaque here, so that nothing of this path is
+      // computed ahead of the branch and held through the other one.
+      int lane_id = tid;
+      asm volatile("" : "+v"(lane_id));
+      for (uint32_t u = (uint32_t)lane_id; u < 3 * total; u += Threads)
+        sites_channel<TY, TX, Threads>(S, J, list, y0, x0, total, u);
+    } else {
+      // One site per thread.  Eligibility already implies the driver gate
+      // passes (reconstruction.py:890-904 re-tests the same row gate and the
+      // same point score), so every listed centre is attempted.
+      for (uint32_t q = tid; q < total; q += Threads) {
+        const int li = list[q];
+        const int ty = li / TX, tx = li % TX;
+        const int y = y0 + ty, x = x0 + tx;
+        const int cr = ty + 5, cc = tx + 5;  // history index of the centre
+        const int ar = ty + 5, ac = tx + 6;  // aux index of the centre
+        const int centre = (ty + 1) * PC + tx + 1;
+        const float fallback = S.writer_coarse[ty];
+        // the site's own working RGB, loaded first so that its latency
+        // overlaps the averaging below
+        uint32_t lanes[4];
+        load_pixel(J.rgbi, (size_t)y * W + x, lanes);
+        const float original[3] = {J.lut[lanes[0]], J.lut[lanes[1]], J.lut[lanes[2]]};
+        // record at table index i with point values (aux, score) of its pixel
+        auto feature = [&](int i, float point_aux, float point_score, float f[4]) {
+          const uint8_t valid = S.record_valid[i];
 #pragma unroll
-        for (int l = 0; l < 3; ++l) f[l] = (valid >> l) & 1u ? S.record_ratio[l][i] : fallback;
-        f[3] = point_score > 0.0f ? point_aux : fallback;
-      };
-      float weights[4], features[5][4];
-      weights[0] = S.record_w[0][centre];
-      weights[1] = S.record_w[1][centre];
-      weights[2] = S.record_w[2][centre];
-      weights[3] = S.score[cr][cc];
-      feature(centre, S.aux[ar][ac], S.score[cr][cc], features[0]);
-      int record_count = 1;
-      if (P.cross_neighbor) {
-        // streaming.py:259-308: neighbours share the centre row's fallback;
-        // out-of-row neighbours are all-zero records; rows -1 and H take the
-        // pseudo-row / repeated-row point values already in the halo.
-        record_count = 5;
-        if (x > 0) {
-          feature(centre - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], features[1]);
-        } else {
-          features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+          for (int l = 0; l < 3; ++l) f[l] = (valid >> l) & 1u ? S.record_ratio[l][i] : fallback;
+          f[3] = point_score > 0.0f ? point_aux : fallback;
+        };
+        float weights[4], features[5][4];
+        weights[0] = S.record_w[0][centre];
+        weights[1] = S.record_w[1][centre];
+        weights[2] = S.record_w[2][centre];
+        weights[3] = S.score[cr][cc];
+        feature(centre, S.aux[ar][ac], S.score[cr][cc], features[0]);
+        int record_count = 1;
+        if (P.cross_neighbor) {
+          // streaming.py:259-308: neighbours share the centre row's fallback;
+          // out-of-row neighbours are all-zero records; rows -1 and H take the
+          // pseudo-row / repeated-row point values already in the halo.
+          record_count = 5;
+          if (x > 0) {
+            feature(centre - 1, S.aux[ar][ac - 1], S.score[cr][cc - 1], features[1]);
+          } else {
+            features[1][0] = features[1][1] = features[1][2] = features[1][3] = 0.0f;
+          }
+          if (x < W - 1) {
+            feature(centre + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], features[2]);
+          } else {
+            features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
+          }
+          feature(centre - PC, S.aux[ar - 1][ac], S.score[cr - 1][cc], features[3]);
+          feature(centre + PC, S.aux[ar + 1][ac], S.score[cr + 1][cc], features[4]);
         }
-        if (x < W - 1) {
-          feature(centre + 1, S.aux[ar][ac + 1], S.score[cr][cc + 1], features[2]);
-        } else {
-          features[2][0] = features[2][1] = features[2][2] = features[2][3] = 0.0f;
-        }
-        feature(centre - PC, S.aux[ar - 1][ac], S.score[cr - 1][cc], features[3]);
-        feature(centre + PC, S.aux[ar + 1][ac], S.score[cr + 1][cc], features[4]);
+        auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
+        float candidates[3][3];
+        rgb_candidates(wrgb_at, weights, candidates);
+        Site site;
+        combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
+                          features, record_count, fallback, P.combine, site.cand);
+        site.orig[0] = original[0];
+        site.orig[1] = original[1];
+        site.orig[2] = original[2];
+        site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
+        J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
       }
-      auto wrgb_at = [&](int dy, int dx, int k) { return S.wrgb[k][ty + dy][tx + dx]; };
-      float candidates[3][3];
-      rgb_candidates(wrgb_at, weights, candidates);
-      Site site;
-      combine_candidate(candidates[0], candidates[1], candidates[2], original, weights,
-                        features, record_count, fallback, P.combine, site.cand);
-      site.orig[0] = original[0];
-      site.orig[1] = original[1];
-      site.orig[2] = original[2];
-      site.pixel = (uint32_t)((size_t)y * W + x) | (S.writer_floor[ty] ? kSiteFloorBit : 0u);
-      J.sites[S.row_offset[ty] + (q - S.row_start[ty])] = site;
     }
   };
   constexpr int kStrip = sites_strip(TY);

@@ -174,27 +174,34 @@ struct CombineParams {
 };
 
 // reconstruction.py:672-887 (feature_band_ranges, _automatic_strengths,
-// combine_recovered_candidate with asymmetric_dead_zone_residual).
-DICE_HD void combine_candidate(const float c69[3], const float c21[3],
-                               const float c16[3], const float source[3],
-                               const float weights[4],
-                               const float features[5][4], int record_count,
-                               float coarse_reference, const CombineParams& P,
-                               double candidate[3]) {
-  double range_min[3], range_max[3];
+// combine_recovered_candidate with asymmetric_dead_zone_residual), in three
+// parts: combine_shared, combine_channel and combine_candidate over them.
+//
+// The parts shared by the three channels: the feature band ranges and the
+// band strengths.
+struct CombineShared {
+  double range_min[3], range_max[3], strengths[3];
+  double coarse_delta;
+};
+
+DICE_HD void combine_shared(const float weights[4], const float features[5][4], int record_count,
+                            float coarse_reference, const CombineParams& P, CombineShared& out) {
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
     double d0 = (double)features[0][t + 1] - (double)features[0][t];
     double minimum = d0, maximum = d0;
-    for (int r = 1; r < record_count; ++r) {
+    // (unrolled over the at most five records, so the records stay in
+    // registers)
+#pragma unroll
+    for (int r = 1; r < 5; ++r) {
+      if (r >= record_count) break;
       double d = (double)features[r][t + 1] - (double)features[r][t];
       if (d < minimum) minimum = d;
       if (d > maximum) maximum = d;
     }
-    range_min[t] = (double)(float)minimum;
-    range_max[t] = (double)(float)maximum;
+    out.range_min[t] = (double)(float)minimum;
+    out.range_max[t] = (double)(float)maximum;
   }
-  double strengths[3];
   if (P.configured_strengths[0] == 0.0f) {
     // min(1, max(0, x)) with the literal first: NaN collapses to 0.
     double doubled = (double)(float)(2.0 * (double)weights[1]);
@@ -202,59 +209,71 @@ DICE_HD void combine_candidate(const float c69[3], const float c21[3],
     if (doubled > 0.0) lower_clamped = doubled;
     double clamped = 1.0;
     if (lower_clamped < 1.0) clamped = lower_clamped;
-    strengths[0] = (double)(float)clamped;
+    out.strengths[0] = (double)(float)clamped;
   } else {
-    strengths[0] = (double)P.configured_strengths[0];
+    out.strengths[0] = (double)P.configured_strengths[0];
   }
   if (P.configured_strengths[1] == 0.0f) {
     float m2 = weights[2];
     if (m2 < 0.0f) m2 = 0.0f;
-    strengths[1] = (double)m2;
+    out.strengths[1] = (double)m2;
   } else {
-    strengths[1] = (double)P.configured_strengths[1];
+    out.strengths[1] = (double)P.configured_strengths[1];
   }
   if (P.configured_strengths[2] == 0.0f)
-    strengths[2] = (double)weights[3] * (double)weights[3];
+    out.strengths[2] = (double)weights[3] * (double)weights[3];
   else
-    strengths[2] = (double)P.configured_strengths[2];
+    out.strengths[2] = (double)P.configured_strengths[2];
+  out.coarse_delta = (double)coarse_reference - (double)features[0][0];
+}
 
-#pragma unroll
-  for (int c = 0; c < 3; ++c) candidate[c] = (double)c69[c];
-  if (P.coarse_enabled) {
-    double coarse_delta = (double)coarse_reference - (double)features[0][0];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      candidate[c] = candidate[c] + (double)P.coarse_slopes[c] * coarse_delta;
-  }
+// Channel c of the combined candidate: the C69 candidate, the coarse slope
+// term, then the dead-zone residual of each enabled band (coarse -> fine:
+// C69 -> C21, C21 -> C16, C16 -> the original), in that order.
+DICE_HD double combine_channel(int c, float c69, float c21, float c16, float source,
+                               const CombineShared& S, const CombineParams& P) {
+  double candidate = (double)c69;
+  if (P.coarse_enabled) candidate = candidate + (double)P.coarse_slopes[c] * S.coarse_delta;
 #pragma unroll
   for (int band = 0; band < 3; ++band) {
     if (!P.band_enabled[band]) continue;
     double scale = (double)P.band_scales[band];
-    double edge_min = range_min[band], edge_max = range_max[band];
+    double edge_min = S.range_min[band], edge_max = S.range_max[band];
     bool negative_band = edge_min < 0.0 && edge_max < 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float* coarse = band == 0 ? c69 : (band == 1 ? c21 : c16);
-      const float* fine = band == 0 ? c21 : (band == 1 ? c16 : source);
-      double difference = scale * ((double)fine[c] - (double)coarse[c]);
-      double upper, lower;
-      if (negative_band) {
-        upper = (double)P.factors_b[band * 3 + c] * edge_max;
-        lower = (double)P.factors_a[band * 3 + c] * edge_min;
-      } else {
-        upper = (double)P.factors_a[band * 3 + c] * edge_max;
-        lower = (double)P.factors_b[band * 3 + c] * edge_min;
-      }
-      double residual;
-      if (difference > upper)
-        residual = difference - upper;
-      else if (difference < lower)
-        residual = difference - lower;
-      else
-        residual = 0.0;
-      candidate[c] = candidate[c] + strengths[band] * residual;
+    const float coarse = band == 0 ? c69 : (band == 1 ? c21 : c16);
+    const float fine = band == 0 ? c21 : (band == 1 ? c16 : source);
+    double difference = scale * ((double)fine - (double)coarse);
+    double upper, lower;
+    if (negative_band) {
+      upper = (double)P.factors_b[band * 3 + c] * edge_max;
+      lower = (double)P.factors_a[band * 3 + c] * edge_min;
+    } else {
+      upper = (double)P.factors_a[band * 3 + c] * edge_max;
+      lower = (double)P.factors_b[band * 3 + c] * edge_min;
     }
+    double residual;
+    if (difference > upper)
+      residual = difference - upper;
+    else if (difference < lower)
+      residual = difference - lower;
+    else
+      residual = 0.0;
+    candidate = candidate + S.strengths[band] * residual;
   }
+  return candidate;
+}
+
+DICE_HD void combine_candidate(const float c69[3], const float c21[3],
+                               const float c16[3], const float source[3],
+                               const float weights[4],
+                               const float features[5][4], int record_count,
+                               float coarse_reference, const CombineParams& P,
+                               double candidate[3]) {
+  CombineShared shared;
+  combine_shared(weights, features, record_count, coarse_reference, P, shared);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    candidate[c] = combine_channel(c, c69[c], c21[c], c16[c], source[c], shared, P);
 }
 
 // dither.py:59-104 with the state threaded explicitly (rng.py:159-172).
