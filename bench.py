@@ -88,7 +88,7 @@ SITE_FP64_OPS = 13 * (9 * 12 + 17) + 3 * 13 + 15 + 9 + 50
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (two --pmc runs over the plain leg, one
 # frame per dispatch), each scaled by what it reads for the plain copy kernel
 # k_copy over 1 GiB (x2.000 and x1.000)
-SITES_TRAFFIC_BYTES = 361_770_000
+SITES_TRAFFIC_BYTES = 364_240_000
 TRAFFIC_SOURCE = ("profiles/kernel_stats_full_frame_5782x3946.md, k_sites<16, 64, 512, 4> row: "
                   "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per dispatch (not measured in this run)")
 
