@@ -29,6 +29,7 @@
 #include "dice_kernels.hpp"
 #include "dice_runtime.hpp"
 #include "dice_routing.hpp"
+#include "dice_sha.hpp"
 #include "dice_stream.hpp"
 
 #pragma clang fp contract(off)
@@ -889,6 +890,49 @@ int32_t dice_t_writer_chain_host(const double* cand, const float* orig, const ui
   *state_out = state;
   *advances_out = draws;
   return DICE_OK;
+}
+
+int32_t dice_t_sha256_pieces(const uint8_t* const* messages, const int64_t* lengths,
+                             int32_t count, int64_t piece, int32_t mode, char* digests,
+                             int32_t* extensions) {
+  if (!messages || !lengths || !digests || count < 1 || count > 64 || piece < 1 || mode < 0 ||
+      mode > 1)
+    return set_error(DICE_INVALID, "dice_t_sha256_pieces: bad arguments");
+  for (int i = 0; i < count; ++i)
+    if (lengths[i] < 0 || (lengths[i] > 0 && !messages[i]))
+      return set_error(DICE_INVALID, "dice_t_sha256_pieces: bad message %d", i);
+  if (extensions) *extensions = dice::sha_extensions() ? 1 : 0;
+  // message i is cut into pieces of piece * (1 + i % 3) + i bytes, so the
+  // messages of one interleaved pass end their blocks at different places
+  auto piece_of = [&](int i) { return (size_t)piece * (1 + i % 3) + (size_t)i; };
+  std::vector<size_t> at(count, 0);
+  std::vector<std::unique_ptr<dice::Sha256Stream>> sha(count);
+  for (int i = 0; i < count; ++i) sha[i] = std::make_unique<dice::Sha256Stream>();
+  for (int g = 0; g < count; g += 4) {
+    const int members = std::min(4, count - g);
+    for (bool more = true; more;) {
+      more = false;
+      dice::Sha256Stream* streams[4];
+      const uint8_t* data[4];
+      size_t bytes[4];
+      for (int k = 0; k < members; ++k) {
+        const int i = g + k;
+        streams[k] = sha[i].get();
+        bytes[k] = std::min(piece_of(i), (size_t)lengths[i] - at[i]);
+        data[k] = bytes[k] ? messages[i] + at[i] : nullptr;
+        at[i] += bytes[k];
+        more = more || bytes[k] > 0;
+      }
+      if (mode == 0)
+        dice::Sha256Stream::update_interleaved(streams, data, bytes, members);
+      else
+        for (int k = 0; k < members; ++k)
+          if (bytes[k]) streams[k]->update(data[k], bytes[k]);
+    }
+  }
+  bool ok = true;
+  for (int i = 0; i < count; ++i) ok = sha[i]->hex(digests + 72 * i) && ok;
+  return ok ? DICE_OK : set_error(DICE_INTERNAL, "SHA-256 failed");
 }
 
 int32_t dice_t_writer_site_host(const double* cand, const float* orig, uint8_t floor, float low,

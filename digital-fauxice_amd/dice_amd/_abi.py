@@ -417,6 +417,8 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "stream_destroy": (None, [VOIDP]),
     "t_writer_site_host": (C.c_int32, [f64p, f32p, C.c_uint8, C.c_float, C.c_float, f32p, u32p,
                                        i64, C.c_int32, u32p, u8p]),
+    "t_sha256_pieces": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.c_int32,
+                                    i64, C.c_int32, C.c_char_p, C.POINTER(C.c_int32)]),
 }
 
 

@@ -487,6 +487,18 @@ int32_t dice_t_writer_site_host(const double* cand, const float* orig,
                                 int64_t count, int32_t generic,
                                 uint32_t* states_out, uint8_t* advances);
 
+/* SHA-256 of `count` (1..64) messages fed in pieces, as the batch runtime
+ * hashes the replay outputs band by band (the reference: one
+ * hashlib.sha256() per replay updated row by row, streaming.py:425, :550).
+ * Message i is cut into pieces of piece*(1 + i%3) + i bytes.  mode 0: groups
+ * of four messages, each round of pieces compressed interleaved; 1: each
+ * piece on its own.  Writes each digest as 64 lowercase hex digits plus a
+ * NUL at digests + 72*i, and whether the x86 SHA extensions were used. */
+int32_t dice_t_sha256_pieces(const uint8_t* const* messages,
+                             const int64_t* lengths, int32_t count,
+                             int64_t piece, int32_t mode, char* digests,
+                             int32_t* extensions);
+
 /* ---- At-floor routing ------------------------------------------------------
  * route_at_floor_mask (hybrid/src/fauxce_hybrid/routing.py:447-687) on the
  * device, for the hybrid step after the core:

@@ -89,6 +89,33 @@ def test_batch_matches_oracle_and_single_runs(frames, oracle_fingerprints):
         assert np.array_equal(alone.output_rgb16, outcome.results[i].output_rgb16)
 
 
+def test_same_geometry_jobs_hashed_in_pairs_match_oracle():
+    """Jobs of one geometry in a batch are hashed two by two, the two
+    messages interleaved on one thread; a third job of that geometry hashes
+    alone.  Every job has its own content, so a digest handed to the wrong
+    member of a pair fails; the host outputs are compared as well."""
+
+    params = stable_profile_params()
+    prepass_params = DEFAULT_PROFILE.prepass_parameters()
+    shapes = [(97, 211)] * 3 + [(256, 256)] * 2
+    frames = []
+    for i, (h, w) in enumerate(shapes):
+        main = synth.frame(synth.scaled_spec(h, w, seed=910 + i))
+        frames.append((main, synth.prepass_for(main)))
+    jobs = [make_job(main, prepass, f"pair-{i}") for i, (main, prepass) in enumerate(frames)]
+    outcome = process_batch_hip(jobs)
+    outcome.raise_first()
+    shas = set()
+    for i, (main, prepass) in enumerate(frames):
+        cpu = oracle_lib.process_job(main, prepass, params, prepass_params)
+        result = outcome.results[i]
+        assert result.replay.output_sha256 == cpu.output_sha256, f"job {i} {main.shape}"
+        assert np.array_equal(result.output_rgb16, cpu.output), f"job {i}"
+        assert result.replay.final_rng_state == cpu.final_rng_state
+        shas.add(cpu.output_sha256)
+    assert len(shas) == len(frames)
+
+
 def test_batch_graph_replays_match_eager(frames, oracle_fingerprints):
     jobs = _jobs(frames)
     eager = process_batch_hip(jobs)
