@@ -935,6 +935,14 @@ int32_t dice_t_sha256_pieces(const uint8_t* const* messages, const int64_t* leng
   return ok ? DICE_OK : set_error(DICE_INTERNAL, "SHA-256 failed");
 }
 
+int32_t dice_t_hash_pairs(int32_t mode, int32_t* previous) {
+  if (mode < dice::kHashPairsNever || mode > dice::kHashPairsWhenBusy)
+    return set_error(DICE_INVALID, "dice_t_hash_pairs: mode is 0, 1 or 2");
+  const int old = dice::set_hash_pair_mode(mode);
+  if (previous) *previous = old;
+  return DICE_OK;
+}
+
 int32_t dice_t_writer_site_host(const double* cand, const float* orig, uint8_t floor, float low,
                                 float high, const float* scales, const uint32_t* states_in,
                                 int64_t count, int32_t generic, uint32_t* states_out,

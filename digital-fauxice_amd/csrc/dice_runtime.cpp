@@ -53,19 +53,44 @@ double trace_origin() {
   return origin;
 }
 
-// The jobs of a batch that hash are taken two by two, each pair's outputs
-// hashed by one thread, the two messages interleaved: about 40 ms of a core
-// per full frame instead of 56-62 for one message on its own, at a frame's
-// hash latency of about 80 ms instead of 60.  DICE_HASH_PAIRS=0 gives every
-// job its own hash thread (OpenSSL) instead.
-bool hash_pairs_enabled() {
-  static const bool on = [] {
+// The jobs of a batch that hash may be taken two by two, each pair's
+// outputs hashed by one thread, the two messages interleaved: about 40 ms of
+// a core per full frame instead of 56-62 for one message on its own, at a
+// frame's hash latency of about 80 ms instead of 60.  That pays only when
+// the hashes outnumber the host's cores, so a batch pairs its jobs when the
+// messages being hashed in the process (every runner's), with its own,
+// would: a batch alone on an idle host keeps the shorter latency.
+// DICE_HASH_PAIRS=0 never pairs, =1 always does (set_hash_pair_mode
+// changes it at run time).
+std::atomic<int> g_messages_hashing{0};
+std::atomic<int> g_pair_mode{-1};  // -1: not read from the environment yet
+
+int hash_pair_mode() {
+  int mode = g_pair_mode.load();
+  if (mode < 0) {
     const char* v = getenv("DICE_HASH_PAIRS");
-    return !(v && v[0] == '0');
-  }();
-  return on;
+    int expected = -1;
+    g_pair_mode.compare_exchange_strong(
+        expected, v && v[0] == '0' ? kHashPairsNever : (v && v[0] == '1' ? kHashPairsAlways
+                                                                          : kHashPairsWhenBusy));
+    mode = g_pair_mode.load();
+  }
+  return mode;
+}
+
+bool hash_in_pairs(int messages) {
+  static const int cores = std::max(1u, std::thread::hardware_concurrency());
+  const int mode = hash_pair_mode();
+  if (mode != kHashPairsWhenBusy) return mode == kHashPairsAlways;
+  return g_messages_hashing.load() + messages > cores;
 }
 }  // namespace
+
+int set_hash_pair_mode(int mode) {
+  const int previous = hash_pair_mode();
+  g_pair_mode.store(mode);
+  return previous;
+}
 
 DevParams device_params(const dice_params& p) {
   DevParams P;
@@ -858,12 +883,15 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
     live[j].store((options.device_chain ? 0 : 1) + (hashes ? 1 : 0));
     early[j] = options.on_job_done && !jobs[j].cb.cb && live[j].load() > 0;
   }
-  // Jobs whose outputs are hashed in pairs (hash_pairs_enabled): a pair
+  // Jobs whose outputs are hashed in pairs (hash_in_pairs): a pair
   // thread, spawned after this loop, stands in for their hash threads.
   // Only jobs of the same geometry pair up: their bands arrive together and
   // they finish together, so a small job never waits for a large one.
   std::vector<int> paired;
-  if (hash_pairs_enabled()) {
+  int messages = 0;
+  for (int j = 0; j < n; ++j)
+    if (!H_jobs[j].skip && (jobs[j].flags & DICE_SKIP_SHA256) == 0) ++messages;
+  if (hash_in_pairs(messages)) {
     std::vector<char> taken(n, 0);
     for (int a = 0; a < n; ++a) {
       if (taken[a] || H_jobs[a].skip || (jobs[a].flags & DICE_SKIP_SHA256) != 0) continue;
@@ -879,6 +907,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   }
   std::vector<char> in_pair(n, 0);
   for (int j : paired) in_pair[j] = 1;
+  g_messages_hashing.fetch_add(messages);  // each hash thread takes its own off when done
   for (int j = 0; j < n; ++j) {
     if (H_jobs[j].skip) continue;
     Workspace& w = *ws_[j];
@@ -1027,6 +1056,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
         return DICE_OK;
       };
       if (int32_t rc = body()) outcome[j].fail(rc);
+      if (want_sha) g_messages_hashing.fetch_sub(1);
       hash_seconds[j] = hashing;
       copy_seconds[j] = copying;
       if (trace_enabled())
@@ -1059,6 +1089,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       auto finish = [&](Member& x) {
         if (x.done) return;
         x.done = true;
+        g_messages_hashing.fetch_sub(1);
         if (x.active && !x.sha.hex(jobs[x.j].result->output_sha256))
           drop(x, set_error(DICE_INTERNAL, "SHA-256 failed"));
         x.active = false;

@@ -5,12 +5,14 @@
 //     zero flags -> producer + setter -> analysis scan -> segment scan
 //   host: site counts, workspace sizing
 //   phase B (device): sites -> hidden startup -> writer draw schedules
-//   host, per job, two threads:
+//   host, per job:
 //     chain thread   the serial LCG chain over the draw schedules, in row
 //                    bands; after each band its start states go to the
 //                    device and the band's writer values are launched
 //     hash thread    waits for each band, downloads its output rows and
-//                    feeds SHA-256 (so the chain and the hash overlap)
+//                    feeds SHA-256 (so the chain and the hash overlap); on a
+//                    busy host, one thread per pair of jobs of one geometry,
+//                    the two messages compressed interleaved
 //
 // Phases A and B have static launch geometry (site counts are read on the
 // device), so a repeated batch replays them from captured hipGraphs.
@@ -31,6 +33,13 @@ namespace dice {
 
 DevParams device_params(const dice_params& p);
 WriterConsts writer_consts(const dice_params& p);
+
+// When the jobs of one geometry in a batch are hashed in pairs: never,
+// always, or when the messages being hashed in the process would outnumber
+// the host's cores (the default; DICE_HASH_PAIRS=0 / =1 select the others).
+constexpr int kHashPairsNever = 0, kHashPairsAlways = 1, kHashPairsWhenBusy = 2;
+// Sets the mode for the batches that start from now on; returns the old one.
+int set_hash_pair_mode(int mode);
 
 // One job as the caller described it, plus its outcome.
 struct JobRequest {

@@ -419,6 +419,7 @@ HIP_SIGNATURES: dict[str, tuple[object, list[object]]] = {
                                        i64, C.c_int32, u32p, u8p]),
     "t_sha256_pieces": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.c_int32,
                                     i64, C.c_int32, C.c_char_p, C.POINTER(C.c_int32)]),
+    "t_hash_pairs": (C.c_int32, [C.c_int32, C.POINTER(C.c_int32)]),
 }
 
 

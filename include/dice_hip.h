@@ -499,6 +499,13 @@ int32_t dice_t_sha256_pieces(const uint8_t* const* messages,
                              int64_t piece, int32_t mode, char* digests,
                              int32_t* extensions);
 
+/* When the batch runtime hashes the jobs of one geometry in pairs, one
+ * thread per pair compressing the two outputs interleaved: 0 never, 1
+ * always, 2 when the messages being hashed in the process would outnumber
+ * the host's cores (the default).  Applies to batches that start afterwards;
+ * writes the previous mode.  Lets the tests reach both hash paths. */
+int32_t dice_t_hash_pairs(int32_t mode, int32_t* previous);
+
 /* ---- At-floor routing ------------------------------------------------------
  * route_at_floor_mask (hybrid/src/fauxce_hybrid/routing.py:447-687) on the
  * device, for the hybrid step after the core:

@@ -89,12 +89,15 @@ def test_batch_matches_oracle_and_single_runs(frames, oracle_fingerprints):
         assert np.array_equal(alone.output_rgb16, outcome.results[i].output_rgb16)
 
 
-def test_same_geometry_jobs_hashed_in_pairs_match_oracle():
-    """Jobs of one geometry in a batch are hashed two by two, the two
-    messages interleaved on one thread; a third job of that geometry hashes
-    alone.  Every job has its own content, so a digest handed to the wrong
+@pytest.mark.parametrize("mode", [1, 0], ids=["pairs", "one-per-job"])
+def test_same_geometry_jobs_hashed_in_pairs_match_oracle(mode):
+    """With pairs forced on, jobs of one geometry in a batch are hashed two
+    by two, the two messages interleaved on one thread, and a third job of
+    that geometry hashes alone; with them off, every job has its own hash
+    thread.  Every job has its own content, so a digest handed to the wrong
     member of a pair fails; the host outputs are compared as well."""
 
+    lib = engine.library()
     params = stable_profile_params()
     prepass_params = DEFAULT_PROFILE.prepass_parameters()
     shapes = [(97, 211)] * 3 + [(256, 256)] * 2
@@ -103,7 +106,12 @@ def test_same_geometry_jobs_hashed_in_pairs_match_oracle():
         main = synth.frame(synth.scaled_spec(h, w, seed=910 + i))
         frames.append((main, synth.prepass_for(main)))
     jobs = [make_job(main, prepass, f"pair-{i}") for i, (main, prepass) in enumerate(frames)]
-    outcome = process_batch_hip(jobs)
+    previous = C.c_int32(-1)
+    assert lib.dice_t_hash_pairs(mode, C.byref(previous)) == 0
+    try:
+        outcome = process_batch_hip(jobs)
+    finally:
+        assert lib.dice_t_hash_pairs(previous.value, None) == 0
     outcome.raise_first()
     shas = set()
     for i, (main, prepass) in enumerate(frames):
@@ -114,6 +122,12 @@ def test_same_geometry_jobs_hashed_in_pairs_match_oracle():
         assert result.replay.final_rng_state == cpu.final_rng_state
         shas.add(cpu.output_sha256)
     assert len(shas) == len(frames)
+
+
+def test_hash_pair_mode_rejects_unknown_modes():
+    lib = engine.library()
+    assert lib.dice_t_hash_pairs(3, None) != 0
+    assert lib.dice_t_hash_pairs(-1, None) != 0
 
 
 def test_batch_graph_replays_match_eager(frames, oracle_fingerprints):
