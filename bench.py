@@ -96,6 +96,8 @@ TRAFFIC_SOURCE = ("profiles/kernel_stats_full_frame_5782x3946.md, k_sites<16, 64
 def parse_args() -> argparse.Namespace:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--hw-queues", type=int, default=16,
+                        help="GPU_MAX_HW_QUEUES for this process (0: keep the environment's)")
     parser.add_argument("--steps", type=int, default=6, help="timed batches (and plain jobs)")
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
@@ -191,6 +193,12 @@ def fingerprint(res) -> tuple:
 
 def main() -> int:
     args = parse_args()
+    if args.hw_queues > 0:
+        # HIP maps every stream of the process onto GPU_MAX_HW_QUEUES hardware
+        # queues (4 by default).  Three runners drive nine streams; on four
+        # queues one runner's kernels wait behind another's copies.  HIP reads
+        # it when it starts, which is below.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
               "(jobs would shard across GPUs by job, one process per GPU, without "
@@ -862,6 +870,7 @@ def main() -> int:
             "frames_timed": jobs_total,
             "drivers": args.drivers,
             "pipeline_batch": pipe_batch,
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
             "writer_chain": "device wavefront" if args.device_chain else "host core",
             "global_batch": args.batch,
             "height": height,

@@ -13,6 +13,7 @@ device raises ``HipBackendUnavailable``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
@@ -68,6 +69,12 @@ def library() -> C.CDLL:
                 f"HIP library not built: {LIBRARY_PATH} is missing "
                 "(run __graft_entry__.build())"
             )
+        # HIP maps all of a process's streams onto GPU_MAX_HW_QUEUES hardware
+        # queues (4 by default); the batch runtime's runners drive three
+        # streams each, which four queues serialize.  Read when HIP starts, so
+        # this default only takes effect if nothing in the process has used
+        # HIP yet; an explicit setting wins.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         try:
             lib = C.CDLL(str(LIBRARY_PATH))
         except OSError as error:
