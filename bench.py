@@ -103,10 +103,10 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--batch", type=int, default=8, help="full frames per batch")
     parser.add_argument("--device-chain", action="store_true",
                         help="batched legs run each job's writer chain on one device wavefront")
-    parser.add_argument("--drivers", type=int, default=3,
+    parser.add_argument("--drivers", type=int, default=2,
                         help="concurrent batches in the pipelined leg")
-    parser.add_argument("--pipeline-batch", type=int, default=None,
-                        help="largest batch the pipelined leg's engine forms (default: --batch)")
+    parser.add_argument("--pipeline-batch", type=int, default=16,
+                        help="largest batch the pipelined leg's engine forms (0: --batch)")
     parser.add_argument("--pipeline-rounds", type=int, default=8,
                         help="the pipelined leg runs steps x batch x this many frames")
     parser.add_argument("--no-hashless", action="store_true",
