@@ -17,8 +17,9 @@ Legs:
            the batch, device phases replayed from captured hipGraphs, tile
            shape autotuned on the batch; W warm-up + K timed batches;
   pipelined  the headline `value`: K x B x rounds frames through the serving
-           engine (batches of B, `drivers` batches in flight, so the device
-           phases of one batch overlap the host work of the others);
+           engine (batches of up to --pipeline-batch, `drivers` batches in
+           flight, so the device phases of one batch overlap the host work
+           of the others);
   pipelined_without_sha256  the same jobs with the output SHA-256 skipped
            (outputs stay in HBM): what the device path and the host writer
            walk sustain without the hashing; informational only;
