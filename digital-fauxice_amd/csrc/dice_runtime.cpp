@@ -78,11 +78,17 @@ int hash_pair_mode() {
   return mode;
 }
 
-bool hash_in_pairs(int messages) {
+// The messages being hashed in the process, with `messages` more, as a
+// share of the host's cores (1: one per core).
+double hash_load(int messages) {
   static const int cores = std::max(1u, std::thread::hardware_concurrency());
+  return (double)(g_messages_hashing.load() + messages) / cores;
+}
+
+bool hash_in_pairs(int messages) {
   const int mode = hash_pair_mode();
   if (mode != kHashPairsWhenBusy) return mode == kHashPairsAlways;
-  return g_messages_hashing.load() + messages > cores;
+  return hash_load(messages) > 1.0;
 }
 }  // namespace
 
@@ -615,6 +621,18 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
   float ms_b = 0.0f;
   (void)hipEventElapsedTime(&ms_b, ev_[2], ev_[3]);
   int max_bands = 1;
+  // Bands of the host chain and hash: with at most one hash for every two
+  // cores, twice the default, so that each job's hash starts sooner after
+  // its chain (plain p50 -1.5 ms, one batch of 8 +7 %); a busier host keeps
+  // the default, which costs less per band (the serving engine under load
+  // lost 12 % of its capacity with the finer bands).
+  int band_count = options.bands;
+  if (!options.device_chain) {
+    int messages = 0;
+    for (int j = 0; j < n; ++j)
+      if (!H_jobs[j].skip && (jobs[j].flags & DICE_SKIP_SHA256) == 0) ++messages;
+    if (hash_load(messages) <= 0.5) band_count = 2 * options.bands;
+  }
   for (int j = 0; j < n; ++j) {
     JobDev& J = H_jobs[j];
     if (J.skip) continue;
@@ -626,7 +644,7 @@ int32_t Runner::run(JobRequest* jobs, int n, const RunOptions& options, RunRepor
       J.skip = 1;
       continue;
     }
-    w.bands = std::max(1, std::min<int>(std::min(options.bands, kMaxBands), J.height));
+    w.bands = std::max(1, std::min<int>(std::min(band_count, kMaxBands), J.height));
     max_bands = std::max(max_bands, w.bands);
     if (options.device_chain) continue;  // the schedules stay on the device
     const size_t su = 6 * (size_t)J.width;
