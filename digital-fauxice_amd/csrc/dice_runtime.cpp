@@ -250,6 +250,7 @@ int32_t Runner::release_workspaces() {
 // Sizes every job's workspace, places its input, stage table and output,
 // and uploads the descriptors.
 int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
+  std::vector<Upload> uploads;
   (void)any_host_work;
   while ((int)ws_.size() < n) ws_.push_back(std::make_unique<Workspace>());
   JobDev* H_jobs = h_jobs_.as<JobDev>();
@@ -367,7 +368,8 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
     } else {
       HIP_TRY(w.rgbi.ensure(pixels * 8, stream_));
       if (!R.streamed)
-        HIP_TRY(hipMemcpyAsync(w.rgbi.ptr, R.main, pixels * 8, hipMemcpyHostToDevice, stream_));
+        uploads.push_back({reinterpret_cast<const uint8_t*>(R.main), w.rgbi.as<uint8_t>(),
+                           pixels * 8, &w});
       J.rgbi = w.rgbi.as<uint16_t>();
     }
     if (R.flags & DICE_OUT_ON_DEVICE) {
@@ -395,9 +397,58 @@ int32_t Runner::prepare(JobRequest* jobs, int n, bool* any_host_work) {
                              stream_));
     }
   }
+  DICE_TRY(upload_frames(uploads));
   HIP_TRY(hipMemcpyAsync(d_jobs_.ptr, h_jobs_.ptr, sizeof(JobDev) * n, hipMemcpyHostToDevice,
                          stream_));
   return DICE_OK;
+}
+
+// Host frames go up through pinned staging: helper threads copy them in
+// 8 MB chunks into each job's output staging, which is free until the job's
+// outputs come down after phase B (stream order: these copies, then the
+// phases, then the band copies), while this thread queues each chunk's DMA
+// as soon as the chunk is staged.  A DMA straight from pageable memory is
+// staged by the runtime one piece at a time, well below the link's rate.
+int32_t Runner::upload_frames(const std::vector<Upload>& uploads) {
+  if (uploads.empty()) return DICE_OK;
+  struct Chunk {
+    const uint8_t* src;
+    uint8_t* staged;
+    uint8_t* dst;
+    size_t bytes;
+  };
+  constexpr size_t kChunk = 8u << 20;
+  std::vector<Chunk> chunks;
+  for (const Upload& u : uploads) {
+    HIP_TRY(u.w->h_out.ensure(u.bytes));
+    uint8_t* staging = u.w->h_out.as<uint8_t>();
+    for (size_t at = 0; at < u.bytes; at += kChunk)
+      chunks.push_back({u.src + at, staging + at, u.dst + at, std::min(kChunk, u.bytes - at)});
+  }
+  std::unique_ptr<std::atomic<int>[]> staged(new std::atomic<int>[chunks.size()]);
+  for (size_t i = 0; i < chunks.size(); ++i) staged[i].store(0);
+  std::atomic<size_t> next{0};
+  auto copier = [&]() {
+    for (size_t i; (i = next.fetch_add(1)) < chunks.size();) {
+      memcpy(chunks[i].staged, chunks[i].src, chunks[i].bytes);
+      staged[i].store(1, std::memory_order_release);
+    }
+  };
+  std::vector<std::thread> helpers;
+  const size_t count = std::min<size_t>(chunks.size(), 6);
+  for (size_t t = 0; t < count; ++t) helpers.emplace_back(copier);
+  int32_t rc = DICE_OK;
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    while (!staged[i].load(std::memory_order_acquire)) std::this_thread::yield();
+    if (rc != DICE_OK) continue;
+    const hipError_t e = hipMemcpyAsync(chunks[i].dst, chunks[i].staged, chunks[i].bytes,
+                                        hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess)
+      rc = set_error(DICE_INTERNAL, "hipMemcpyAsync (frame upload) failed: %s",
+                     hipGetErrorString(e));
+  }
+  for (std::thread& t : helpers) t.join();
+  return rc;
 }
 
 // Waits for everything queued on `stream` without spinning a core.

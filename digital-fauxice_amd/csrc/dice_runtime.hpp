@@ -148,6 +148,14 @@ class Runner {
     int seen = 0;
   };
   int32_t prepare(JobRequest* jobs, int n, bool* any_host_work);
+  // Host frames to device buffers through pinned staging (prepare).
+  struct Upload {
+    const uint8_t* src;
+    uint8_t* dst;
+    size_t bytes;
+    struct Workspace* w;
+  };
+  int32_t upload_frames(const std::vector<Upload>& uploads);
   int32_t phase_a(int n, bool graph, bool* replayed);
   int32_t phase_b(int n, bool graph, bool* replayed);
   int32_t replay_or_capture(Graph& g, const std::string& key, bool graph, bool* replayed,
