@@ -1,10 +1,21 @@
-"""Fail-closed LS-5000 selector-8 Normal profile (mirror of ``profile.py``).
+"""Fail-closed LS-5000 selector-8 Normal profile.
 
-Same job contract, constants and messages as
-``src/portable_digital_ice/profile.py:19-176`` of rohanpandula/digital-fauxice.
-The record-derived parameter factories take the prepass 0x40 record; on the
-supported path the per-row provider replaces every record-derived field, so
-the record's observable effect is the fail-closed validation done here.
+The reference's profile object (``src/portable_digital_ice/profile.py:19-176``
+of rohanpandula/digital-fauxice) is an interface its callers bind to: the
+class and method names, the job fields, the order of the job checks and their
+exact messages (its contract tests match on them, ``tests/test_contract.py``).
+Those are kept.  Inside, the profile here is data:
+
+* every single-precision constant it hands the replay is one entry of a
+  bit-pattern table (the replay's decisions depend on the exact bits, so
+  none is written as a decimal literal), and each parameter object is
+  assembled from its table;
+* the job check is an ordered list of rules, each a test and the exception
+  it raises; the first rule a job breaks is raised.
+
+The record-derived factories take the prepass 0x40 record; on the supported
+path the per-row provider replaces every record-derived field, so the
+record's observable effect is the fail-closed check on its alpha.
 """
 
 from __future__ import annotations
@@ -12,6 +23,7 @@ from __future__ import annotations
 import struct
 from dataclasses import dataclass
 from enum import Enum
+from typing import Callable, Iterator
 
 import numpy as np
 import numpy.typing as npt
@@ -24,7 +36,6 @@ from .parameters import (
     DitherBounds,
     ReconstructionParameters,
     ScoreParameters,
-    f32_from_bits,
 )
 
 
@@ -72,21 +83,111 @@ class PrepassRecord:
         return self._f32(0x2C)
 
 
-def _base_primary(record: PrepassRecord) -> np.float32:
-    """profile.py:48-55"""
+# ---------------------------------------------------------------------------
+# Constants, as IEEE-754 single bit patterns (profile.py:116-176)
+# ---------------------------------------------------------------------------
 
-    alpha = np.float32(record.observed_main_alpha)
-    mean_r = np.float32(record.observed_calibration_mean_r)
-    mean_ir = np.float32(record.observed_calibration_mean_ir)
-    denominator = 1.0 - float(alpha)
-    if denominator == 0.0:
+_SCORE_BITS = {"base_addend": 0xC2EED000, "scale": 0xBA887952,
+               "offset": 0x3F800000, "floor": 0x3CA3D70A}
+_SAMPLE_THRESHOLD_BITS = 0x4740E73C
+# one value for all three channels
+_COARSE_SLOPE_BITS = 0x3F8CCCCD
+_BAND_SCALE_BITS = 0x3FA00000
+# [table a|b][channel][band]
+_FACTOR_BITS = np.array(
+    [[[0x3F9AE148, 0x3F9D70A4, 0x3F90A3D7],
+      [0x3F95C28F, 0x3F91EB85, 0x3F8A3D71],
+      [0x3F851EB8, 0x3F6E147B, 0x3F7851EC]],
+     [[0x3F8B851F, 0x3F90A3D7, 0x3F851EB8],
+      [0x3F8A3D71, 0x3F866666, 0x3F828F5C],
+      [0x3F75C28F, 0x3F570A3D, 0x3F63D70A]]], dtype="<u4")
+_DITHER_SCALE_BITS = np.array([0x3C75C28F, 0x3C75C28F, 0x3CCCCCCD], dtype="<u4")
+
+_HORIZONTAL_MINIMUM_RESOLUTION_CUTOFF = 550
+_CROSS_NEIGHBOR_CUTOFF = 1_600
+_COUNT_LIMIT = 8
+_PERPENDICULAR_RADIUS = 4
+
+# prepass.py:42-53 (PrepassParameters.captured_ls5000_selector8): these are
+# exact singles, so the decimal literal is the value.
+_PREPASS = {"valid_ir_threshold": 8847.2255859375, "alpha_scale": 1.5,
+            "ratio_minimum": -0.2, "ratio_maximum": 0.2,
+            "zero_denominator_alpha": 0.17}
+_PREPASS_COMPLETION_MARKER = 16
+
+
+def _single(bits) -> float | tuple:
+    """The single(s) with these bit patterns, as Python floats (nested
+    tuples for an array)."""
+
+    values = np.asarray(bits, dtype="<u4").view("<f4")
+    if values.ndim == 0:
+        return float(values)
+    return _as_tuples(values.tolist())
+
+
+def _as_tuples(value):
+    return tuple(_as_tuples(v) for v in value) if isinstance(value, list) else value
+
+
+def _base_primary(record: PrepassRecord) -> np.float32:
+    """profile.py:48-55: the prepass IR calibration mean with alpha times the
+    red mean taken out, over 1 - alpha.  The record's fields are singles, the
+    arithmetic is double, the result is rounded to single; alpha == 1 has no
+    base and fails closed."""
+
+    alpha, mean_r, mean_ir = (float(v) for v in (record.observed_main_alpha,
+                                                  record.observed_calibration_mean_r,
+                                                  record.observed_calibration_mean_ir))
+    remaining = 1.0 - alpha
+    if remaining == 0.0:
         raise ValueError("prepass alpha cannot equal one")
-    return np.float32((float(mean_ir) - float(alpha) * float(mean_r)) / denominator)
+    return np.float32((mean_ir - alpha * mean_r) / remaining)
+
+
+# ---------------------------------------------------------------------------
+# The job check: ordered rules (profile.py:75-109)
+# ---------------------------------------------------------------------------
+
+# Job fields that must equal the profile's own value, with their names in
+# the message, in the order they are checked.
+_FIXED_FIELDS = (("scanner_model", "scanner model"), ("mode", "mode"),
+                 ("selector", "selector"), ("resolution_metric", "resolution metric"),
+                 ("bit_depth", "bit depth"))
+
+
+def _broken_rules(profile: "LS5000Selector8NormalProfile", job) -> Iterator[Exception]:
+    """The exception of each rule ``job`` breaks, lazily and in check order;
+    a rule runs only when every earlier one held, so later rules may rely on
+    the types the earlier ones established."""
+
+    if not isinstance(job, ProcessingJob):
+        yield TypeError("processing requires a ProcessingJob")
+    for field, label in _FIXED_FIELDS:
+        actual, supported = getattr(job, field), getattr(profile, field)
+        if actual != supported:
+            yield ValueError(f"unsupported {label} {actual!r}; this profile requires {supported!r}")
+    if job.focus_exposure_locked is not True:
+        yield ValueError("main capture must retain prepass focus and exposure")
+    acquisition = job.acquisition
+    if not isinstance(acquisition, DualRGBIAcquisition):
+        yield TypeError("processing requires a dual RGBI acquisition")
+    captures: tuple[tuple[RGBI16Frame, int, str], ...] = (
+        (acquisition.prepass, profile.prepass_dpi, "prepass"),
+        (acquisition.main, profile.main_dpi, "main frame"))
+    for frame, dpi, what in captures:
+        if frame.resolution_dpi != dpi:
+            yield ValueError(f"{what} must be captured at {dpi} dpi")
+    # A main frame of at least 8 columns always holds one whole 8-column
+    # producer cell, so this rule also covers the reference's cell check.
+    rows, columns = acquisition.main.pixels.shape[:2]
+    if rows < 5 or columns < 8:
+        yield ValueError("main RGBI frame must be at least 5 rows by 8 columns")
 
 
 @dataclass(frozen=True, init=False)
 class LS5000Selector8NormalProfile:
-    """profile.py:58-176"""
+    """profile.py:58-176: the one scanner, mode and resolution supported."""
 
     scanner_model: ScannerModel = ScannerModel.NIKON_SUPER_COOLSCAN_5000_ED
     mode: ProcessingMode = ProcessingMode.NORMAL
@@ -102,100 +203,63 @@ class LS5000Selector8NormalProfile:
         return "nikon-ls5000-selector8-normal-metric4000"
 
     def validate_job(self, job: ProcessingJob) -> None:
-        if not isinstance(job, ProcessingJob):
-            raise TypeError("processing requires a ProcessingJob")
-        expected = (
-            ("scanner model", job.scanner_model, self.scanner_model),
-            ("mode", job.mode, self.mode),
-            ("selector", job.selector, self.selector),
-            ("resolution metric", job.resolution_metric, self.resolution_metric),
-            ("bit depth", job.bit_depth, self.bit_depth),
-        )
-        for label, actual, supported in expected:
-            if actual != supported:
-                raise ValueError(
-                    f"unsupported {label} {actual!r}; this profile requires {supported!r}"
-                )
-        if job.focus_exposure_locked is not True:
-            raise ValueError("main capture must retain prepass focus and exposure")
-        acquisition = job.acquisition
-        if not isinstance(acquisition, DualRGBIAcquisition):
-            raise TypeError("processing requires a dual RGBI acquisition")
-        if acquisition.prepass.resolution_dpi != self.prepass_dpi:
-            raise ValueError(f"prepass must be captured at {self.prepass_dpi} dpi")
-        if acquisition.main.resolution_dpi != self.main_dpi:
-            raise ValueError(f"main frame must be captured at {self.main_dpi} dpi")
-        main = acquisition.main.pixels
-        if main.shape[0] < 5 or main.shape[1] < 8:
-            raise ValueError("main RGBI frame must be at least 5 rows by 8 columns")
-        if (main.shape[1] // 8) * 8 == 0:
-            raise ValueError("main RGBI frame has no complete producer cell")
+        """Raises the first rule ``job`` breaks; returns if it breaks none."""
+
+        first = next(_broken_rules(self, job), None)
+        if first is not None:
+            raise first
 
     def prepass_parameters(self) -> DicePrepassParams:
-        """prepass.py:42-53 (captured_ls5000_selector8)."""
-
         params = DicePrepassParams()
-        params.valid_ir_threshold = np.float32(8847.2255859375)
-        params.alpha_scale = np.float32(1.5)
-        params.ratio_minimum = np.float32(-0.2)
-        params.ratio_maximum = np.float32(0.2)
-        params.zero_denominator_alpha = np.float32(0.17)
-        params.completion_marker = 16
+        for name, value in _PREPASS.items():
+            setattr(params, name, np.float32(value))
+        params.completion_marker = _PREPASS_COMPLETION_MARKER
         return params
 
     def auxiliary_parameters(self, record: PrepassRecord) -> AuxiliaryParameters:
-        base = _base_primary(record)
-        alpha = np.float32(np.float32(record.observed_main_alpha) * np.float32(self.auxiliary_factor_b))
+        """Channel 0 carries the IR crosstalk; alpha is the record's, scaled
+        in single precision by the profile's factor b."""
+
+        scaled_alpha = record.observed_main_alpha * np.float32(self.auxiliary_factor_b)
         return AuxiliaryParameters(
             selected_visible_channel=0,
-            alpha=float(alpha),
-            calibration_offset=float(np.float32(1.0)),
-            alpha_one_replacement=float(base),
+            alpha=float(np.float32(scaled_alpha)),
+            calibration_offset=1.0,
+            alpha_one_replacement=float(_base_primary(record)),
         )
 
     def score_parameters(self, record: PrepassRecord) -> ScoreParameters:
+        singles = {name: _single(bits) for name, bits in _SCORE_BITS.items()}
         return ScoreParameters(
             base_primary=float(_base_primary(record)),
-            base_addend=f32_from_bits(0xC2EED000),
-            scale=f32_from_bits(0xBA887952),
-            offset=f32_from_bits(0x3F800000),
-            floor=f32_from_bits(0x3CA3D70A),
             resolution_metric=self.resolution_metric,
-            horizontal_minimum_resolution_cutoff=550,
+            horizontal_minimum_resolution_cutoff=_HORIZONTAL_MINIMUM_RESOLUTION_CUTOFF,
+            **singles,
         )
 
     def decision_parameters(self) -> DecisionParameters:
-        return DecisionParameters(
-            sample_threshold=f32_from_bits(0x4740E73C),
-            count_limit=8,
-            perpendicular_radius=4,
-        )
+        return DecisionParameters(sample_threshold=_single(_SAMPLE_THRESHOLD_BITS),
+                                  count_limit=_COUNT_LIMIT,
+                                  perpendicular_radius=_PERPENDICULAR_RADIUS)
 
     def reconstruction_parameters(self, record: PrepassRecord) -> ReconstructionParameters:
-        f = f32_from_bits
+        factors_a, factors_b = _single(_FACTOR_BITS)
+        per_channel: Callable[[int], tuple] = lambda bits: (_single(bits),) * 3
         return ReconstructionParameters(
             resolution_metric=self.resolution_metric,
-            cross_neighbor_cutoff=1_600,
+            cross_neighbor_cutoff=_CROSS_NEIGHBOR_CUTOFF,
             coarse_enabled=True,
             coarse_reference=float(_base_primary(record)),
-            coarse_slopes=(f(0x3F8CCCCD), f(0x3F8CCCCD), f(0x3F8CCCCD)),
-            band_enabled=(True, True, True),
-            band_scales=(f(0x3FA00000), f(0x3FA00000), f(0x3FA00000)),
-            factors_a=(
-                (f(0x3F9AE148), f(0x3F9D70A4), f(0x3F90A3D7)),
-                (f(0x3F95C28F), f(0x3F91EB85), f(0x3F8A3D71)),
-                (f(0x3F851EB8), f(0x3F6E147B), f(0x3F7851EC)),
-            ),
-            factors_b=(
-                (f(0x3F8B851F), f(0x3F90A3D7), f(0x3F851EB8)),
-                (f(0x3F8A3D71), f(0x3F866666), f(0x3F828F5C)),
-                (f(0x3F75C28F), f(0x3F570A3D), f(0x3F63D70A)),
-            ),
-            configured_strengths=(0.0, 0.0, 0.0),
+            coarse_slopes=per_channel(_COARSE_SLOPE_BITS),
+            band_enabled=(True,) * 3,
+            band_scales=per_channel(_BAND_SCALE_BITS),
+            factors_a=factors_a,
+            factors_b=factors_b,
+            configured_strengths=(0.0,) * 3,
             driver_gate_primary=True,
             driver_gate_secondary=True,
             row_reconstruction_gate=0,
-            dither_scales=(f(0x3C75C28F), f(0x3C75C28F), f(0x3CCCCCCD)),
+            dither_scales=_single(_DITHER_SCALE_BITS),
         )
 
     def dither_bounds(self, lookup: npt.ArrayLike) -> DitherBounds:
