@@ -74,6 +74,42 @@ def test_profile_rejects_frames_below_the_minimum(supported_job, shape, message)
         DEFAULT_PROFILE.validate_job(job)
 
 
+def test_first_broken_rule_in_check_order_is_raised(supported_job):
+    """profile.py:75-109 checks scanner model, mode, selector, resolution
+    metric, bit depth, focus lock, prepass dpi, main dpi, then frame size. A
+    job that breaks all of them reports them one at a time, in that order,
+    as each is mended."""
+
+    acquisition = supported_job.acquisition
+    small = np.full((4, 7, 4), 40000, np.uint16)
+
+    def captures(prepass_dpi, main_dpi, main_pixels):
+        return DualRGBIAcquisition(
+            RGBI16Frame(acquisition.prepass.pixels, AcquisitionEpoch.PREPASS, prepass_dpi, "p"),
+            RGBI16Frame(main_pixels, AcquisitionEpoch.MAIN, main_dpi, "m"),
+            acquisition.same_frame_id)
+
+    job = replace(supported_job, scanner_model="other-scanner", mode="fine", selector=7,
+                  resolution_metric=500, bit_depth=8, focus_exposure_locked=False,
+                  acquisition=captures(500, 2000, small))
+    mends = (
+        ("unsupported scanner model", dict(scanner_model=supported_job.scanner_model)),
+        ("unsupported mode", dict(mode=supported_job.mode)),
+        ("unsupported selector", dict(selector=8)),
+        ("unsupported resolution metric", dict(resolution_metric=4000)),
+        ("unsupported bit depth", dict(bit_depth=16)),
+        ("focus and exposure", dict(focus_exposure_locked=True)),
+        ("prepass must be captured at 285 dpi", dict(acquisition=captures(285, 2000, small))),
+        ("main frame must be captured at 4000 dpi", dict(acquisition=captures(285, 4000, small))),
+        ("at least 5 rows by 8 columns", dict(acquisition=acquisition)),
+    )
+    for message, mend in mends:
+        with pytest.raises(ValueError, match=message):
+            DEFAULT_PROFILE.validate_job(job)
+        job = replace(job, **mend)
+    DEFAULT_PROFILE.validate_job(job)
+
+
 def test_profile_requires_a_job_and_a_dual_acquisition(supported_job):
     with pytest.raises(TypeError, match="ProcessingJob"):
         DEFAULT_PROFILE.validate_job("not a job")
